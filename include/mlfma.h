@@ -1,0 +1,204 @@
+/*
+ * mlfma.h -- C ABI of the MI355X MLFMA solver engine (libmlfma.so).
+ *
+ * Plain C types only: pointers, sizes, doubles.  Complex vectors are
+ * interleaved (re, im) doubles.  A handle owns every device buffer of one
+ * operator (mesh + octree + translation/interpolation tables + radiation
+ * patterns + near-field blocks) and one HIP stream; a handle is not
+ * thread-safe, distinct handles are independent.
+ *
+ * Unknown ordering is the reference's: RWG functions numbered by their
+ * sorted (min vertex id, max vertex id) edge key (Code/RWGGenerator.cpp:39-113);
+ * PMCHWT vectors are x[0:N) = eta0*J and x[N:2N) = M (Code/MLFMM.cpp:238-245).
+ *
+ * Every entry point returns MLFMA_OK (0) or an error code; the message of the
+ * last failure is available from mlfma_last_error().
+ */
+#ifndef MLFMA_H
+#define MLFMA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLFMA_OK 0
+#define MLFMA_ERR_ARG 1          /* invalid argument (reference: std::invalid_argument) */
+#define MLFMA_ERR_HIP 2          /* HIP runtime failure */
+#define MLFMA_ERR_OOM 3          /* device or host allocation failed */
+#define MLFMA_ERR_NONFINITE 4    /* NaN/Inf met in a residual */
+#define MLFMA_ERR_NOT_CONVERGED 5
+#define MLFMA_ERR_MESH 6         /* unreadable / degenerate mesh */
+#define MLFMA_ERR_STATE 7        /* call not valid for this handle */
+
+typedef struct mlfma_handle mlfma_handle;
+
+/* Integral equation ids are the reference's selectIntegralEqu values
+ * (Code/RCSExportConfig.h:17). */
+#define MLFMA_EFIE 0
+#define MLFMA_CFIE 1
+#define MLFMA_PMCHWT 2
+
+/* Arithmetic of the far-field stages (aggregation, interpolation, translation,
+ * anterpolation, disaggregation).  Near field, translation-operator generation
+ * and Krylov steps are FP64 in both modes. */
+#define MLFMA_FP64 0
+#define MLFMA_FP32 1
+
+typedef struct mlfma_params {
+    int integral_equation;  /* MLFMA_EFIE / MLFMA_CFIE / MLFMA_PMCHWT */
+    int gauss_points;       /* Dunavant rule: 1,3,4,6,7,12,13,16 (Code/GaussPoints.cpp:286-301) */
+    double frequency_hz;    /* Code/EMSource.cpp:6-14 */
+    double eps_r_re, eps_r_im;   /* interior medium, eps' - j eps'' (Code/EMSource.cpp:19-32) */
+    double mu_r_re, mu_r_im;
+    double cfie_alpha;      /* reference compiles 0.2 (Code/PhyConstant.h:15) */
+    double leaf_coeff;      /* finest box = leaf_coeff * lambda0; reference 0.3 (Code/PhyConstant.h:14) */
+    int far_precision;      /* MLFMA_FP64 or MLFMA_FP32 */
+    int tf_full_order;      /* 0: reference series length min(L, ceil|kR|) (Code/AlgoMLFMM.h:263); 1: L */
+    int max_rhs;            /* largest batch a single apply/solve will be given */
+    int device;             /* HIP device ordinal */
+    int verbose;            /* 1: setup timings on stderr */
+} mlfma_params;
+
+/* Reference defaults: PMCHWT, 7 points, 299792458 Hz (lambda = 1 m), eps_r 1,
+ * alpha 0.2, leaf 0.3, FP64, reference TF truncation, 1 RHS, device 0. */
+void mlfma_params_default(mlfma_params* p);
+
+/* Builds the whole operator on the device.  Replaces the setup half of
+ * MLFMM::MLFMM (Code/MLFMM.cpp:15-104): readNasData + GenerateRWG + OCTree +
+ * AlgoMLFMM::computeBase_MLFMM / computeTransferFactorMatrix /
+ * computeInterpolationMatrix + ProcessFMM::computeVsmi_Vfmj_* +
+ * Zmartix::computeZ_fmm_* .  face_vertex_ids holds 3 point ids per face,
+ * in file order (Code/NASReader.cpp:98-119). */
+int mlfma_create(const mlfma_params* p, int n_points, const int* point_ids, const double* xyz,
+                 int n_faces, const int* face_ids, const int* face_vertex_ids, mlfma_handle** out);
+
+/* Same, reading the reference's NAS format (Code/NASReader.cpp:60-144). */
+int mlfma_create_nas(const mlfma_params* p, const char* nas_path, mlfma_handle** out);
+
+void mlfma_destroy(mlfma_handle* h);
+
+/* Message of the last failure on this handle (h may be NULL for create errors). */
+const char* mlfma_last_error(const mlfma_handle* h);
+
+/* n_rwg (N), n_unknowns (N or 2N), max tree level, leaves, near-field complex
+ * entries stored, device bytes held. */
+int mlfma_dims(const mlfma_handle* h, long long* out6);
+
+/* y = A x on host vectors of length n (= n_unknowns).  Replaces the ax(n, x, w)
+ * lambda of MLFMM::matrix_solver (Code/MLFMM.cpp:189-369) and the proposed
+ * MLFMM::applyAx (README/矩阵向量乘法导出头文件设计.md:392-437): y is
+ * overwritten; n must match. */
+int mlfma_apply(mlfma_handle* h, int n, const double* x, double* y);
+
+/* nrhs vectors at once, column-major (vector j at x + 2*n*j). */
+int mlfma_apply_batch(mlfma_handle* h, int n, int nrhs, const double* x, double* y);
+
+/* Device-resident form: x_dev / y_dev are device pointers (interleaved complex
+ * double, column-major n x nrhs), work is queued on the handle's stream and
+ * the call returns after it completes. */
+int mlfma_apply_device(mlfma_handle* h, int n, int nrhs, const void* x_dev, void* y_dev);
+
+/* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)
+ * and MCGS::CGS::solve (Code/CGS.h:72-234) as driven by MLFMM::matrix_solver
+ * (Code/MLFMM.cpp:370-397): x holds the initial guess on entry and the
+ * solution on exit; returns MLFMA_ERR_NOT_CONVERGED (x still written) when
+ * itr_max is exhausted. */
+#define MLFMA_SOLVER_GMRES 0
+#define MLFMA_SOLVER_CGS 1
+typedef struct mlfma_solve_params {
+    int solver;         /* MLFMA_SOLVER_GMRES / MLFMA_SOLVER_CGS (Code/MLFMM.cpp:372-395) */
+    int itr_max;        /* total inner iterations */
+    int restart;        /* GMRES(mr) */
+    double tol_rel;     /* stop when ||r|| / ||b|| < tol_rel (or < tol_abs) */
+    double tol_abs;
+    int precond;        /* 0 none (reference), 1 near-field block-diagonal */
+} mlfma_solve_params;
+
+typedef struct mlfma_solve_stats {
+    int iterations;
+    int matvecs;
+    int converged;
+    double rel_residual;      /* true ||b - A x|| / ||b|| with the FP64 operator */
+    double seconds;
+} mlfma_solve_stats;
+
+int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const double* rhs, double* x,
+                mlfma_solve_stats* stats);
+
+/* Plane-wave excitation for the handle's equation; replaces RHS::computeV_E,
+ * computeV_C and computeV_PMCHWT (Code/Vm.h:10-203) with EMSource::initPW
+ * (Code/EMSource.cpp:34-72).  pol 0 = "V" (E along -theta_hat), 90 = "H". */
+int mlfma_excitation(mlfma_handle* h, double pol_deg, double theta_deg, double phi_deg, double* v);
+
+/* RCS (dBsm) of solved currents I at n_angles observation angles:
+ * out[3*i] = total, out[3*i+1] = V part, out[3*i+2] = H part.  Replaces
+ * RCSUtils::calculateRCS / calculateRCS_PMCHWT and the dB formula of
+ * compute{Dual,Mono}Static[_PMCHWT] (Code/RCS.h:54-192, :247-259). */
+int mlfma_rcs(mlfma_handle* h, const double* I, int n_angles, const double* theta_deg, const double* phi_deg,
+              double pol_deg, double* out);
+
+/* ------------------------------------------------------------------ stages
+ * One MLFMA stage on the handle's device workspaces, for per-stage parity and
+ * profiling; mirrors ProcessMLFMM (Code/ProcessMLFMM.h:14-249).
+ *   MLFMA_STAGE_AGGREGATE  clearSmBm + mexp     (x = host vector of n_unknowns)
+ *   MLFMA_STAGE_M2M        m2m, all levels
+ *   MLFMA_STAGE_M2L_L2L    m2l + l2l, all levels (fused per level)
+ *   MLFMA_STAGE_FAR        finest-level disaggregation into y (host)
+ *   MLFMA_STAGE_NEAR       near-field product into y (host)
+ * Workspaces can be read back with mlfma_get_workspace. */
+#define MLFMA_STAGE_AGGREGATE 0
+#define MLFMA_STAGE_M2M 1
+#define MLFMA_STAGE_M2L_L2L 2
+#define MLFMA_STAGE_FAR 3
+#define MLFMA_STAGE_NEAR 4
+int mlfma_stage(mlfma_handle* h, int stage, const double* x, double* y);
+
+/* Sm (which = 0) or Bm (which = 1) of wavenumber wn (1, 2), current cur
+ * (0 = J, 1 = M; 0 for PEC), tree level `level` (2..maxLevel), node `node`
+ * (Morton order): K x (x, y, z) complex. */
+int mlfma_get_workspace(mlfma_handle* h, int wn, int cur, int which, int level, int node, double* out);
+
+/* ------------------------------------------------------------------ inspection
+ * Read-back of the setup products in the reference's layouts, for parity
+ * tests.  Counts first (pass NULL buffers), then data. */
+
+/* rwg: 6 ints per function (pos face, neg face, edge v0, edge v1, free+, free-
+ * as point indices / face indices in input order), length, edge midpoint. */
+int mlfma_get_rwg(mlfma_handle* h, int* ints6, double* len, double* mid3);
+
+/* Tree level: n nodes, Morton codes, centres, box length, parent index. */
+int mlfma_get_level(mlfma_handle* h, int level, int* n, unsigned long long* codes, double* centres3,
+                    double* len, int* parent);
+
+/* CSR list of a level: which = 0 near neighbours, 1 distant relatives,
+ * 2 their translation-vector ids, 3 sons, 4 RWG ids (leaves only). */
+int mlfma_get_level_list(mlfma_handle* h, int level, int which, int* total, int* ptr, int* idx);
+
+/* Unique translation vectors of a level (Code/OCTree.cpp:500-551). */
+int mlfma_get_drvec(mlfma_handle* h, int level, int* n, double* xyz);
+
+/* Angular grid of wavenumber wn at tree level `level`: L, theta, phi,
+ * Gauss-Legendre weights (node order), dphi. */
+int mlfma_get_grid(mlfma_handle* h, int wn, int level, int* L, double* theta, double* phi, double* wgl,
+                   double* dphi);
+
+/* 3x3 Lagrange table from level+1 to level (level >= 2): per parent sample 9
+ * child sample indices and 9 weights (Code/AlgoMLFMM.h:298-368). */
+int mlfma_get_interp(mlfma_handle* h, int wn, int level, int* taps, double* weights);
+
+/* Translation operators of a level without quadrature weight
+ * (n_vec x K complex, Code/AlgoMLFMM.h:192-295). */
+int mlfma_get_tf(mlfma_handle* h, int wn, int level, double* out);
+
+/* Radiation (which = 0, Vsmi) or receiving (which = 1, Vfmj) pattern of one
+ * RWG: K x (x, y, z) complex (Code/ProcessFMM.h:17-233). */
+int mlfma_get_pattern(mlfma_handle* h, int wn, int which, int rwg, double* out);
+
+/* One near-field row in the reference's column order (Code/Zmartix.h:188-643):
+ * *len entries, column ids and complex values. */
+int mlfma_get_near_row(mlfma_handle* h, int row, int* len, int* cols, double* vals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLFMA_H */

@@ -1,0 +1,259 @@
+"""ctypes binding of libmlfma.so (include/mlfma.h).
+
+The product path has no CPU fallback: if the HIP library is missing or no GPU is
+present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .geometry import Mesh
+
+_LIB_PATH = Path(__file__).resolve().parent / "libmlfma.so"
+
+EFIE, CFIE, PMCHWT = 0, 1, 2
+FP64, FP32 = 0, 1
+GMRES, CGS = 0, 1
+STAGE_AGGREGATE, STAGE_M2M, STAGE_M2L_L2L, STAGE_FAR, STAGE_NEAR = range(5)
+
+ERRORS = {1: "MLFMA_ERR_ARG", 2: "MLFMA_ERR_HIP", 3: "MLFMA_ERR_OOM", 4: "MLFMA_ERR_NONFINITE",
+          5: "MLFMA_ERR_NOT_CONVERGED", 6: "MLFMA_ERR_MESH", 7: "MLFMA_ERR_STATE"}
+
+
+class Params(C.Structure):
+    _fields_ = [("integral_equation", C.c_int), ("gauss_points", C.c_int), ("frequency_hz", C.c_double),
+                ("eps_r_re", C.c_double), ("eps_r_im", C.c_double), ("mu_r_re", C.c_double),
+                ("mu_r_im", C.c_double), ("cfie_alpha", C.c_double), ("leaf_coeff", C.c_double),
+                ("far_precision", C.c_int), ("tf_full_order", C.c_int), ("max_rhs", C.c_int),
+                ("device", C.c_int), ("verbose", C.c_int)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("solver", C.c_int), ("itr_max", C.c_int), ("restart", C.c_int), ("tol_rel", C.c_double),
+                ("tol_abs", C.c_double), ("precond", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("converged", C.c_int),
+                ("rel_residual", C.c_double), ("seconds", C.c_double)]
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libmlfma.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(f"{_LIB_PATH} not built: run python -m mlfma_pmchwt_amd.build")
+        lib = C.CDLL(str(_LIB_PATH))
+        vp = C.c_void_p
+        lib.mlfma_params_default.argtypes = [C.POINTER(Params)]
+        lib.mlfma_create.argtypes = [C.POINTER(Params), C.c_int, vp, vp, C.c_int, vp, vp, C.POINTER(vp)]
+        lib.mlfma_create_nas.argtypes = [C.POINTER(Params), C.c_char_p, C.POINTER(vp)]
+        lib.mlfma_destroy.argtypes = [vp]
+        lib.mlfma_destroy.restype = None
+        lib.mlfma_last_error.argtypes = [vp]
+        lib.mlfma_last_error.restype = C.c_char_p
+        lib.mlfma_dims.argtypes = [vp, vp]
+        lib.mlfma_apply.argtypes = [vp, C.c_int, vp, vp]
+        lib.mlfma_apply_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_apply_device.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_solve.argtypes = [vp, C.POINTER(SolveParams), C.c_int, vp, vp, C.POINTER(SolveStats)]
+        lib.mlfma_excitation.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
+        lib.mlfma_rcs.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
+        lib.mlfma_stage.argtypes = [vp, C.c_int, vp, vp]
+        lib.mlfma_get_workspace.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        lib.mlfma_get_rwg.argtypes = [vp, vp, vp, vp]
+        lib.mlfma_get_level.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        lib.mlfma_get_level_list.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+        lib.mlfma_get_drvec.argtypes = [vp, C.c_int, vp, vp]
+        lib.mlfma_get_grid.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        lib.mlfma_get_interp.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_get_tf.argtypes = [vp, C.c_int, C.c_int, vp]
+        lib.mlfma_get_pattern.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
+        lib.mlfma_get_near_row.argtypes = [vp, C.c_int, vp, vp, vp]
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    library().mlfma_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class MlfmaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Operator:
+    """One MLFMA operator on the GPU (a mlfma_handle)."""
+
+    def __init__(self, mesh: Mesh | str | os.PathLike, **params):
+        lib = library()
+        self.params = default_params(**params)
+        h = C.c_void_p()
+        if isinstance(mesh, (str, os.PathLike)):
+            rc = lib.mlfma_create_nas(C.byref(self.params), str(mesh).encode(), C.byref(h))
+        else:
+            rc = lib.mlfma_create(C.byref(self.params), mesh.n_points, _p(mesh.point_ids), _p(mesh.xyz),
+                                  mesh.n_faces, _p(mesh.face_ids), _p(mesh.faces), C.byref(h))
+        if rc != 0:
+            raise MlfmaError(rc, lib.mlfma_last_error(None).decode())
+        self._h = h
+        d = np.zeros(6, dtype=np.int64)
+        lib.mlfma_dims(self._h, _p(d))
+        self.n_rwg, self.n, self.max_level, self.n_leaves, self.near_entries, self.device_bytes = map(int, d)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mlfma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MlfmaError(rc, library().mlfma_last_error(self._h).decode())
+
+    # ---------------------------------------------------------------- compute
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.ndim == 1:
+            y = np.empty_like(x)
+            self._check(library().mlfma_apply(self._h, x.shape[0], _p(x), _p(y)))
+            return y
+        xb = np.asfortranarray(x)
+        y = np.empty_like(xb)
+        self._check(library().mlfma_apply_batch(self._h, x.shape[0], x.shape[1], _p(xb), _p(y)))
+        return y
+
+    def apply_device(self, x_ptr: int, y_ptr: int, nrhs: int = 1):
+        self._check(library().mlfma_apply_device(self._h, self.n, nrhs, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
+
+    def solve(self, rhs: np.ndarray, x0: np.ndarray | None = None, solver=GMRES, itr_max=None, restart=30,
+              tol_rel=1e-3, tol_abs=0.0, precond=0, check=False):
+        rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+        x = np.zeros_like(rhs) if x0 is None else np.array(x0, dtype=np.complex128)
+        sp = SolveParams(solver, itr_max if itr_max is not None else 4 * self.n, restart, tol_rel, tol_abs, precond)
+        st = SolveStats()
+        rc = library().mlfma_solve(self._h, C.byref(sp), self.n, _p(rhs), _p(x), C.byref(st))
+        if rc not in (0, 5) or (check and rc != 0):
+            self._check(rc)
+        return x, {"iterations": st.iterations, "matvecs": st.matvecs, "converged": bool(st.converged),
+                   "rel_residual": st.rel_residual, "seconds": st.seconds}
+
+    def excitation(self, pol=0.0, theta=0.0, phi=0.0) -> np.ndarray:
+        v = np.empty(self.n, dtype=np.complex128)
+        self._check(library().mlfma_excitation(self._h, pol, theta, phi, _p(v)))
+        return v
+
+    def rcs(self, currents: np.ndarray, theta, phi, pol=0.0) -> np.ndarray:
+        I = np.ascontiguousarray(currents, dtype=np.complex128)
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        ph = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(phi), th.shape), dtype=np.float64)
+        out = np.empty((th.size, 3))
+        self._check(library().mlfma_rcs(self._h, _p(I), th.size, _p(th), _p(ph), pol, _p(out)))
+        return out
+
+    def stage(self, which: int, x: np.ndarray | None = None) -> np.ndarray:
+        y = np.zeros(self.n, dtype=np.complex128)
+        xp = None if x is None else np.ascontiguousarray(x, dtype=np.complex128)
+        self._check(library().mlfma_stage(self._h, which, None if xp is None else _p(xp), _p(y)))
+        return y
+
+    # ---------------------------------------------------------------- inspection
+    def grid(self, wn: int, level: int):
+        L = C.c_int()
+        library().mlfma_get_grid(self._h, wn, level, C.byref(L), None, None, None, None)
+        th = np.empty(L.value)
+        ph = np.empty(2 * L.value)
+        w = np.empty(L.value)
+        dphi = np.zeros(1)
+        self._check(library().mlfma_get_grid(self._h, wn, level, C.byref(L), _p(th), _p(ph), _p(w), _p(dphi)))
+        return L.value, th, ph, w, float(dphi[0])
+
+    def workspace(self, wn, cur, which, level, node):
+        L = self.grid(wn, level)[0]
+        out = np.empty((2 * L * L, 3), dtype=np.complex128)
+        self._check(library().mlfma_get_workspace(self._h, wn, cur, which, level, node, _p(out)))
+        return out
+
+    def rwg(self):
+        ints = np.empty((self.n_rwg, 6), dtype=np.int32)
+        ln = np.empty(self.n_rwg)
+        mid = np.empty((self.n_rwg, 3))
+        self._check(library().mlfma_get_rwg(self._h, _p(ints), _p(ln), _p(mid)))
+        return ints, ln, mid
+
+    def level(self, level: int):
+        n = C.c_int()
+        self._check(library().mlfma_get_level(self._h, level, C.byref(n), None, None, None, None))
+        codes = np.empty(n.value, dtype=np.uint64)
+        cen = np.empty((n.value, 3))
+        ln = np.empty(n.value)
+        par = np.empty(n.value, dtype=np.int32)
+        self._check(library().mlfma_get_level(self._h, level, C.byref(n), _p(codes), _p(cen), _p(ln), _p(par)))
+        return codes, cen, ln, par
+
+    def level_list(self, level: int, which: int):
+        tot = C.c_int()
+        self._check(library().mlfma_get_level_list(self._h, level, which, C.byref(tot), None, None))
+        n = self.level(level)[0].size
+        ptr = np.empty(n + 1, dtype=np.int32)
+        idx = np.empty(max(tot.value, 1), dtype=np.int32)
+        self._check(library().mlfma_get_level_list(self._h, level, which, C.byref(tot), _p(ptr), _p(idx)))
+        return ptr, idx[:tot.value]
+
+    def drvec(self, level: int):
+        n = C.c_int()
+        self._check(library().mlfma_get_drvec(self._h, level, C.byref(n), None))
+        out = np.empty((n.value, 3))
+        self._check(library().mlfma_get_drvec(self._h, level, C.byref(n), _p(out)))
+        return out
+
+    def interp(self, wn: int, level: int):
+        L = self.grid(wn, level)[0]
+        taps = np.empty((2 * L * L, 9), dtype=np.int32)
+        w = np.empty((2 * L * L, 9))
+        self._check(library().mlfma_get_interp(self._h, wn, level, _p(taps), _p(w)))
+        return taps, w
+
+    def tf(self, wn: int, level: int):
+        L = self.grid(wn, level)[0]
+        nv = self.drvec(level).shape[0]
+        out = np.empty((nv, 2 * L * L), dtype=np.complex128)
+        self._check(library().mlfma_get_tf(self._h, wn, level, _p(out)))
+        return out
+
+    def pattern(self, wn: int, which: int, rwg: int):
+        L = self.grid(wn, self.max_level)[0]
+        out = np.empty((2 * L * L, 3), dtype=np.complex128)
+        self._check(library().mlfma_get_pattern(self._h, wn, which, rwg, _p(out)))
+        return out
+
+    def near_row(self, row: int):
+        n = C.c_int()
+        self._check(library().mlfma_get_near_row(self._h, row, C.byref(n), None, None))
+        cols = np.empty(n.value, dtype=np.int32)
+        vals = np.empty(n.value, dtype=np.complex128)
+        self._check(library().mlfma_get_near_row(self._h, row, C.byref(n), _p(cols), _p(vals)))
+        return cols, vals

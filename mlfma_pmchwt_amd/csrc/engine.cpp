@@ -1,0 +1,780 @@
+// Engine: setup orchestration, the matvec pipeline and device Krylov solvers.
+#include "engine.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "kernels.hpp"
+#include "setup.hpp"
+
+namespace mlfma {
+
+namespace {
+double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
+inline cplx cx(double2 d) { return cplx(d.x, d.y); }
+constexpr int kRedBlocks = 256;   // fixed partition of every reduction
+}  // namespace
+
+// Code/EMSource.cpp:34-72: incidence -r_hat, v = -theta_hat, h = phi_hat,
+// E = v (pol 0) or h (pol 90), H = normalise(k x E).
+void plane_wave(double pol_deg, double th_deg, double ph_deg, double* k, double* e, double* h, double* vth,
+                double* vph) {
+    auto rad = [](double a) {
+        double b = a * kPi / 180.0;
+        return std::fabs(b) < 1e-6 ? 0.0 : b;
+    };
+    auto snap = [](double a) { return std::fabs(a) < 1e-6 ? 0.0 : a; };
+    const double pol = rad(pol_deg), th = rad(th_deg), ph = rad(ph_deg);
+    k[0] = snap(-std::sin(th) * std::cos(ph));
+    k[1] = snap(-std::sin(th) * std::sin(ph));
+    k[2] = snap(-std::cos(th));
+    vth[0] = snap(-std::cos(th) * std::cos(ph));
+    vth[1] = snap(-std::cos(th) * std::sin(ph));
+    vth[2] = snap(std::sin(th));
+    vph[0] = snap(-std::sin(ph));
+    vph[1] = snap(std::cos(ph));
+    vph[2] = 0.0;
+    for (int i = 0; i < 3; ++i) e[i] = h[i] = 0.0;
+    const double* src = nullptr;
+    if (std::fabs(pol - rad(90.0)) < 1e-6) src = vph;
+    if (std::fabs(pol) < 1e-6) src = vth;
+    if (src) {
+        for (int i = 0; i < 3; ++i) e[i] = src[i];
+        double c[3] = {k[1] * e[2] - k[2] * e[1], k[2] * e[0] - k[0] * e[2], k[0] * e[1] - k[1] * e[0]};
+        double nn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (int i = 0; i < 3; ++i) h[i] = nn > 1e-15 ? c[i] / nn : c[i];
+    }
+}
+
+Engine::Engine(const mlfma_params& p) : P(p) {
+    ie = p.integral_equation;
+    nw = (ie == MLFMA_PMCHWT) ? 2 : 1;
+    ncur = nw;
+    MLFMA_HIP(hipSetDevice(p.device));
+    MLFMA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+}
+
+Engine::~Engine() {
+    if (st) hipStreamSynchronize(st);
+    for (void* a : allocs_) hipFree(a);
+    if (st) hipStreamDestroy(st);
+}
+
+template <typename T>
+T* Engine::dalloc(size_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    MLFMA_HIP(hipMalloc(&p, count * sizeof(T)));
+    allocs_.push_back(p);
+    dev_bytes += count * sizeof(T);
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+T* Engine::upload(const std::vector<T>& v) {
+    T* d = dalloc<T>(v.size());
+    if (!v.empty()) MLFMA_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+void Engine::build(int np, const int* ids, const double* xyz, int nf, const int* fids, const int* fvid) {
+    const double t0 = wall();
+    std::string err;
+    if (!build_mesh(np, ids, xyz, nf, fids, fvid, mesh, err)) throw Error(MLFMA_ERR_MESH, err);
+    build_basis(mesh, basis);
+    N = basis.n;
+    if (N == 0) throw Error(MLFMA_ERR_MESH, "mesh has no interior edges");
+    nunk = (ie == MLFMA_PMCHWT) ? 2 * N : N;
+    med = make_medium(P.frequency_hz, cplx(P.eps_r_re, P.eps_r_im), cplx(P.mu_r_re, P.mu_r_im), ie == MLFMA_PMCHWT);
+    build_octree(basis, med.lambda, P.leaf_coeff, tree);
+    const int L = tree.max_level;
+    nleaf = tree.lv[L].n;
+    inv_perm.assign(N, 0);
+    leaf_of.assign(N, 0);
+    for (int i = 0; i < N; ++i) inv_perm[tree.perm[i]] = i;
+    max_leaf = 0;
+    for (int t = 0; t < nleaf; ++t) {
+        max_leaf = std::max(max_leaf, tree.leaf_ptr[t + 1] - tree.leaf_ptr[t]);
+        for (int i = tree.leaf_ptr[t]; i < tree.leaf_ptr[t + 1]; ++i) leaf_of[i] = t;
+    }
+    build_wave_tables(tree, med.k1, std::abs(med.k1), P.tf_full_order != 0, W[0]);
+    if (nw == 2) build_wave_tables(tree, med.k2, std::abs(med.k2), P.tf_full_order != 0, W[1]);
+    const double t1 = wall();
+
+    std::vector<double> l1, l2, l3, w;
+    if (!dunavant(P.gauss_points, l1, l2, l3, w)) throw Error(MLFMA_ERR_ARG, "unsupported Gauss point count");
+    gauss_.n = P.gauss_points;
+    for (int q = 0; q < gauss_.n; ++q) gauss_.l1[q] = l1[q], gauss_.l2[q] = l2[q], gauss_.l3[q] = l3[q], gauss_.w[q] = w[q];
+
+    std::vector<RwgGeo> geo(N);
+    for (int i = 0; i < N; ++i) {
+        const int r = tree.perm[i];
+        RwgGeo& g = geo[i];
+        const int faces[2] = {basis.fneg[r], basis.fpos[r]};
+        const int frees[2] = {basis.vneg[r], basis.vpos[r]};
+        for (int s = 0; s < 2; ++s) {
+            const int f = faces[s];
+            for (int v = 0; v < 3; ++v) {
+                const Vec3& p = mesh.pts[mesh.tri[3 * f + v]];
+                g.tri[s][3 * v] = p.x, g.tri[s][3 * v + 1] = p.y, g.tri[s][3 * v + 2] = p.z;
+            }
+            const Vec3& fv = mesh.pts[frees[s]];
+            g.vf[s][0] = fv.x, g.vf[s][1] = fv.y, g.vf[s][2] = fv.z;
+            g.nrm[s][0] = mesh.normal[f].x, g.nrm[s][1] = mesh.normal[f].y, g.nrm[s][2] = mesh.normal[f].z;
+            g.area[s] = mesh.area[f];
+            g.face[s] = mesh.face_id[f];
+        }
+        g.len = basis.len[r];
+    }
+    d_geo_ = upload(geo);
+    d_perm_ = upload(tree.perm);
+    d_leaf_ptr_ = upload(tree.leaf_ptr);
+    d_leaf_of_ = upload(leaf_of);
+    std::vector<double> lc(3 * (size_t)nleaf);
+    for (int t = 0; t < nleaf; ++t) {
+        lc[3 * t] = tree.lv[L].centre[t].x;
+        lc[3 * t + 1] = tree.lv[L].centre[t].y;
+        lc[3 * t + 2] = tree.lv[L].centre[t].z;
+    }
+    d_leaf_c_ = upload(lc);
+    build_device_tree();
+    for (int wv = 0; wv < nw; ++wv) build_wave(wv);
+    const double t2 = wall();
+    build_near();
+    MLFMA_HIP(hipStreamSynchronize(st));
+    MLFMA_HIP(hipGetLastError());
+    const double t3 = wall();
+    if (P.verbose)
+        std::fprintf(stderr,
+                     "[mlfma] N=%d unknowns=%d levels=%d leaves=%d max_leaf=%d | host tables %.3fs, patterns "
+                     "%.3fs, near field %.3fs (%lld entries), device %.2f GB\n",
+                     N, nunk, L, nleaf, max_leaf, t1 - t0, t2 - t1, t3 - t2, near_entries, dev_bytes / 1e9);
+}
+
+void Engine::build_device_tree() {
+    const int L = tree.max_level;
+    lv_.assign(L + 1, LevelDev());
+    for (int l = 2; l <= L; ++l) {
+        const Level& v = tree.lv[l];
+        LevelDev& d = lv_[l];
+        d.n = v.n;
+        d.dr_ptr = upload(v.dr_ptr);
+        d.dr_idx = upload(v.dr_idx);
+        d.dr_vec = upload(v.dr_vec);
+        if (l < L) {
+            d.son_ptr = upload(v.son_ptr);
+            d.son_idx = upload(v.son_idx);
+        }
+        d.parent = upload(v.parent);
+        std::vector<int> oct(v.n);
+        for (int i = 0; i < v.n; ++i) oct[i] = (int)(v.code[i] & 7);
+        d.oct = upload(oct);
+    }
+}
+
+void Engine::build_wave(int w) {
+    const WaveTables& T = W[w];
+    WaveDev& D = wd_[w];
+    const int L = tree.max_level;
+    D.K.assign(L + 1, 0);
+    D.Sm.assign(L + 1, nullptr);
+    D.Bm.assign(L + 1, nullptr);
+    D.TFw.assign(L + 1, nullptr);
+    D.tap.assign(L + 1, nullptr);
+    D.tw.assign(L + 1, nullptr);
+    D.t_ptr.assign(L + 1, nullptr);
+    D.t_kp.assign(L + 1, nullptr);
+    D.t_w.assign(L + 1, nullptr);
+    D.sh_up.assign(L + 1, nullptr);
+    D.sh_dn.assign(L + 1, nullptr);
+    for (int l = 2; l <= L; ++l) D.K[l] = T.grid[l].K();
+    const int K0 = D.K[L];
+    std::vector<double> kp(3 * (size_t)K0);
+    for (int q = 0; q < K0; ++q) {
+        kp[q] = T.grid[L].kp[q].x;
+        kp[K0 + q] = T.grid[L].kp[q].y;
+        kp[2 * K0 + q] = T.grid[L].kp[q].z;
+    }
+    D.kp0 = upload(kp);
+    D.Vs = dalloc<double2>((size_t)N * 3 * K0);
+    D.Vf = dalloc<double2>((size_t)N * 3 * K0);
+    launch_patterns(st, ie == MLFMA_CFIE, N, K0, d_geo_, d_leaf_of_, d_leaf_c_, D.kp0, d2(T.k), gauss_, P.cfie_alpha,
+                    D.Vs, D.Vf);
+    MLFMA_HIP(hipGetLastError());
+    for (int l = 2; l <= L; ++l) {
+        const size_t ws = (size_t)tree.lv[l].n * ncur * 3 * D.K[l];
+        D.Sm[l] = dalloc<double2>(ws);
+        D.Bm[l] = dalloc<double2>(ws);
+        // quadrature weight w_theta * dphi folded into the translation operator
+        // (the reference applies it inside m2l, Code/ProcessMLFMM.h:138-142)
+        std::vector<double2> tfw(T.tf[l].size());
+        const int K = D.K[l];
+        for (size_t e = 0; e < tfw.size(); ++e) tfw[e] = d2(T.tf[l][e] * T.qw[l][e % K]);
+        D.TFw[l] = upload(tfw);
+    }
+    for (int c = 3; c <= L; ++c) {
+        const InterpTable& it = T.interp[c];
+        D.tap[c] = upload(it.tap);
+        D.tw[c] = upload(it.w);
+        D.t_ptr[c] = upload(it.t_ptr);
+        D.t_kp[c] = upload(it.t_kp);
+        D.t_w[c] = upload(it.t_w);
+        std::vector<double2> su(T.shift_up[c].size()), sd(T.shift_down[c].size());
+        for (size_t e = 0; e < su.size(); ++e) su[e] = d2(T.shift_up[c][e]);
+        for (size_t e = 0; e < sd.size(); ++e) sd[e] = d2(T.shift_down[c][e]);
+        D.sh_up[c] = upload(su);
+        D.sh_dn[c] = upload(sd);
+    }
+}
+
+// Leaf-pair blocks: for every leaf its near neighbours in list order, then
+// itself (the column order of Code/Zmartix.h:595-629).
+void Engine::build_near() {
+    const Level& lf = tree.lv[tree.max_level];
+    const int planes = (ie == MLFMA_PMCHWT) ? 3 : 1;
+    pair_ptr.assign(1, 0);
+    pair_tgt.clear();
+    pair_src.clear();
+    pair_off.clear();
+    long long off = 0;
+    for (int t = 0; t < nleaf; ++t) {
+        const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
+        auto add = [&](int s) {
+            const int ns = tree.leaf_ptr[s + 1] - tree.leaf_ptr[s];
+            pair_tgt.push_back(t);
+            pair_src.push_back(s);
+            pair_off.push_back(off);
+            off += (long long)planes * nt * ns;
+        };
+        for (int j = lf.near_ptr[t]; j < lf.near_ptr[t + 1]; ++j) add(lf.near_idx[j]);
+        add(t);
+        pair_ptr.push_back((int)pair_src.size());
+    }
+    near_entries = off;
+    d_pair_ptr_ = upload(pair_ptr);
+    d_pair_tgt_ = upload(pair_tgt);
+    d_pair_src_ = upload(pair_src);
+    d_pair_off_ = upload(pair_off);
+    d_Z_ = dalloc<double2>((size_t)off);
+    NearConsts c;
+    c.k1 = d2(med.k1);
+    c.k2 = d2(med.k2);
+    c.eta1 = d2(med.eta1);
+    c.etai = d2(med.etai);
+    c.inv_etai = d2(1.0 / med.etai);
+    c.alpha = P.cfie_alpha;
+    launch_near_fill(st, ie, (int)pair_src.size(), d_pair_tgt_, d_pair_src_, d_pair_off_, d_leaf_ptr_, d_geo_, gauss_, c,
+                     d_Z_);
+    MLFMA_HIP(hipGetLastError());
+}
+
+// mexp, m2m (fine -> coarse), then m2l with the gather-form l2l (coarse -> fine)
+void Engine::far_pass(int w, const double2* x) {
+    WaveDev& D = wd_[w];
+    const int L = tree.max_level;
+    launch_aggregate(st, ncur, nleaf, D.K[L], N, d_leaf_ptr_, d_perm_, D.Vs, x, D.Sm[L]);
+    for (int l = L - 1; l >= 2; --l)
+        launch_m2m(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
+                   D.tap[l + 1], D.tw[l + 1], D.sh_up[l + 1], D.Sm[l + 1], D.Sm[l]);
+    for (int l = 2; l <= L; ++l) {
+        const bool up = l > 2;
+        launch_m2l_l2l(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, D.TFw[l], D.Sm[l],
+                       up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l], D.t_w[l], D.sh_dn[l],
+                       up ? D.Bm[l - 1] : nullptr, D.Bm[l]);
+    }
+}
+
+void Engine::apply_one(const double2* x, double2* y) {
+    const int L = tree.max_level;
+    for (int w = 0; w < nw; ++w) far_pass(w, x);
+    launch_near(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_pair_ptr_, d_pair_src_, d_pair_off_,
+                d_Z_, max_leaf, x, y);
+    if (ie != MLFMA_PMCHWT) {
+        // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
+        const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
+        launch_disagg_pec(st, nleaf, wd_[0].K[L], d_leaf_ptr_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), y);
+    } else {
+        PmchwtFarArgs a;
+        a.K1 = wd_[0].K[L];
+        a.K2 = wd_[1].K[L];
+        a.N = N;
+        a.Vf1 = wd_[0].Vf;
+        a.Vf2 = wd_[1].Vf;
+        a.B1 = wd_[0].Bm[L];
+        a.B2 = wd_[1].Bm[L];
+        a.kp1 = wd_[0].kp0;
+        a.kp2 = wd_[1].kp0;
+        a.c1 = d2(med.k1 * med.k1 / (16.0 * kPi * kPi));   // Code/MLFMM.cpp:252-253
+        a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
+        a.etai = d2(med.etai);
+        a.inv_etai = d2(1.0 / med.etai);
+        launch_disagg_pmchwt(st, nleaf, d_leaf_ptr_, d_perm_, a, y);
+    }
+}
+
+void Engine::apply_device(const double2* x, double2* y, int nrhs) {
+    for (int r = 0; r < nrhs; ++r) apply_one(x + (size_t)r * nunk, y + (size_t)r * nunk);
+    MLFMA_HIP(hipGetLastError());
+}
+
+void Engine::stage(int which, const double2* dx, double2* dy) {
+    const int L = tree.max_level;
+    switch (which) {
+        case MLFMA_STAGE_AGGREGATE:
+            for (int w = 0; w < nw; ++w)
+                launch_aggregate(st, ncur, nleaf, wd_[w].K[L], N, d_leaf_ptr_, d_perm_, wd_[w].Vs, dx, wd_[w].Sm[L]);
+            break;
+        case MLFMA_STAGE_M2M:
+            for (int w = 0; w < nw; ++w) {
+                WaveDev& D = wd_[w];
+                for (int l = L - 1; l >= 2; --l)
+                    launch_m2m(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
+                               D.tap[l + 1], D.tw[l + 1], D.sh_up[l + 1], D.Sm[l + 1], D.Sm[l]);
+            }
+            break;
+        case MLFMA_STAGE_M2L_L2L:
+            for (int w = 0; w < nw; ++w) {
+                WaveDev& D = wd_[w];
+                for (int l = 2; l <= L; ++l) {
+                    const bool up = l > 2;
+                    launch_m2l_l2l(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, D.TFw[l],
+                                   D.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l],
+                                   D.t_w[l], D.sh_dn[l], up ? D.Bm[l - 1] : nullptr, D.Bm[l]);
+                }
+            }
+            break;
+        case MLFMA_STAGE_FAR: {
+            MLFMA_HIP(hipMemsetAsync(dy, 0, sizeof(double2) * nunk, st));
+            if (ie != MLFMA_PMCHWT) {
+                const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
+                launch_disagg_pec(st, nleaf, wd_[0].K[L], d_leaf_ptr_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), dy);
+            } else {
+                PmchwtFarArgs a;
+                a.K1 = wd_[0].K[L], a.K2 = wd_[1].K[L], a.N = N;
+                a.Vf1 = wd_[0].Vf, a.Vf2 = wd_[1].Vf, a.B1 = wd_[0].Bm[L], a.B2 = wd_[1].Bm[L];
+                a.kp1 = wd_[0].kp0, a.kp2 = wd_[1].kp0;
+                a.c1 = d2(med.k1 * med.k1 / (16.0 * kPi * kPi));
+                a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
+                a.etai = d2(med.etai), a.inv_etai = d2(1.0 / med.etai);
+                launch_disagg_pmchwt(st, nleaf, d_leaf_ptr_, d_perm_, a, dy);
+            }
+            break;
+        }
+        case MLFMA_STAGE_NEAR:
+            launch_near(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_pair_ptr_, d_pair_src_,
+                        d_pair_off_, d_Z_, max_leaf, dx, dy);
+            break;
+        default: throw Error(MLFMA_ERR_ARG, "unknown stage");
+    }
+    MLFMA_HIP(hipGetLastError());
+    MLFMA_HIP(hipStreamSynchronize(st));
+}
+
+double Engine::norm(const double2* v, int n) {
+    launch_norm2_partial(st, n, v, d_rpart_, kRedBlocks);
+    std::vector<double> h(kRedBlocks);
+    MLFMA_HIP(hipMemcpyAsync(h.data(), d_rpart_, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+    MLFMA_HIP(hipStreamSynchronize(st));
+    double s = 0;
+    for (double x : h) s += x;
+    return std::sqrt(s);
+}
+
+// Restarted GMRES(m) on the device.  Same outer logic as Code/GMRES.h:65-217
+// (Givens QR of the Hessenberg matrix, exact residual at every restart,
+// divergence guard); the Arnoldi step uses classical Gram-Schmidt applied twice
+// (two block reductions per pass instead of one per basis vector), which has
+// the orthogonality of the reference's MGS + re-orthogonalisation.  With
+// precond = 1 the Krylov space is built for A M^{-1} (right preconditioning) and
+// the reported residual is the true ||b - A x|| / ||b||.
+void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, mlfma_solve_stats& stats) {
+    const double t0 = wall();
+    const int n = nunk;
+    const int m = std::max(1, sp.restart);
+    const double eps = 1e-16;
+    if (V_cols_ < std::max(m + 1, 10)) {
+        V_cols_ = std::max(m + 1, 10);
+        d_V_ = dalloc<double2>((size_t)V_cols_ * n);
+        d_w_ = dalloc<double2>(n);
+        d_r_ = dalloc<double2>(n);
+        d_t_ = dalloc<double2>(n);
+        d_part_ = dalloc<double2>((size_t)V_cols_ * kRedBlocks);
+        d_small_ = dalloc<double2>(V_cols_);
+        d_rpart_ = dalloc<double>(kRedBlocks);
+    }
+    const bool pc = sp.precond != 0;
+    if (pc) precond_setup();
+    int matvecs = 0, total = 0;
+    auto A = [&](const double2* in, double2* out) {
+        if (pc) {
+            precond_apply(in, d_t_);
+            apply_one(d_t_, out);
+        } else {
+            apply_one(in, out);
+        }
+        ++matvecs;
+    };
+    auto true_residual = [&]() {
+        apply_one(d_x, d_r_);
+        ++matvecs;
+        launch_residual(st, n, d_b, d_r_);
+        return norm(d_r_, n);
+    };
+    auto dots = [&](int cnt, const double2* w, std::vector<cplx>& h) {
+        launch_multidot(st, n, cnt, d_V_, (size_t)n, w, d_part_, kRedBlocks);
+        launch_reduce_partials(st, cnt, kRedBlocks, d_part_, d_small_);
+        std::vector<double2> tmp(cnt);
+        MLFMA_HIP(hipMemcpyAsync(tmp.data(), d_small_, cnt * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        h.resize(cnt);
+        for (int j = 0; j < cnt; ++j) h[j] = cx(tmp[j]);
+    };
+    auto put_small = [&](const std::vector<cplx>& v, int cnt) {
+        std::vector<double2> tmp(cnt);
+        for (int j = 0; j < cnt; ++j) tmp[j] = d2(v[j]);
+        MLFMA_HIP(hipMemcpyAsync(d_small_, tmp.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+    };
+
+    double bnorm = norm(d_b, n);
+    if (bnorm < eps) bnorm = 1.0;
+    double rnorm = true_residual();
+    double err = rnorm / bnorm;
+    bool converged = (err < sp.tol_rel || err < sp.tol_abs);
+    if (!std::isfinite(err)) throw Error(MLFMA_ERR_NONFINITE, "non-finite initial residual");
+
+    if (!converged && sp.solver == MLFMA_SOLVER_GMRES) {
+        std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), h1, h2;
+        auto Hm = [&](int i, int j) -> cplx& { return H[i + (size_t)j * (m + 1)]; };
+        for (int cyc = 1; cyc <= sp.itr_max && !converged; ++cyc) {
+            launch_scale_copy(st, n, 1.0 / rnorm, d_r_, d_V_);
+            std::fill(s.begin(), s.end(), cplx(0, 0));
+            s[0] = rnorm;
+            std::fill(H.begin(), H.end(), cplx(0, 0));
+            int kused = 0;
+            for (int k = 0; k < m; ++k) {
+                if (total >= sp.itr_max) break;
+                ++total;
+                kused = k + 1;
+                A(d_V_ + (size_t)k * n, d_w_);
+                dots(k + 1, d_w_, h1);
+                put_small(h1, k + 1);
+                launch_sub_combination(st, n, k + 1, d_V_, (size_t)n, d_small_, d_w_);
+                dots(k + 1, d_w_, h2);
+                put_small(h2, k + 1);
+                launch_sub_combination(st, n, k + 1, d_V_, (size_t)n, d_small_, d_w_);
+                for (int j = 0; j <= k; ++j) Hm(j, k) = h1[j] + h2[j];
+                const double hn = norm(d_w_, n);
+                if (!std::isfinite(hn)) throw Error(MLFMA_ERR_NONFINITE, "non-finite Arnoldi vector");
+                Hm(k + 1, k) = hn;
+                if (hn < eps * rnorm) break;
+                launch_scale_copy(st, n, 1.0 / hn, d_w_, d_V_ + (size_t)(k + 1) * n);
+                // Code/GMRES.h:22-48, 144-152
+                for (int j = 0; j < k; ++j) {
+                    cplx tmp = std::conj(cs[j]) * Hm(j, k) + std::conj(sn[j]) * Hm(j + 1, k);
+                    Hm(j + 1, k) = -sn[j] * Hm(j, k) + cs[j] * Hm(j + 1, k);
+                    Hm(j, k) = tmp;
+                }
+                cplx a = Hm(k, k), b = Hm(k + 1, k);
+                if (std::abs(b) < eps) {
+                    cs[k] = 1.0, sn[k] = 0.0;
+                } else if (std::abs(a) < eps) {
+                    cs[k] = 0.0, sn[k] = 1.0;
+                } else {
+                    double r = std::sqrt(std::real(a * std::conj(a) + b * std::conj(b)));
+                    cs[k] = a / r, sn[k] = b / r;
+                }
+                cplx tmp = std::conj(cs[k]) * a + std::conj(sn[k]) * b;
+                Hm(k, k) = tmp;
+                Hm(k + 1, k) = 0.0;
+                tmp = std::conj(cs[k]) * s[k] + std::conj(sn[k]) * s[k + 1];
+                s[k + 1] = -sn[k] * s[k] + cs[k] * s[k + 1];
+                s[k] = tmp;
+                err = std::abs(s[k + 1]) / bnorm;
+                if (err < sp.tol_rel || err < sp.tol_abs) break;
+            }
+            const int mm = std::min(kused, m);
+            for (int i = mm - 1; i >= 0; --i) {
+                y[i] = s[i];
+                for (int j = i + 1; j < mm; ++j) y[i] -= Hm(i, j) * y[j];
+                y[i] = (std::abs(Hm(i, i)) > eps) ? y[i] / Hm(i, i) : cplx(0, 0);
+            }
+            put_small(y, mm);
+            if (pc) {
+                MLFMA_HIP(hipMemsetAsync(d_w_, 0, sizeof(double2) * n, st));
+                launch_add_combination(st, n, mm, d_V_, (size_t)n, d_small_, d_w_);
+                precond_apply(d_w_, d_t_);
+                launch_axpy(st, n, make_double2(1.0, 0.0), d_t_, d_x, d_x);
+            } else {
+                launch_add_combination(st, n, mm, d_V_, (size_t)n, d_small_, d_x);
+            }
+            rnorm = true_residual();
+            err = rnorm / bnorm;
+            if (!std::isfinite(err)) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
+            if (err < sp.tol_rel || err < sp.tol_abs) converged = true;
+            if (total >= sp.itr_max) break;
+            if (err > 1000.0 * sp.tol_rel && cyc > 8) break;   // Code/GMRES.h:208-214
+        }
+    } else if (!converged) {
+        // Code/CGS.h:104-234 (bilinear inner products); columns of V as vectors
+        double2 *r = d_r_, *Rt = d_V_, *u = d_V_ + (size_t)n, *p = d_V_ + 2 * (size_t)n, *Vv = d_V_ + 3 * (size_t)n,
+                *q = d_V_ + 4 * (size_t)n, *uq = d_V_ + 5 * (size_t)n, *Q = d_V_ + 6 * (size_t)n,
+                *tmp = d_V_ + 7 * (size_t)n;
+        auto bidot = [&](const double2* a, const double2* b) {
+            launch_bidot_partial(st, n, a, b, d_part_, kRedBlocks);
+            launch_reduce_partials(st, 1, kRedBlocks, d_part_, d_small_);
+            double2 h;
+            MLFMA_HIP(hipMemcpyAsync(&h, d_small_, sizeof(double2), hipMemcpyDeviceToHost, st));
+            MLFMA_HIP(hipStreamSynchronize(st));
+            return cx(h);
+        };
+        MLFMA_HIP(hipMemcpyAsync(Rt, r, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+        cplx rho_prev(0, 0);
+        for (int it = 1; it <= sp.itr_max; ++it) {
+            const cplx rho = bidot(r, Rt);
+            if (std::abs(rho) < eps) break;
+            if (it == 1) {
+                MLFMA_HIP(hipMemcpyAsync(u, r, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+                MLFMA_HIP(hipMemcpyAsync(p, u, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+            } else {
+                const cplx beta = rho / rho_prev;
+                launch_axpy(st, n, d2(beta), q, r, u);      // u = r + beta q
+                launch_axpy(st, n, d2(beta), p, q, tmp);    // tmp = q + beta p
+                launch_axpy(st, n, d2(beta), tmp, u, p);    // p = u + beta tmp
+            }
+            A(p, Vv);
+            const cplx den = bidot(Rt, Vv);
+            if (std::abs(den) < eps) break;
+            const cplx al = rho / den;
+            launch_axpy(st, n, d2(-al), Vv, u, q);                       // q = u - al V
+            launch_axpy(st, n, make_double2(1.0, 0.0), q, u, uq);        // uq = u + q
+            if (pc) {
+                precond_apply(uq, tmp);
+                launch_axpy(st, n, d2(al), tmp, d_x, d_x);
+                apply_one(tmp, Q);
+                ++matvecs;
+            } else {
+                launch_axpy(st, n, d2(al), uq, d_x, d_x);
+                A(uq, Q);
+            }
+            launch_axpy(st, n, d2(-al), Q, r, r);
+            total = it;
+            err = norm(r, n) / bnorm;
+            if (!std::isfinite(err)) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
+            if (err < sp.tol_rel) break;
+            rho_prev = rho;
+        }
+        rnorm = true_residual();
+        err = rnorm / bnorm;
+        converged = err < sp.tol_rel || err < sp.tol_abs;
+    }
+    stats.iterations = total;
+    stats.matvecs = matvecs;
+    stats.converged = converged ? 1 : 0;
+    stats.rel_residual = err;
+    stats.seconds = wall() - t0;
+}
+
+// Near-field block-diagonal preconditioner: exact inverse of every leaf's self
+// block (no reference counterpart; the reference solves unpreconditioned).
+void Engine::precond_setup() {
+    if (d_Dinv_) return;
+    const bool pm = ie == MLFMA_PMCHWT;
+    std::vector<long long> off(nleaf + 1, 0);
+    for (int t = 0; t < nleaf; ++t) {
+        const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
+        const int m = pm ? 2 * nt : nt;
+        off[t + 1] = off[t] + (long long)m * m;
+    }
+    std::vector<double2> Dinv((size_t)off[nleaf]);
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < nleaf; ++t) {
+        const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
+        const int m = pm ? 2 * nt : nt;
+        const int p = pair_ptr[t + 1] - 1;   // self block is the last pair of the row
+        std::vector<double2> blk((size_t)(pm ? 3 : 1) * nt * nt);
+        hipMemcpy(blk.data(), d_Z_ + pair_off[p], blk.size() * sizeof(double2), hipMemcpyDeviceToHost);
+        std::vector<cplx> A((size_t)m * m), I((size_t)m * m, cplx(0, 0));
+        for (int j = 0; j < nt; ++j)
+            for (int i = 0; i < nt; ++i) {
+                if (!pm) {
+                    A[i + (size_t)j * m] = cx(blk[(size_t)j * nt + i]);
+                } else {
+                    const cplx z11 = cx(blk[(size_t)j * nt + i]), zk = cx(blk[(size_t)(nt + j) * nt + i]),
+                               z22 = cx(blk[(size_t)(2 * nt + j) * nt + i]);
+                    A[i + (size_t)j * m] = z11;
+                    A[i + (size_t)(nt + j) * m] = -zk;
+                    A[(nt + i) + (size_t)j * m] = zk;
+                    A[(nt + i) + (size_t)(nt + j) * m] = z22;
+                }
+            }
+        for (int i = 0; i < m; ++i) I[i + (size_t)i * m] = 1.0;
+        // Gauss-Jordan with partial pivoting
+        for (int c = 0; c < m; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < m; ++r)
+                if (std::abs(A[r + (size_t)c * m]) > std::abs(A[piv + (size_t)c * m])) piv = r;
+            if (piv != c)
+                for (int j = 0; j < m; ++j) {
+                    std::swap(A[c + (size_t)j * m], A[piv + (size_t)j * m]);
+                    std::swap(I[c + (size_t)j * m], I[piv + (size_t)j * m]);
+                }
+            const cplx d = A[c + (size_t)c * m];
+            for (int j = 0; j < m; ++j) {
+                A[c + (size_t)j * m] /= d;
+                I[c + (size_t)j * m] /= d;
+            }
+            for (int r = 0; r < m; ++r) {
+                if (r == c) continue;
+                const cplx f = A[r + (size_t)c * m];
+                if (f == cplx(0, 0)) continue;
+                for (int j = 0; j < m; ++j) {
+                    A[r + (size_t)j * m] -= f * A[c + (size_t)j * m];
+                    I[r + (size_t)j * m] -= f * I[c + (size_t)j * m];
+                }
+            }
+        }
+        for (size_t e = 0; e < I.size(); ++e) Dinv[off[t] + e] = d2(I[e]);
+    }
+    d_Dinv_ = upload(Dinv);
+    d_Doff_ = upload(off);
+}
+
+void Engine::precond_apply(const double2* x, double2* y) {
+    launch_block_diag(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_Doff_, d_Dinv_, max_leaf, x, y);
+}
+
+void Engine::excitation(double pol, double th, double ph, double2* d_v) {
+    ExcitationArgs a;
+    double vth[3], vph[3];
+    plane_wave(pol, th, ph, a.kinc, a.einc, a.hinc, vth, vph);
+    a.k = d2(med.k1);
+    a.alpha = P.cfie_alpha;
+    launch_excitation(st, N, ie, d_geo_, d_perm_, gauss_, a, d_v);
+    MLFMA_HIP(hipGetLastError());
+    MLFMA_HIP(hipStreamSynchronize(st));
+}
+
+void Engine::rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out) {
+    std::vector<double> dirs(3 * (size_t)nang);
+    for (int a = 0; a < nang; ++a) {
+        const double t = th[a] / 180.0 * kPi, p = ph[a] / 180.0 * kPi;
+        dirs[3 * a] = std::sin(t) * std::cos(p);
+        dirs[3 * a + 1] = std::sin(t) * std::sin(p);
+        dirs[3 * a + 2] = std::cos(t);
+    }
+    double* d_dirs = nullptr;
+    double2* d_part = nullptr;
+    const int nblk = (N + 255) / 256;
+    MLFMA_HIP(hipMalloc(&d_dirs, dirs.size() * sizeof(double)));
+    MLFMA_HIP(hipMalloc(&d_part, (size_t)nang * nblk * 3 * sizeof(double2)));
+    MLFMA_HIP(hipMemcpy(d_dirs, dirs.data(), dirs.size() * sizeof(double), hipMemcpyHostToDevice));
+    launch_farfield(st, N, ie == MLFMA_PMCHWT, d_geo_, d_perm_, gauss_, d2(med.k1), nang, d_dirs, d_I, d_part, nblk);
+    std::vector<double2> part((size_t)nang * nblk * 3);
+    MLFMA_HIP(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+    MLFMA_HIP(hipStreamSynchronize(st));
+    hipFree(d_dirs);
+    hipFree(d_part);
+    // PEC: jk eta F.I (Code/RCS.h:109-112); PMCHWT: jk1 (F_J I_J - F_M I_M) (:186-190)
+    const cplx coeff = (ie == MLFMA_PMCHWT) ? cplx(0, 1) * med.k1 : cplx(0, 1) * med.k1 * med.eta1;
+    for (int a = 0; a < nang; ++a) {
+        cplx F[3] = {0, 0, 0};
+        for (int b = 0; b < nblk; ++b)
+            for (int c = 0; c < 3; ++c) F[c] += cx(part[((size_t)a * nblk + b) * 3 + c]);
+        for (int c = 0; c < 3; ++c) F[c] *= coeff;
+        double k[3], e[3], h[3], vth[3], vph[3];
+        plane_wave(pol, th[a], ph[a], k, e, h, vth, vph);
+        const cplx Ev = vth[0] * F[0] + vth[1] * F[1] + vth[2] * F[2];
+        const cplx Eh = vph[0] * F[0] + vph[1] * F[1] + vph[2] * F[2];
+        // Code/RCS.h:255-257
+        out[3 * a] = 10.0 * std::log10((std::norm(Ev) + std::norm(Eh)) / (4.0 * kPi));
+        out[3 * a + 1] = 10.0 * std::log10(std::norm(Ev) / (4.0 * kPi));
+        out[3 * a + 2] = 10.0 * std::log10(std::norm(Eh) / (4.0 * kPi));
+    }
+}
+
+void Engine::get_workspace(int wn, int cur, int which, int level, int node, double* out) {
+    if (wn < 1 || wn > nw || cur < 0 || cur >= ncur || level < 2 || level > tree.max_level || node < 0 ||
+        node >= tree.lv[level].n)
+        throw Error(MLFMA_ERR_ARG, "workspace index out of range");
+    const WaveDev& D = wd_[wn - 1];
+    const int K = D.K[level];
+    const double2* src = (which == 0 ? D.Sm[level] : D.Bm[level]) + ((size_t)node * ncur + cur) * 3 * K;
+    std::vector<double2> h(3 * (size_t)K);
+    MLFMA_HIP(hipMemcpy(h.data(), src, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int q = 0; q < K; ++q)
+        for (int c = 0; c < 3; ++c) {
+            out[6 * q + 2 * c] = h[(size_t)c * K + q].x;
+            out[6 * q + 2 * c + 1] = h[(size_t)c * K + q].y;
+        }
+}
+
+void Engine::get_pattern(int wn, int which, int rwg, double* out) {
+    if (wn < 1 || wn > nw || rwg < 0 || rwg >= N) throw Error(MLFMA_ERR_ARG, "pattern index out of range");
+    const WaveDev& D = wd_[wn - 1];
+    const int K = D.K[tree.max_level];
+    const int i = inv_perm[rwg];
+    std::vector<double2> h(3 * (size_t)K);
+    MLFMA_HIP(hipMemcpy(h.data(), (which == 0 ? D.Vs : D.Vf) + (size_t)i * 3 * K, h.size() * sizeof(double2),
+                        hipMemcpyDeviceToHost));
+    for (int q = 0; q < K; ++q)
+        for (int c = 0; c < 3; ++c) {
+            out[6 * q + 2 * c] = h[(size_t)c * K + q].x;
+            out[6 * q + 2 * c + 1] = h[(size_t)c * K + q].y;
+        }
+}
+
+void Engine::get_near_row(int row, int* len, int* cols, double* vals) {
+    if (row < 0 || row >= nunk) throw Error(MLFMA_ERR_ARG, "row out of range");
+    const bool pm = ie == MLFMA_PMCHWT;
+    const bool mrow = pm && row >= N;
+    const int i = inv_perm[mrow ? row - N : row];
+    const int t = leaf_of[i];
+    const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
+    const int rr = i - tree.leaf_ptr[t];
+    int cnt = 0;
+    for (int p = pair_ptr[t]; p < pair_ptr[t + 1]; ++p) {
+        const int s = pair_src[p];
+        const int j0 = tree.leaf_ptr[s], ns = tree.leaf_ptr[s + 1] - j0;
+        if (!cols) {
+            cnt += pm ? 2 * ns : ns;
+            continue;
+        }
+        const int planes = pm ? 3 : 1;
+        std::vector<double2> blk((size_t)planes * nt * ns);
+        MLFMA_HIP(hipMemcpy(blk.data(), d_Z_ + pair_off[p], blk.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        for (int j = 0; j < ns; ++j) {
+            const int sid = tree.perm[j0 + j];
+            auto put = [&](double2 v, int col) {
+                cols[cnt] = col;
+                vals[2 * cnt] = v.x;
+                vals[2 * cnt + 1] = v.y;
+                ++cnt;
+            };
+            const double2 a = blk[(size_t)j * nt + rr];
+            if (!pm) {
+                put(a, sid);
+            } else {
+                const double2 k = blk[(size_t)(ns + j) * nt + rr], c = blk[(size_t)(2 * ns + j) * nt + rr];
+                if (!mrow) {
+                    put(a, sid);
+                    put(make_double2(-k.x, -k.y), N + sid);
+                } else {
+                    put(k, sid);
+                    put(c, N + sid);
+                }
+            }
+        }
+    }
+    *len = cnt;
+}
+
+}  // namespace mlfma

@@ -1,0 +1,131 @@
+// Engine: one MLFMA operator resident on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mlfma.h"
+#include "geo.hpp"
+#include "model.hpp"
+
+namespace mlfma {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MLFMA_HIP(call)                                                                                     \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            throw ::mlfma::Error(e_ == hipErrorOutOfMemory ? MLFMA_ERR_OOM : MLFMA_ERR_HIP,                  \
+                                 std::string(#call) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+struct LevelDev {
+    int n = 0;
+    int* dr_ptr = nullptr;
+    int* dr_idx = nullptr;
+    int* dr_vec = nullptr;
+    int* son_ptr = nullptr;
+    int* son_idx = nullptr;
+    int* parent = nullptr;
+    int* oct = nullptr;   // Morton octant of each node inside its parent
+};
+
+struct WaveDev {
+    std::vector<int> K;                  // per tree level
+    double2* Vs = nullptr;               // [N][3][K0], internal RWG order
+    double2* Vf = nullptr;
+    double* kp0 = nullptr;               // [3][K0] finest-level directions
+    std::vector<double2*> Sm, Bm, TFw;   // per tree level
+    std::vector<int*> tap, t_ptr, t_kp;  // per child tree level
+    std::vector<double*> tw, t_w;
+    std::vector<double2*> sh_up, sh_dn;
+};
+
+class Engine {
+public:
+    explicit Engine(const mlfma_params& p);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void build(int np, const int* ids, const double* xyz, int nf, const int* fids, const int* fvid);
+
+    // y = A x for nrhs device vectors (column-major, n_unknowns each)
+    void apply_device(const double2* x, double2* y, int nrhs);
+    void stage(int which, const double2* dx, double2* dy);
+    void solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, mlfma_solve_stats& st);
+    void excitation(double pol, double th, double ph, double2* d_v);
+    void rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out);
+
+    // inspection
+    void get_workspace(int wn, int cur, int which, int level, int node, double* out);
+    void get_pattern(int wn, int which, int rwg, double* out);
+    void get_near_row(int row, int* len, int* cols, double* vals);
+
+    mlfma_params P;
+    int ie = 2, nw = 2, ncur = 2;
+    int N = 0, nunk = 0, nleaf = 0, max_leaf = 0;
+    Mesh mesh;
+    Basis basis;
+    Octree tree;
+    Medium med;
+    WaveTables W[2];
+    std::vector<int> inv_perm, leaf_of;
+    hipStream_t st = nullptr;
+    size_t dev_bytes = 0;
+    long long near_entries = 0;
+    std::string last_error;
+
+    // near-field block-sparse structure (host copies kept for inspection)
+    std::vector<int> pair_ptr, pair_tgt, pair_src;
+    std::vector<long long> pair_off;
+
+private:
+    template <typename T>
+    T* dalloc(size_t count);
+    template <typename T>
+    T* upload(const std::vector<T>& v);
+    void build_device_tree();
+    void build_wave(int w);
+    void build_near();
+    void far_pass(int w, const double2* x);
+    void apply_one(const double2* x, double2* y);
+    void precond_setup();
+    void precond_apply(const double2* x, double2* y);
+    double norm(const double2* v, int n);
+
+    std::vector<void*> allocs_;
+    std::vector<LevelDev> lv_;
+    WaveDev wd_[2];
+    int* d_perm_ = nullptr;
+    int* d_leaf_ptr_ = nullptr;
+    int* d_leaf_of_ = nullptr;
+    double* d_leaf_c_ = nullptr;
+    RwgGeo* d_geo_ = nullptr;
+    GaussRule gauss_{};
+    int* d_pair_ptr_ = nullptr;
+    int* d_pair_tgt_ = nullptr;
+    int* d_pair_src_ = nullptr;
+    long long* d_pair_off_ = nullptr;
+    double2* d_Z_ = nullptr;
+    // solver scratch
+    double2* d_V_ = nullptr;
+    int V_cols_ = 0;
+    double2* d_w_ = nullptr;
+    double2* d_r_ = nullptr;
+    double2* d_t_ = nullptr;
+    double2* d_part_ = nullptr;
+    double2* d_small_ = nullptr;
+    double* d_rpart_ = nullptr;
+    // block-diagonal preconditioner
+    double2* d_Dinv_ = nullptr;
+    long long* d_Doff_ = nullptr;
+};
+
+}  // namespace mlfma

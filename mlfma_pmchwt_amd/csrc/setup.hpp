@@ -1,0 +1,58 @@
+// Launch declarations of the FP64 setup / excitation / far-field kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "geo.hpp"
+
+namespace mlfma {
+
+struct NearConsts {
+    double2 k1, k2, eta1, etai, inv_etai;
+    double alpha;
+};
+
+struct ExcitationArgs {
+    double kinc[3], einc[3], hinc[3];
+    double2 k;
+    double alpha;
+};
+
+// mode 0 EFIE, 1 CFIE, 2 PMCHWT; one workgroup per leaf-pair block.
+void launch_near_fill(hipStream_t st, int mode, int npairs, const int* pair_tgt, const int* pair_src,
+                      const long long* pair_off, const int* leaf_ptr, const RwgGeo* geo, const GaussRule& g,
+                      const NearConsts& c, double2* Z);
+void launch_pair_probe(hipStream_t st, const RwgGeo* geo, int f, int s, const GaussRule& g, const NearConsts& c,
+                       double2* out);
+void launch_patterns(hipStream_t st, bool cfie, int N, int K, const RwgGeo* geo, const int* leaf_of,
+                     const double* centre, const double* kp, double2 k, const GaussRule& g, double alpha, double2* Vs,
+                     double2* Vf);
+void launch_excitation(hipStream_t st, int N, int mode, const RwgGeo* geo, const int* perm, const GaussRule& g,
+                       const ExcitationArgs& a, double2* v);
+void launch_farfield(hipStream_t st, int N, int pmchwt, const RwgGeo* geo, const int* perm, const GaussRule& g,
+                     double2 k, int nang, const double* dirs, const double2* I, double2* partial, int nblocks);
+
+// Krylov vector kernels (k_solve.hip)
+// partial[j*nblk + b] = sum over block b of conj(V_j) . w, j < m
+void launch_multidot(hipStream_t st, int n, int m, const double2* V, size_t ldv, const double2* w, double2* partial,
+                     int nblk);
+// out[j] = sum_b partial[j*nblk + b] (fixed order), j < m
+void launch_reduce_partials(hipStream_t st, int m, int nblk, const double2* partial, double2* out);
+// w -= sum_j h[j] V_j
+void launch_sub_combination(hipStream_t st, int n, int m, const double2* V, size_t ldv, const double2* h, double2* w);
+// x += sum_j y[j] V_j
+void launch_add_combination(hipStream_t st, int n, int m, const double2* V, size_t ldv, const double2* y, double2* x);
+// y = a * x (a real)
+void launch_scale_copy(hipStream_t st, int n, double a, const double2* x, double2* y);
+// r = b - r
+void launch_residual(hipStream_t st, int n, const double2* b, double2* r);
+// partial[b] = sum |x|^2 over block b
+void launch_norm2_partial(hipStream_t st, int n, const double2* x, double* partial, int nblk);
+// y = x + a * z   (complex a)
+void launch_axpy(hipStream_t st, int n, double2 a, const double2* z, const double2* x, double2* y);
+// sum_i a_i b_i (bilinear) partials
+void launch_bidot_partial(hipStream_t st, int n, const double2* a, const double2* b, double2* partial, int nblk);
+// block-diagonal preconditioner: y_t = D_t^{-1} x_t per leaf (dense inverse blocks)
+void launch_block_diag(hipStream_t st, int planes, int nleaf, int N, const int* leaf_ptr, const int* perm,
+                       const long long* inv_off, const double2* Dinv, int max_leaf, const double2* x, double2* y);
+
+}  // namespace mlfma

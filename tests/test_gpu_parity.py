@@ -1,0 +1,161 @@
+"""HIP product path (through the C ABI) against the CPU oracle on the same mesh.
+
+FP64 bar: every setup table bit-exact or to <= 1e-12 relative, the assembled
+matvec y = A x to <= 1e-10 relative (the north-star bar for the FP64 path is
+1e-6); solver runs reach the requested tolerance on the true residual.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr, seeded_x
+from mlfma_pmchwt_amd import api, geometry
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (mesh args, ie, eps)
+    "efie_ml": ((2, 1.2), api.EFIE, (1.0, 0.0)),
+    "cfie_ml": ((2, 1.2), api.CFIE, (1.0, 0.0)),
+    "pmchwt_1lvl": ((2, 0.5), api.PMCHWT, (4.0, 0.0)),
+    "pmchwt_lossy_ml": ((2, 1.2), api.PMCHWT, (2.5, -0.1)),
+}
+
+_cache = {}
+
+
+def pair(name):
+    if name not in _cache:
+        (s, r), ie, eps = CASES[name]
+        mesh = geometry.icosphere(s, r)
+        print(f"[case] {name}: icosphere s={s} r={r} ie={ie} eps={eps}", flush=True)
+        op = api.Operator(mesh, integral_equation=ie, eps_r_re=eps[0], eps_r_im=eps[1])
+        orc = pyoracle.Oracle(mesh, ie=ie, eps=eps)
+        _cache[name] = (mesh, op, orc)
+    return _cache[name]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_tree_and_basis_identical(name):
+    _, op, orc = pair(name)
+    assert (op.n_rwg, op.n, op.max_level) == (orc.n_rwg, orc.n, orc.max_level)
+    a, b = op.rwg(), orc.rwg()
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    for lvl in range(orc.max_level + 1):
+        pa, pb = op.level(lvl), orc.level(lvl)
+        assert np.array_equal(pa[0], pb[0]), f"codes level {lvl}"
+        assert np.array_equal(pa[1], pb[1]), f"centres level {lvl}"
+        for which in (0, 1, 2, 3, 4) if lvl >= 2 else (0, 3):
+            if which == 3 and lvl == orc.max_level:
+                continue
+            la, lb = op.level_list(lvl, which), orc.level_list(lvl, which)
+            assert np.array_equal(la[0], lb[0]) and np.array_equal(la[1], lb[1]), (lvl, which)
+        if lvl >= 2:
+            assert np.array_equal(op.drvec(lvl), orc.drvec(lvl))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_tables(name):
+    _, op, orc = pair(name)
+    nw = 2 if op.params.integral_equation == api.PMCHWT else 1
+    for wn in range(1, nw + 1):
+        for lvl in range(2, op.max_level + 1):
+            ga, gb = op.grid(wn, lvl), orc.grid(wn, lvl)
+            assert ga[0] == gb[0]
+            for u, v in zip(ga[1:4], gb[1:4]):
+                assert np.max(np.abs(u - v)) <= 1e-14
+            assert relerr(op.tf(wn, lvl), orc.tf(wn, lvl)) <= 1e-12
+            if lvl < op.max_level:
+                ta, tb = op.interp(wn, lvl), orc.interp(wn, lvl)
+                assert np.array_equal(ta[0], tb[0])
+                assert np.max(np.abs(ta[1] - tb[1])) <= 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_patterns(name):
+    _, op, orc = pair(name)
+    nw = 2 if op.params.integral_equation == api.PMCHWT else 1
+    for wn in range(1, nw + 1):
+        for rwg in (0, 7, op.n_rwg // 2, op.n_rwg - 1):
+            for which in (0, 1):
+                assert relerr(op.pattern(wn, which, rwg), orc.pattern(wn, which, rwg)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_near_rows(name):
+    _, op, orc = pair(name)
+    rows = [0, 3, op.n_rwg // 3, op.n_rwg - 1]
+    if op.n > op.n_rwg:
+        rows += [op.n_rwg, op.n - 1]
+    for r in rows:
+        ca, va = op.near_row(r)
+        cb, vb = orc.near_row(r)
+        assert np.array_equal(ca, cb)
+        assert relerr(va, vb) <= 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stages(name):
+    _, op, orc = pair(name)
+    x = seeded_x(op.n, 7)
+    op.stage(api.STAGE_AGGREGATE, x)
+    op.stage(api.STAGE_M2M)
+    op.stage(api.STAGE_M2L_L2L)
+    pm = op.params.integral_equation == api.PMCHWT
+    for wn in (1, 2) if pm else (1,):
+        for cur in (0, 1) if pm else (0,):
+            orc.stage_pass(wn, cur, x)
+            for lvl in range(2, op.max_level + 1):
+                n = op.level(lvl)[0].size
+                for node in sorted({0, n // 2, n - 1}):
+                    for which in (0, 1):
+                        a = op.workspace(wn, cur, which, lvl, node)
+                        b = orc.workspace(wn, cur, which, lvl, node)
+                        assert relerr(a, b) <= 1e-11, (wn, cur, which, lvl, node)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_matvec(name):
+    _, op, orc = pair(name)
+    for seed in (1, 2):
+        x = seeded_x(op.n, seed)
+        y = op.apply(x)
+        yo = orc.apply(x)
+        err = relerr(y, yo)
+        print(f"[matvec] {name} seed={seed} rel={err:.3e}")
+        assert err <= 1e-10
+    ones = np.ones(op.n, dtype=np.complex128)
+    assert relerr(op.apply(ones), orc.apply(ones)) <= 1e-10
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_excitation(name):
+    _, op, orc = pair(name)
+    for pol, th, ph in ((0.0, 0.0, 0.0), (90.0, 37.0, 120.0)):
+        assert relerr(op.excitation(pol, th, ph), orc.rhs(pol, th, ph)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["efie_ml", "pmchwt_1lvl"])
+def test_gmres_and_rcs(name):
+    _, op, orc = pair(name)
+    b = op.excitation(0.0, 0.0, 0.0)
+    x, st = op.solve(b, tol_rel=1e-6, restart=30)
+    assert st["converged"] and st["rel_residual"] < 1e-6
+    xo, sto = orc.gmres(b, tol_rel=1e-6, restart=30)
+    assert relerr(x, xo) <= 1e-5
+    # RCS of the same currents: product vs oracle
+    for th in (0.0, 60.0, 180.0):
+        a = op.rcs(x, th, 0.0)[0]
+        o, _ = orc.rcs(x, th, 0.0)
+        assert np.max(np.abs(a - o)) <= 1e-9
+
+
+def test_cgs_and_block_diag(sphere_small):
+    op = api.Operator(sphere_small, integral_equation=api.CFIE)
+    b = op.excitation(0.0, 0.0, 0.0)
+    x1, s1 = op.solve(b, solver=api.CGS, tol_rel=1e-6)
+    assert s1["converged"] and s1["rel_residual"] < 1e-6
+    x2, s2 = op.solve(b, tol_rel=1e-6, precond=1)
+    assert s2["converged"] and s2["rel_residual"] < 1e-6
+    assert relerr(x1, x2) < 1e-4
