@@ -158,6 +158,18 @@ int mlfma_stage(mlfma_handle* h, int stage, const double* x, double* y);
  * (Morton order): K x (x, y, z) complex. */
 int mlfma_get_workspace(mlfma_handle* h, int wn, int cur, int which, int level, int node, double* out);
 
+/* ------------------------------------------------------------------ profiling
+ * No reference counterpart (the reference has no per-matvec timing).  When
+ * enabled, every matvec brackets its kernel groups with hipEvents on the
+ * handle's stream and accumulates their durations:
+ *   group 0 aggregation, 1 interpolation (m2m), 2 translation + anterpolation,
+ *   3 near-field product, 4 disaggregation.
+ * mlfma_get_profile fills ms[5] (summed milliseconds), launches[5] (kernel
+ * launches counted) and returns the number of matvecs profiled in *napply. */
+#define MLFMA_PROFILE_GROUPS 5
+int mlfma_set_profiling(mlfma_handle* h, int enable);
+int mlfma_get_profile(mlfma_handle* h, double* ms, long long* launches, int* napply);
+
 /* ------------------------------------------------------------------ inspection
  * Read-back of the setup products in the reference's layouts, for parity
  * tests.  Counts first (pass NULL buffers), then data. */

@@ -69,6 +69,8 @@ def library() -> C.CDLL:
         lib.mlfma_rcs.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
         lib.mlfma_stage.argtypes = [vp, C.c_int, vp, vp]
         lib.mlfma_get_workspace.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        lib.mlfma_set_profiling.argtypes = [vp, C.c_int]
+        lib.mlfma_get_profile.argtypes = [vp, vp, vp, vp]
         lib.mlfma_get_rwg.argtypes = [vp, vp, vp, vp]
         lib.mlfma_get_level.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         lib.mlfma_get_level_list.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
@@ -179,6 +181,21 @@ class Operator:
         xp = None if x is None else np.ascontiguousarray(x, dtype=np.complex128)
         self._check(library().mlfma_stage(self._h, which, None if xp is None else _p(xp), _p(y)))
         return y
+
+    # ---------------------------------------------------------------- profiling
+    PROFILE_GROUPS = ("aggregate", "interpolate", "translate_anterpolate", "near", "disaggregate")
+
+    def set_profiling(self, on: bool = True):
+        self._check(library().mlfma_set_profiling(self._h, int(on)))
+
+    def profile(self) -> dict:
+        ms = np.zeros(5)
+        launches = np.zeros(5, dtype=np.int64)
+        n = C.c_int()
+        self._check(library().mlfma_get_profile(self._h, _p(ms), _p(launches), C.byref(n)))
+        return {"matvecs": n.value,
+                "groups": {g: {"ms": float(ms[i]), "launches": int(launches[i])}
+                           for i, g in enumerate(self.PROFILE_GROUPS)}}
 
     # ---------------------------------------------------------------- inspection
     def grid(self, wn: int, level: int):

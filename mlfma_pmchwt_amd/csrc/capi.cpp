@@ -230,6 +230,22 @@ int mlfma_get_workspace(mlfma_handle* h, int wn, int cur, int which, int level, 
     return guarded(h, [&] { h->eng->get_workspace(wn, cur, which, level, node, out); });
 }
 
+int mlfma_set_profiling(mlfma_handle* h, int enable) {
+    if (!h) return MLFMA_ERR_ARG;
+    return guarded(h, [&] { h->eng->set_profiling(enable != 0); });
+}
+
+int mlfma_get_profile(mlfma_handle* h, double* ms, long long* launches, int* napply) {
+    if (!h) return MLFMA_ERR_ARG;
+    const mlfma::Engine& e = *h->eng;
+    for (int g = 0; g < MLFMA_PROFILE_GROUPS; ++g) {
+        if (ms) ms[g] = e.prof_ms[g];
+        if (launches) launches[g] = e.prof_launch[g];
+    }
+    if (napply) *napply = e.prof_apply;
+    return MLFMA_OK;
+}
+
 int mlfma_get_rwg(mlfma_handle* h, int* ints6, double* len, double* mid3) {
     if (!h) return MLFMA_ERR_ARG;
     const mlfma::Basis& b = h->eng->basis;

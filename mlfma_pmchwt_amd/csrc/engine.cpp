@@ -64,6 +64,8 @@ Engine::Engine(const mlfma_params& p) : P(p) {
 
 Engine::~Engine() {
     if (st) hipStreamSynchronize(st);
+    for (hipEvent_t e : ev_)
+        if (e) hipEventDestroy(e);
     for (void* a : allocs_) hipFree(a);
     if (st) hipStreamDestroy(st);
 }
@@ -280,23 +282,67 @@ void Engine::build_near() {
 void Engine::far_pass(int w, const double2* x) {
     WaveDev& D = wd_[w];
     const int L = tree.max_level;
+    mark(6 * w + 0);
     launch_aggregate(st, ncur, nleaf, D.K[L], N, d_leaf_ptr_, d_perm_, D.Vs, x, D.Sm[L]);
+    mark(6 * w + 1);
+    mark(6 * w + 2);
     for (int l = L - 1; l >= 2; --l)
         launch_m2m(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
                    D.tap[l + 1], D.tw[l + 1], D.sh_up[l + 1], D.Sm[l + 1], D.Sm[l]);
+    mark(6 * w + 3);
+    mark(6 * w + 4);
     for (int l = 2; l <= L; ++l) {
         const bool up = l > 2;
         launch_m2l_l2l(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, D.TFw[l], D.Sm[l],
                        up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l], D.t_w[l], D.sh_dn[l],
                        up ? D.Bm[l - 1] : nullptr, D.Bm[l]);
     }
+    mark(6 * w + 5);
+}
+
+void Engine::mark(int slot) {
+    if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[slot], st));
+}
+
+void Engine::set_profiling(bool on) {
+    if (on && !ev_[0])
+        for (hipEvent_t& e : ev_) MLFMA_HIP(hipEventCreate(&e));
+    prof_on_ = on;
+    for (int g = 0; g < 5; ++g) prof_ms[g] = 0, prof_launch[g] = 0;
+    prof_apply = 0;
+}
+
+void Engine::collect_profile() {
+    MLFMA_HIP(hipEventSynchronize(ev_[15]));
+    auto span = [&](int a) {
+        float ms = 0;
+        MLFMA_HIP(hipEventElapsedTime(&ms, ev_[a], ev_[a + 1]));
+        return (double)ms;
+    };
+    const int L = tree.max_level;
+    for (int w = 0; w < nw; ++w) {
+        prof_ms[0] += span(6 * w);
+        prof_ms[1] += span(6 * w + 2);
+        prof_ms[2] += span(6 * w + 4);
+        prof_launch[0] += 1;
+        prof_launch[1] += L - 2;
+        prof_launch[2] += L - 1;
+    }
+    prof_ms[3] += span(12);
+    prof_ms[4] += span(14);
+    prof_launch[3] += 1;
+    prof_launch[4] += 1;
+    prof_apply += 1;
 }
 
 void Engine::apply_one(const double2* x, double2* y) {
     const int L = tree.max_level;
     for (int w = 0; w < nw; ++w) far_pass(w, x);
+    mark(12);
     launch_near(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_pair_ptr_, d_pair_src_, d_pair_off_,
                 d_Z_, max_leaf, x, y);
+    mark(13);
+    mark(14);
     if (ie != MLFMA_PMCHWT) {
         // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
         const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
@@ -318,10 +364,14 @@ void Engine::apply_one(const double2* x, double2* y) {
         a.inv_etai = d2(1.0 / med.etai);
         launch_disagg_pmchwt(st, nleaf, d_leaf_ptr_, d_perm_, a, y);
     }
+    mark(15);
 }
 
 void Engine::apply_device(const double2* x, double2* y, int nrhs) {
-    for (int r = 0; r < nrhs; ++r) apply_one(x + (size_t)r * nunk, y + (size_t)r * nunk);
+    for (int r = 0; r < nrhs; ++r) {
+        apply_one(x + (size_t)r * nunk, y + (size_t)r * nunk);
+        if (prof_on_) collect_profile();
+    }
     MLFMA_HIP(hipGetLastError());
 }
 

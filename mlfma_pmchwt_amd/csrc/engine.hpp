@@ -63,6 +63,12 @@ public:
     void excitation(double pol, double th, double ph, double2* d_v);
     void rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out);
 
+    // per-group kernel timing with hipEvents on st
+    void set_profiling(bool on);
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    long long prof_launch[5] = {0, 0, 0, 0, 0};
+    int prof_apply = 0;
+
     // inspection
     void get_workspace(int wn, int cur, int which, int level, int node, double* out);
     void get_pattern(int wn, int which, int rwg, double* out);
@@ -95,6 +101,8 @@ private:
     void build_wave(int w);
     void build_near();
     void far_pass(int w, const double2* x);
+    void mark(int slot);
+    void collect_profile();
     void apply_one(const double2* x, double2* y);
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
@@ -123,6 +131,9 @@ private:
     double2* d_part_ = nullptr;
     double2* d_small_ = nullptr;
     double* d_rpart_ = nullptr;
+    // profiling events: per wave 3 groups x 2, near 2, far 2
+    bool prof_on_ = false;
+    hipEvent_t ev_[16] = {};
     // block-diagonal preconditioner
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;

@@ -316,8 +316,8 @@ __global__ __launch_bounds__(256) void k_patterns(int K, const RwgGeo* __restric
                                                   const double* __restrict__ centre, const double* __restrict__ kp,
                                                   double2 k, GaussRule g, double alpha, double2* __restrict__ Vs,
                                                   double2* __restrict__ Vf) {
-    const int i = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
+    const int q = blockIdx.y * blockDim.x + threadIdx.x;
     if (q >= K) return;
     const RwgGeo& G = geo[i];
     const int lf = leaf_of[i];
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void k_patterns(int K, const RwgGeo* __restric
 void launch_patterns(hipStream_t st, bool cfie, int N, int K, const RwgGeo* geo, const int* leaf_of,
                      const double* centre, const double* kp, double2 k, const GaussRule& g, double alpha, double2* Vs,
                      double2* Vf) {
-    dim3 grid((K + 255) / 256, N);
+    dim3 grid(N, (K + 255) / 256);   // gridDim.y is limited to 65535
     if (cfie) k_patterns<true><<<grid, 256, 0, st>>>(K, geo, leaf_of, centre, kp, k, g, alpha, Vs, Vf);
     else k_patterns<false><<<grid, 256, 0, st>>>(K, geo, leaf_of, centre, kp, k, g, alpha, Vs, Vf);
 }

@@ -129,5 +129,6 @@ cd op_L(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp);
 cd op_K(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp);
 cd op_EFIE(const Mesh& m, const Rwg& f, const Rwg& s, cd k, cd eta, const Gauss& gp);
 cd op_MFIE(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp);
+void sing_probe(const double* tri9, const double* n3, const double* r3, double* out);
 
 }  // namespace orc

@@ -1,5 +1,7 @@
 // ORACLE -- test infrastructure only (see orc.hpp).
 // Flat C entry points so tests can drive the oracle through ctypes.
+#include <omp.h>
+
 #include <cstring>
 #include <memory>
 
@@ -339,6 +341,8 @@ void orc_sph_jy(int L, double x, double* j, double* y) {
     std::memcpy(j, a.data(), (L + 1) * sizeof(double));
     std::memcpy(y, b.data(), (L + 1) * sizeof(double));
 }
+void orc_sing(const double* tri9, const double* n3, const double* r3, double* out) { sing_probe(tri9, n3, r3, out); }
+int orc_threads() { return omp_get_max_threads(); }
 double orc_legendre_p(int l, double x) { return legendre_p(l, x); }
 double orc_legendre_p_prime(int l, double x) { return legendre_p_prime(l, x); }
 // out: kinc, einc, hinc, vth, vph

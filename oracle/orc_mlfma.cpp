@@ -314,41 +314,59 @@ void Oracle::build_near() {
 }
 
 // Values of the right shape without the setup cost; used only to time the CPU
-// matvec on workloads whose FP64 setup would take hours on the host.
+// matvec on workloads whose FP64 setup would take hours on the host (the
+// matvec's running time does not depend on the values).
 void Oracle::fill_synthetic(unsigned seed) {
-    std::mt19937_64 rng(seed);
-    std::uniform_real_distribution<double> U(-1.0, 1.0);
-    auto fillP = [&](std::vector<std::vector<C3>>& V, int K) {
-        V.assign(N, std::vector<C3>(K));
-        for (auto& row : V)
-            for (auto& c : row) c = {cd(U(rng), U(rng)), cd(U(rng), U(rng)), cd(U(rng), U(rng))};
+    auto val = [seed](uint64_t a, uint64_t b) {
+        uint64_t h = (a * 0x9E3779B97F4A7C15ULL) ^ (b + 0x632BE59BD9B4E019ULL + seed);
+        h ^= h >> 31;
+        h *= 0xD6E8FEB86659FD93ULL;
+        h ^= h >> 32;
+        return cd((double)(h & 0xffff) / 32768.0 - 1.0, (double)((h >> 16) & 0xffff) / 32768.0 - 1.0);
     };
-    fillP(Vs1, (int)g1[0].kp.size());
-    fillP(Vf1, (int)g1[0].kp.size());
+    auto fillP = [&](std::vector<std::vector<C3>>& V, int K, uint64_t tag) {
+        V.assign(N, {});
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < N; ++i) {
+            V[i].resize(K);
+            for (int q = 0; q < K; ++q) {
+                uint64_t a = ((uint64_t)i << 20) ^ (uint64_t)q ^ (tag << 56);
+                V[i][q] = {val(a, 1), val(a, 2), val(a, 3)};
+            }
+        }
+    };
+    fillP(Vs1, (int)g1[0].kp.size(), 1);
+    fillP(Vf1, (int)g1[0].kp.size(), 2);
     if (ie == 2) {
-        fillP(Vs2, (int)g2[0].kp.size());
-        fillP(Vf2, (int)g2[0].kp.size());
+        fillP(Vs2, (int)g2[0].kp.size(), 3);
+        fillP(Vf2, (int)g2[0].kp.size(), 4);
     }
     const auto& leaves = tree.lv[tree.maxLevel];
     int rows = unknowns();
     Zn.assign(rows, {});
     Zid.assign(rows, {});
-    for (const Node& nd : leaves) {
+    const int h2 = (ie == 2) ? 2 : 1;
+#pragma omp parallel for schedule(dynamic)
+    for (int li = 0; li < (int)leaves.size(); ++li) {
+        const Node& nd = leaves[li];
         std::vector<int> cols;
         for (int nb : nd.nearn)
             for (int s : leaves[nb].rwg) cols.push_back(s);
         for (int s : nd.rwg) cols.push_back(s);
         for (int id : nd.rwg) {
-            int rl[2] = {id, N + id};
-            for (int q = 0; q < (ie == 2 ? 2 : 1); ++q) {
-                auto& z = Zn[rl[q]];
-                auto& zi = Zid[rl[q]];
-                for (int s : cols) {
-                    for (int h = 0; h < (ie == 2 ? 2 : 1); ++h) {
-                        z.push_back(cd(U(rng), U(rng)));
-                        zi.push_back(s + h * N);
+            for (int q = 0; q < h2; ++q) {
+                int row = id + q * N;
+                auto& z = Zn[row];
+                auto& zi = Zid[row];
+                z.resize(cols.size() * h2);
+                zi.resize(cols.size() * h2);
+                size_t e = 0;
+                for (int s : cols)
+                    for (int h = 0; h < h2; ++h) {
+                        z[e] = val(((uint64_t)row << 24) ^ e, 7);
+                        zi[e] = s + h * N;
+                        ++e;
                     }
-                }
             }
         }
     }

@@ -398,3 +398,23 @@ cd op_MFIE(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp) {
 }
 
 }  // namespace orc
+
+namespace orc {
+// Analytic triangle integrals at one observation point (test access):
+// out = Ivec(3), Isca, Ivec3(3), Isca3, rho(3).
+void sing_probe(const double* tri9, const double* n3, const double* r3, double* out) {
+    Tri S;
+    S.v1 = {tri9[0], tri9[1], tri9[2]};
+    S.v2 = {tri9[3], tri9[4], tri9[5]};
+    S.v3 = {tri9[6], tri9[7], tri9[8]};
+    S.n = {n3[0], n3[1], n3[2]};
+    S.vf = S.v1;
+    S.area = 0;
+    S.face = 0;
+    SingFrame fr = sing_frame(S);
+    SingTerms st = sing_terms({r3[0], r3[1], r3[2]}, S, fr, true);
+    const double o[11] = {st.Ivec.x, st.Ivec.y, st.Ivec.z, st.Isca, st.Ivec3.x, st.Ivec3.y, st.Ivec3.z, st.Isca3,
+                          st.rho.x, st.rho.y, st.rho.z};
+    for (int i = 0; i < 11; ++i) out[i] = o[i];
+}
+}  // namespace orc

@@ -72,6 +72,8 @@ def library() -> C.CDLL:
         lib.orc_legendre_p_prime.argtypes = [ci, cd]
         lib.orc_legendre_p_prime.restype = cd
         lib.orc_plane_wave.argtypes = [cd, cd, cd, vp]
+        lib.orc_sing.argtypes = [vp, vp, vp, vp]
+        lib.orc_threads.restype = ci
         _lib = lib
     return _lib
 
@@ -254,6 +256,16 @@ def plane_wave(pol, th, ph):
     out = np.empty(15)
     library().orc_plane_wave(pol, th, ph, _p(out))
     return out.reshape(5, 3)
+
+
+def sing(tri, normal, r):
+    """Analytic integrals over triangle `tri` (3x3) seen from r: Ivec, Isca, Ivec3, Isca3, rho."""
+    t = np.ascontiguousarray(tri, dtype=np.float64).reshape(9)
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    p = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty(11)
+    library().orc_sing(_p(t), _p(n), _p(p), _p(out))
+    return out[0:3], out[3], out[4:7], out[7], out[8:11]
 
 
 def harness(*args) -> dict:

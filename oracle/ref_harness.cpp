@@ -13,6 +13,7 @@
 //   ref_harness emsource
 //   ref_harness stages <nas> <ie> <eps_re> <eps_im> <seed>
 //   ref_harness solvers <nas> <ie> <eps_re> <eps_im>
+//   ref_harness solution <nas> <ie> <eps_re> <eps_im> <out.bin>
 #include "PhyConstant.h"
 #include "MyStruct.h"
 #include "OCTree.h"
@@ -312,6 +313,30 @@ int cmd_solvers(const char* nas, int ie, double er, double ei) {
     return 0;
 }
 
+// Reference GMRES (Code/GMRES.h) solution of the EFIE/CFIE/PMCHWT system built
+// by the oracle, written as raw complex128: rhs then x.
+int cmd_solution(const char* nas, int ie, double er, double ei, const char* out) {
+    orc::Oracle* o = make_oracle(nas, ie, er, ei, 3);
+    if (!o) return 2;
+    int n = o->unknowns();
+    orc::PlaneWave pw = orc::init_pw(0.0, 0.0, 0.0);
+    std::vector<cdd> b;
+    if (ie == 0) orc::rhs_E(*o, o->wave.k1, pw.kinc, pw.einc, b);
+    else if (ie == 1) orc::rhs_C(*o, o->wave.k1, pw.kinc, pw.einc, pw.hinc, o->alpha, b);
+    else orc::rhs_PMCHWT(*o, o->wave.k1, pw.kinc, pw.einc, pw.hinc, b);
+    auto ax = [o](int, const cdd* a, cdd* w) { o->apply(a, w); };
+    std::vector<cdd> x(n, cdd(0, 0));
+    bool c = MGMRES::mgmres(n, x.data(), b.data(), 4 * n, 30, 1e-12, 1e-6, ax, false);
+    FILE* f = std::fopen(out, "wb");
+    if (!f) return 3;
+    std::fwrite(b.data(), sizeof(cdd), n, f);
+    std::fwrite(x.data(), sizeof(cdd), n, f);
+    std::fclose(f);
+    std::printf("{\"cmd\":\"solution\",\"n\":%d,\"converged\":%d}\n", n, (int)c);
+    delete o;
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -322,6 +347,8 @@ int main(int argc, char** argv) {
     if (c == "stages" && argc >= 7)
         return cmd_stages(argv[2], std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]),
                           (unsigned)std::atoi(argv[6]));
+    if (c == "solution" && argc >= 7)
+        return cmd_solution(argv[2], std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]), argv[6]);
     if (c == "solvers" && argc >= 6) return cmd_solvers(argv[2], std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]));
     return 1;
 }

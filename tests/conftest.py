@@ -40,8 +40,8 @@ def sphere_small():
 
 @pytest.fixture(scope="session")
 def sphere_ml():
-    """480-RWG icosphere, radius 1.2 lambda (tree levels 2..4)."""
-    return geometry.icosphere(2, 1.2)
+    """480-RWG icosphere, radius 1.5 lambda (tree depth 4: translations on levels 2..4)."""
+    return geometry.icosphere(2, 1.5)
 
 
 def seeded_x(n, seed=12345):

@@ -1,0 +1,51 @@
+"""Write the reference-derived fixtures of tests/golden/.
+
+Runs oracle/_ref/ref_harness (built from the reference's own Boost-free units,
+oracle/Makefile `ref`) on meshes produced by mlfma_pmchwt_amd.geometry and
+stores its outputs: bit-exactness counts, stage and solver agreement, and one
+reference-GMRES solution vector.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+import json
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+import pyoracle  # noqa: E402
+from mlfma_pmchwt_amd import geometry  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def main():
+    if not pyoracle.HARNESS.exists():
+        pyoracle.build(ref=True)
+    with tempfile.TemporaryDirectory() as d:
+        s2 = Path(d) / "s2.nas"
+        ml = Path(d) / "ml.nas"
+        geometry.write_nas(s2, geometry.icosphere(2, 0.5))
+        geometry.write_nas(ml, geometry.icosphere(2, 1.2))
+        res = {"gauss": pyoracle.harness("gauss"), "emsource": pyoracle.harness("emsource"), "stages": [],
+               "solvers": []}
+        for ie, er, ei in ((0, 1.0, 0.0), (1, 1.0, 0.0), (2, 4.0, 0.0), (2, 2.5, -0.1)):
+            res["stages"].append(pyoracle.harness("stages", ml, ie, er, ei, 11))
+        for ie in (0, 1):
+            res["solvers"].append(pyoracle.harness("solvers", s2, ie, 1.0, 0.0))
+        (OUT / "reference_checks.json").write_text(json.dumps(res, indent=1) + "\n")
+        binf = Path(d) / "sol.bin"
+        pyoracle.harness("solution", s2, 0, 1.0, 0.0, binf)
+        v = np.fromfile(binf, dtype=np.complex128)
+        n = v.size // 2
+        np.savez_compressed(OUT / "gmres_efie_s2.npz", rhs=v[:n], x_ref=v[n:], restart=30, tol=1e-6)
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,10 +15,10 @@ pytestmark = pytest.mark.gpu
 
 CASES = {
     # name: (mesh args, ie, eps)
-    "efie_ml": ((2, 1.2), api.EFIE, (1.0, 0.0)),
-    "cfie_ml": ((2, 1.2), api.CFIE, (1.0, 0.0)),
+    "efie_ml": ((2, 1.5), api.EFIE, (1.0, 0.0)),
+    "cfie_ml": ((2, 1.5), api.CFIE, (1.0, 0.0)),
     "pmchwt_1lvl": ((2, 0.5), api.PMCHWT, (4.0, 0.0)),
-    "pmchwt_lossy_ml": ((2, 1.2), api.PMCHWT, (2.5, -0.1)),
+    "pmchwt_lossy_ml": ((2, 1.5), api.PMCHWT, (2.5, -0.1)),
 }
 
 _cache = {}

@@ -1,0 +1,255 @@
+"""The oracle pinned by mathematics: closed forms, library routines, brute force.
+
+Each check is chosen so that a plausible slip (a dropped term, a wrong sign or
+index, a transposed operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+import pyoracle
+from conftest import relerr
+from mlfma_pmchwt_amd import geometry
+
+
+# ------------------------------------------------------------------ quadrature
+@pytest.mark.parametrize("n,degree", [(1, 1), (3, 2), (4, 3), (6, 4), (7, 5), (12, 6), (13, 7), (16, 8)])
+def test_dunavant_moments(n, degree):
+    """sum w l2^a l3^b = a! b! / (a+b+2)!  (reference triangle of area 1/2).
+
+    The reference's 13-point centroid weight is -0.074785044733841
+    (Code/GaussPoints.cpp), where Dunavant's rule gives -0.149570044467682 / 2
+    = -0.074785022233841; the oracle keeps the reference's value (bit-exact
+    against the compiled reference), so that rule is exact only to 2.25e-8."""
+    l1, l2, l3, w = pyoracle.gauss(n)
+    tol = 3e-8 if n == 13 else 2e-12
+    if n == 13:
+        assert w[0] == -0.074785044733841
+        assert abs((w[0] - (-0.149570044467682 / 2)) - (w.sum() - 0.5)) < 1e-15
+    assert abs(w.sum() - 0.5) < tol
+    assert np.allclose(l1 + l2 + l3, 1.0, atol=1e-11)
+    for a in range(degree + 1):
+        for b in range(degree + 1 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(w * l2 ** a * l3 ** b) - exact) < tol, (a, b)
+
+
+@pytest.mark.parametrize("n", [3, 7, 13, 28, 63, 120])
+def test_gauss_legendre_vs_numpy(n):
+    x, w = pyoracle.gauss_legendre(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.max(np.abs(x - xr)) < 1e-14
+    assert np.max(np.abs(w - wr)) < 1e-13
+
+
+def test_legendre_vs_scipy():
+    lib = pyoracle.library()
+    for l in (0, 1, 2, 5, 17, 40, 90):
+        for x in (-1.0, -0.731, 0.0, 0.25, 0.999, 1.0):
+            assert abs(lib.orc_legendre_p(l, x) - special.eval_legendre(l, x)) < 1e-13
+            if abs(x) < 1 and l > 0:
+                # P'_l(x) = l (x P_l - P_{l-1}) / (x^2 - 1)
+                d = l * (x * special.eval_legendre(l, x) - special.eval_legendre(l - 1, x)) / (x * x - 1)
+                assert abs(lib.orc_legendre_p_prime(l, x) - d) < 1e-10 * max(1.0, abs(d))
+
+
+@pytest.mark.parametrize("x", [0.7, 3.8, 9.8, 31.0, 120.0, 300.0])
+def test_spherical_bessel_vs_scipy(x):
+    L = int(x) + 25
+    j, y = pyoracle.sph_jy(L, x)
+    ls = np.arange(L + 1)
+    jr, yr = special.spherical_jn(ls, x), special.spherical_yn(ls, x)
+    scale = np.maximum(np.abs(jr), 1e-300)
+    ok = np.abs(jr) > 1e-250
+    assert np.max(np.abs(j - jr)[ok] / scale[ok]) < 1e-11
+    fin = np.isfinite(yr) & (np.abs(yr) < 1e250)
+    assert np.max(np.abs(y - yr)[fin] / np.abs(yr[fin])) < 1e-11
+
+
+# ------------------------------------------------------------------ plane wave
+@pytest.mark.parametrize("pol", [0.0, 90.0])
+@pytest.mark.parametrize("th,ph", [(0.0, 0.0), (30.0, 45.0), (90.0, 0.0), (137.5, 270.0)])
+def test_plane_wave_frame(pol, th, ph):
+    k, e, h, vth, vph = pyoracle.plane_wave(pol, th, ph)
+    t, p = np.radians(th), np.radians(ph)
+    rhat = np.array([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)])
+    assert np.allclose(k, -rhat, atol=1e-6)
+    assert abs(np.dot(k, e)) < 1e-6 and abs(np.linalg.norm(e) - 1) < 1e-6
+    assert np.allclose(h, np.cross(k, e) / np.linalg.norm(np.cross(k, e)), atol=1e-6)
+    assert np.allclose(e, vth if pol == 0 else vph)
+    if (th, ph) == (0.0, 0.0) and pol == 0:
+        assert np.array_equal(e, [-1.0, 0.0, 0.0]) and np.array_equal(k, [0.0, 0.0, -1.0])
+
+
+# ------------------------------------------------------------------ mesh / RWG / octree
+def test_rwg_structure():
+    m = geometry.icosphere(2, 0.7)
+    o = pyoracle.Oracle(m, ie=0, mode=0)
+    ints, ln, mid = o.rwg()
+    assert o.n_rwg == 3 * m.n_faces // 2
+    fpos, fneg, e0, e1, vp, vn = ints.T
+    assert np.all(fpos < fneg)
+    ids = m.point_ids
+    keys = list(zip(ids[e0], ids[e1]))
+    assert all(a < b for a, b in keys) and keys == sorted(keys)
+    tri = m.faces   # vertex ids
+    for r in range(0, o.n_rwg, 7):
+        for f, v in ((fpos[r], vp[r]), (fneg[r], vn[r])):
+            fv = set(tri[f])
+            assert {ids[e0[r]], ids[e1[r]]} <= fv and ids[v] in fv and ids[v] not in (ids[e0[r]], ids[e1[r]])
+        assert abs(ln[r] - np.linalg.norm(m.xyz[e0[r]] - m.xyz[e1[r]])) < 1e-15
+        assert np.allclose(mid[r], 0.5 * (m.xyz[e0[r]] + m.xyz[e1[r]]))
+
+
+def test_octree_invariants():
+    """Brute-force restatement of the self-check of Code/OCTree.cpp:553-640."""
+    m = geometry.icosphere(3, 1.6)
+    o = pyoracle.Oracle(m, ie=0, mode=0)
+    L = o.max_level
+    _, _, mid = o.rwg()
+    codes, cen, ln, _ = o.level(L)
+    ptr, idx = o.level_list(L, 4)
+    assert np.array_equal(np.sort(idx), np.arange(o.n_rwg))
+    for t in range(codes.size):
+        inside = np.all(np.abs(mid[idx[ptr[t]:ptr[t + 1]]] - cen[t]) <= 0.5 * ln[t] * (1 + 1e-9), axis=1)
+        assert inside.all()
+    assert np.all(np.diff(codes.astype(np.int64)) > 0)
+    for lvl in range(2, L + 1):
+        codes, cen, ln, par = o.level(lvl)
+        pcen = o.level(lvl - 1)[1]
+        near = o.level_list(lvl, 0)
+        dr = o.level_list(lvl, 1)
+        drv = o.level_list(lvl, 2)
+        vecs = o.drvec(lvl)
+        n = codes.size
+        for a in range(n):
+            d = np.max(np.abs(cen - cen[a]), axis=1) <= 1.01 * ln[a]
+            d[a] = False
+            truth_near = set(np.nonzero(d)[0])
+            pd = np.max(np.abs(pcen[par] - pcen[par[a]]), axis=1) <= 1.01 * 2 * ln[a]
+            truth_dr = set(np.nonzero(~d & pd)[0]) - {a}
+            assert set(near[1][near[0][a]:near[0][a + 1]]) == truth_near
+            got = dr[1][dr[0][a]:dr[0][a + 1]]
+            assert set(got) == truth_dr and len(got) == len(truth_dr)
+            for j, v in zip(got, drv[1][dr[0][a]:dr[0][a + 1]]):
+                assert np.allclose(vecs[v], cen[a] - cen[j], atol=1e-9)
+
+
+# ------------------------------------------------------------------ translation operators
+def test_tf_closed_form():
+    """TF = sum_l (-j)^l (2l+1) h2_l(|kR|) P_l(Rhat.khat), L' = min(L, ceil|kR|), via SciPy."""
+    m = geometry.icosphere(2, 1.2)
+    o = pyoracle.Oracle(m, ie=2, eps=(2.5, -0.1), mode=0)
+    for wn, k in ((1, o.wave()["k1"]), (2, o.wave()["k2"])):
+        lvl = o.max_level
+        L, _, _, _, _, kp = o.grid(wn, lvl)
+        vecs = o.drvec(lvl)
+        tf = o.tf(wn, lvl)
+        for v in range(0, vecs.shape[0], 17):
+            R = np.linalg.norm(vecs[v])
+            x = np.abs(k * R)
+            Lv = min(L, math.ceil(x))
+            ls = np.arange(Lv + 1)
+            h2 = special.spherical_jn(ls, x) - 1j * special.spherical_yn(ls, x)
+            P = special.eval_legendre(ls[:, None], (kp @ (vecs[v] / R))[None, :])
+            ref = np.sum(((-1j) ** ls * (2 * ls + 1) * h2)[:, None] * P, axis=0)
+            assert relerr(tf[v], ref) < 1e-12
+
+
+def test_addition_theorem():
+    """With the full series, (-jk/4pi) sum_q w_q e^{-jk khat.d} T(khat, X) = e^{-jk|X+d|}/|X+d|.
+
+    This fixes the sign conventions of TF against the e^{-jkR}/R kernel of the
+    near field (Code/Operator.h:803)."""
+    m = geometry.icosphere(2, 1.2)
+    o = pyoracle.Oracle(m, ie=0, tf_full=1, mode=0)
+    k = o.wave()["k1"].real
+    lvl = o.max_level
+    L, th, ph, wgl, dphi, kp = o.grid(1, lvl)
+    w = np.repeat(wgl, 2 * L) * dphi
+    vecs = o.drvec(lvl)
+    box = o.level(lvl)[2][0]
+    rng = np.random.default_rng(3)
+    for v in range(0, vecs.shape[0], 29):
+        X = vecs[v]
+        T = o.tf(1, lvl)[v]
+        for _ in range(3):
+            d = (rng.random(3) - 0.5) * box * 0.9
+            approx = (-1j * k / (4 * np.pi)) * np.sum(w * np.exp(-1j * k * (kp @ d)) * T)
+            Rd = np.linalg.norm(X + d)
+            exact = np.exp(-1j * k * Rd) / Rd
+            assert abs(approx - exact) / abs(exact) < 1e-3
+
+
+def test_interpolation_rows():
+    m = geometry.icosphere(2, 1.2)
+    o = pyoracle.Oracle(m, ie=0, mode=0)
+    for lvl in range(2, o.max_level):
+        taps, w = o.interp(1, lvl)
+        Lc, thc, phc, _, _, _ = o.grid(1, lvl + 1)
+        Lp, thp, php, _, _, _ = o.grid(1, lvl)
+        assert np.allclose(w.sum(axis=1), 1.0, atol=1e-12)
+        # quadratic in theta and phi reproduced where the stencil does not wrap
+        tc, pc = taps // (2 * Lc), taps % (2 * Lc)
+        f_child = (thc[tc] ** 2 + 0.3 * thc[tc]) * (1 + php[0] * 0) + 0.5 * phc[pc] ** 2
+        tq = np.repeat(thp, 2 * Lp)
+        pq = np.tile(php, Lp)
+        interior = (np.ptp(tc, axis=1) <= 2) & (np.ptp(pc, axis=1) <= 2)
+        exact = tq ** 2 + 0.3 * tq + 0.5 * pq ** 2
+        assert interior.sum() > 0.5 * interior.size
+        assert np.max(np.abs(np.sum(w * f_child, axis=1) - exact)[interior]) < 1e-10
+
+
+# ------------------------------------------------------------------ singular integrals
+def _duffy(tri, r, f, n=128):
+    """Numerical integral over triangle `tri` of f(r') with the integrable
+    singularity at the projection of r resolved by the Duffy map."""
+    x, wx = np.polynomial.legendre.leggauss(n)
+    u = 0.5 * (x + 1)
+    wu = 0.5 * wx
+    nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    nrm /= np.linalg.norm(nrm)
+    rho = r - np.dot(r - tri[0], nrm) * nrm
+    total = 0.0
+    for a, b in ((tri[0], tri[1]), (tri[1], tri[2]), (tri[2], tri[0])):
+        ca, cb = a - rho, b - rho
+        jac = np.dot(np.cross(ca, cb), nrm)
+        U, V = np.meshgrid(u, u, indexing="ij")
+        W = np.outer(wu, wu)
+        P = rho + U[..., None] * ((1 - V)[..., None] * ca + V[..., None] * cb)
+        total = total + np.einsum("ij,ij...->...", W * U * jac, f(P))
+    return total
+
+
+@pytest.mark.parametrize("r", [[0.21, 0.33, 0.0], [0.21, 0.33, 0.07], [0.6, 0.6, 0.05], [1.2, -0.3, 0.2]])
+def test_singular_integrals_vs_quadrature(r):
+    tri = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 0.9, 0.0]])
+    r = np.array(r)
+    Iv, Is, Iv3, Is3, rho = pyoracle.sing(tri, [0, 0, 1.0], r)
+    R = lambda P: np.linalg.norm(P - r, axis=-1)
+    assert abs(Is - _duffy(tri, r, lambda P: 1.0 / R(P))) < 1e-9
+    assert np.max(np.abs(Iv - _duffy(tri, r, lambda P: (P - rho) / R(P)[..., None]))) < 1e-9
+    if r[2] != 0:
+        d = abs(r[2])
+        assert abs(Is3 * d - d * _duffy(tri, r, lambda P: 1.0 / R(P) ** 3, n=192)) < 1e-6
+        num = _duffy(tri, r, lambda P: (P - rho) / (R(P) ** 3)[..., None], n=192)
+        assert np.max(np.abs(Iv3 - num)) < 1e-6
+
+
+def test_near_operator_symmetry():
+    """Galerkin L (and EFIE) entries of non-touching pairs are symmetric."""
+    m = geometry.icosphere(2, 0.5)
+    o = pyoracle.Oracle(m, ie=2, eps=(4.0, 0.0), mode=0)
+    ints, _, mid = o.rwg()
+    rng = np.random.default_rng(5)
+    faces = [set(f) for f in ints[:, :2]]
+    for _ in range(20):
+        a, b = rng.integers(0, o.n_rwg, 2)
+        if faces[a] & faces[b]:
+            continue
+        # L(k1), L(k2) and K(k1): f_m.(R x f_n) is unchanged when m, n swap (R -> -R)
+        scale = abs(o.op(0, a, b))
+        for which in (0, 1, 2):
+            assert abs(o.op(which, a, b) - o.op(which, b, a)) <= 1e-12 * max(abs(o.op(which, a, b)), scale)
