@@ -140,6 +140,14 @@ void Engine::build(int np, const int* ids, const double* xyz, int nf, const int*
     d_perm_ = upload(tree.perm);
     d_leaf_ptr_ = upload(tree.leaf_ptr);
     d_leaf_of_ = upload(leaf_of);
+    {
+        std::vector<int> order(nleaf);
+        for (int t = 0; t < nleaf; ++t) order[t] = t;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return tree.leaf_ptr[a + 1] - tree.leaf_ptr[a] > tree.leaf_ptr[b + 1] - tree.leaf_ptr[b];
+        });
+        d_leaf_order_ = upload(order);
+    }
     std::vector<double> lc(3 * (size_t)nleaf);
     for (int t = 0; t < nleaf; ++t) {
         lc[3 * t] = tree.lv[L].centre[t].x;
@@ -237,35 +245,61 @@ void Engine::build_wave(int w) {
     }
 }
 
-// Leaf-pair blocks: for every leaf its near neighbours in list order, then
-// itself (the column order of Code/Zmartix.h:595-629).
+// Near field: for every target leaf its near neighbours in list order, then
+// itself (the column order of Code/Zmartix.h:595-629).  All blocks of one target
+// are stored as one column-major nt x C_t matrix per plane (NearDev, geo.hpp).
 void Engine::build_near() {
     const Level& lf = tree.lv[tree.max_level];
     const int planes = (ie == MLFMA_PMCHWT) ? 3 : 1;
+    const std::vector<int>& lp = tree.leaf_ptr;
     pair_ptr.assign(1, 0);
     pair_tgt.clear();
     pair_src.clear();
-    pair_off.clear();
-    long long off = 0;
+    pair_col.clear();
+    tgt_off.assign(nleaf + 1, 0);
+    tgt_cols.assign(nleaf, 0);
+    std::vector<int> col_ptr(nleaf + 1, 0), colref;
     for (int t = 0; t < nleaf; ++t) {
-        const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
+        const int nt = lp[t + 1] - lp[t];
+        int cols = 0;
         auto add = [&](int s) {
-            const int ns = tree.leaf_ptr[s + 1] - tree.leaf_ptr[s];
             pair_tgt.push_back(t);
             pair_src.push_back(s);
-            pair_off.push_back(off);
-            off += (long long)planes * nt * ns;
+            pair_col.push_back(cols);
+            for (int j = lp[s]; j < lp[s + 1]; ++j) colref.push_back(tree.perm[j]);
+            cols += lp[s + 1] - lp[s];
         };
         for (int j = lf.near_ptr[t]; j < lf.near_ptr[t + 1]; ++j) add(lf.near_idx[j]);
         add(t);
         pair_ptr.push_back((int)pair_src.size());
+        tgt_cols[t] = cols;
+        col_ptr[t + 1] = (int)colref.size();
+        if ((long long)planes * cols * nt >= (1LL << 31))
+            throw Error(MLFMA_ERR_ARG, "near-field block row of one leaf exceeds 2^31 entries (leaf too large)");
+        tgt_off[t + 1] = tgt_off[t] + (long long)planes * cols * nt;
     }
-    near_entries = off;
-    d_pair_ptr_ = upload(pair_ptr);
-    d_pair_tgt_ = upload(pair_tgt);
-    d_pair_src_ = upload(pair_src);
-    d_pair_off_ = upload(pair_off);
-    d_Z_ = dalloc<double2>((size_t)off);
+    near_entries = tgt_off[nleaf];
+    // launch order: largest block rows first, so the last wave of workgroups is short
+    std::vector<int> order(nleaf);
+    for (int t = 0; t < nleaf; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (long long)tgt_cols[a] * (lp[a + 1] - lp[a]) > (long long)tgt_cols[b] * (lp[b + 1] - lp[b]);
+    });
+    nd_.planes = planes;
+    nd_.nleaf = nleaf;
+    nd_.npairs = (int)pair_src.size();
+    nd_.N = N;
+    nd_.leaf_ptr = d_leaf_ptr_;
+    nd_.perm = d_perm_;
+    nd_.order = upload(order);
+    nd_.tgt_off = upload(tgt_off);
+    nd_.tgt_cols = upload(tgt_cols);
+    nd_.col_ptr = upload(col_ptr);
+    nd_.colref = upload(colref);
+    nd_.pair_tgt = upload(pair_tgt);
+    nd_.pair_src = upload(pair_src);
+    nd_.pair_col = upload(pair_col);
+    d_Z_ = dalloc<double2>((size_t)near_entries);
     NearConsts c;
     c.k1 = d2(med.k1);
     c.k2 = d2(med.k2);
@@ -273,9 +307,21 @@ void Engine::build_near() {
     c.etai = d2(med.etai);
     c.inv_etai = d2(1.0 / med.etai);
     c.alpha = P.cfie_alpha;
-    launch_near_fill(st, ie, (int)pair_src.size(), d_pair_tgt_, d_pair_src_, d_pair_off_, d_leaf_ptr_, d_geo_, gauss_, c,
-                     d_Z_);
+    launch_near_fill(st, ie, nd_, d_geo_, gauss_, c, d_Z_);
     MLFMA_HIP(hipGetLastError());
+}
+
+// Copy of one leaf-pair block, planes x (ns x nt column-major).
+std::vector<double2> Engine::near_block(int p) {
+    const int t = pair_tgt[p], s = pair_src[p];
+    const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t], ns = tree.leaf_ptr[s + 1] - tree.leaf_ptr[s];
+    const int planes = (ie == MLFMA_PMCHWT) ? 3 : 1;
+    const size_t blk = (size_t)nt * ns, plane = (size_t)tgt_cols[t] * nt;
+    std::vector<double2> out(planes * blk);
+    for (int pl = 0; pl < planes; ++pl)
+        MLFMA_HIP(hipMemcpy(out.data() + pl * blk, d_Z_ + tgt_off[t] + pl * plane + (size_t)pair_col[p] * nt,
+                            blk * sizeof(double2), hipMemcpyDeviceToHost));
+    return out;
 }
 
 // mexp, m2m (fine -> coarse), then m2l with the gather-form l2l (coarse -> fine)
@@ -283,7 +329,7 @@ void Engine::far_pass(int w, const double2* x) {
     WaveDev& D = wd_[w];
     const int L = tree.max_level;
     mark(6 * w + 0);
-    launch_aggregate(st, ncur, nleaf, D.K[L], N, d_leaf_ptr_, d_perm_, D.Vs, x, D.Sm[L]);
+    launch_aggregate(st, ncur, nleaf, D.K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_, D.Vs, x, D.Sm[L]);
     mark(6 * w + 1);
     mark(6 * w + 2);
     for (int l = L - 1; l >= 2; --l)
@@ -339,14 +385,13 @@ void Engine::apply_one(const double2* x, double2* y) {
     const int L = tree.max_level;
     for (int w = 0; w < nw; ++w) far_pass(w, x);
     mark(12);
-    launch_near(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_pair_ptr_, d_pair_src_, d_pair_off_,
-                d_Z_, max_leaf, x, y);
+    launch_near(st, nd_, d_Z_, x, y);
     mark(13);
     mark(14);
     if (ie != MLFMA_PMCHWT) {
         // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
         const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
-        launch_disagg_pec(st, nleaf, wd_[0].K[L], d_leaf_ptr_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), y);
+        launch_disagg_pec(st, N, wd_[0].K[L], d_leaf_of_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), y);
     } else {
         PmchwtFarArgs a;
         a.K1 = wd_[0].K[L];
@@ -362,7 +407,7 @@ void Engine::apply_one(const double2* x, double2* y) {
         a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
         a.etai = d2(med.etai);
         a.inv_etai = d2(1.0 / med.etai);
-        launch_disagg_pmchwt(st, nleaf, d_leaf_ptr_, d_perm_, a, y);
+        launch_disagg_pmchwt(st, d_leaf_of_, d_perm_, a, y);
     }
     mark(15);
 }
@@ -380,7 +425,7 @@ void Engine::stage(int which, const double2* dx, double2* dy) {
     switch (which) {
         case MLFMA_STAGE_AGGREGATE:
             for (int w = 0; w < nw; ++w)
-                launch_aggregate(st, ncur, nleaf, wd_[w].K[L], N, d_leaf_ptr_, d_perm_, wd_[w].Vs, dx, wd_[w].Sm[L]);
+                launch_aggregate(st, ncur, nleaf, wd_[w].K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_, wd_[w].Vs, dx, wd_[w].Sm[L]);
             break;
         case MLFMA_STAGE_M2M:
             for (int w = 0; w < nw; ++w) {
@@ -405,7 +450,7 @@ void Engine::stage(int which, const double2* dx, double2* dy) {
             MLFMA_HIP(hipMemsetAsync(dy, 0, sizeof(double2) * nunk, st));
             if (ie != MLFMA_PMCHWT) {
                 const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
-                launch_disagg_pec(st, nleaf, wd_[0].K[L], d_leaf_ptr_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), dy);
+                launch_disagg_pec(st, N, wd_[0].K[L], d_leaf_of_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), dy);
             } else {
                 PmchwtFarArgs a;
                 a.K1 = wd_[0].K[L], a.K2 = wd_[1].K[L], a.N = N;
@@ -414,13 +459,12 @@ void Engine::stage(int which, const double2* dx, double2* dy) {
                 a.c1 = d2(med.k1 * med.k1 / (16.0 * kPi * kPi));
                 a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
                 a.etai = d2(med.etai), a.inv_etai = d2(1.0 / med.etai);
-                launch_disagg_pmchwt(st, nleaf, d_leaf_ptr_, d_perm_, a, dy);
+                launch_disagg_pmchwt(st, d_leaf_of_, d_perm_, a, dy);
             }
             break;
         }
         case MLFMA_STAGE_NEAR:
-            launch_near(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_pair_ptr_, d_pair_src_,
-                        d_pair_off_, d_Z_, max_leaf, dx, dy);
+            launch_near(st, nd_, d_Z_, dx, dy);
             break;
         default: throw Error(MLFMA_ERR_ARG, "unknown stage");
     }
@@ -645,13 +689,14 @@ void Engine::precond_setup() {
         off[t + 1] = off[t] + (long long)m * m;
     }
     std::vector<double2> Dinv((size_t)off[nleaf]);
+    std::vector<std::vector<double2>> self(nleaf);
+    for (int t = 0; t < nleaf; ++t) self[t] = near_block(pair_ptr[t + 1] - 1);   // self block: last pair of the row
 #pragma omp parallel for schedule(dynamic)
     for (int t = 0; t < nleaf; ++t) {
         const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
         const int m = pm ? 2 * nt : nt;
-        const int p = pair_ptr[t + 1] - 1;   // self block is the last pair of the row
-        std::vector<double2> blk((size_t)(pm ? 3 : 1) * nt * nt);
-        hipMemcpy(blk.data(), d_Z_ + pair_off[p], blk.size() * sizeof(double2), hipMemcpyDeviceToHost);
+        std::vector<double2> blk;
+        blk.swap(self[t]);
         std::vector<cplx> A((size_t)m * m), I((size_t)m * m, cplx(0, 0));
         for (int j = 0; j < nt; ++j)
             for (int i = 0; i < nt; ++i) {
@@ -798,9 +843,7 @@ void Engine::get_near_row(int row, int* len, int* cols, double* vals) {
             cnt += pm ? 2 * ns : ns;
             continue;
         }
-        const int planes = pm ? 3 : 1;
-        std::vector<double2> blk((size_t)planes * nt * ns);
-        MLFMA_HIP(hipMemcpy(blk.data(), d_Z_ + pair_off[p], blk.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        const std::vector<double2> blk = near_block(p);
         for (int j = 0; j < ns; ++j) {
             const int sid = tree.perm[j0 + j];
             auto put = [&](double2 v, int col) {

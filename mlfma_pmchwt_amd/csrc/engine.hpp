@@ -88,9 +88,11 @@ public:
     long long near_entries = 0;
     std::string last_error;
 
-    // near-field block-sparse structure (host copies kept for inspection)
-    std::vector<int> pair_ptr, pair_tgt, pair_src;
-    std::vector<long long> pair_off;
+    // near-field structure (host copies kept for inspection): pairs of leaves,
+    // and per target leaf its block row (layout of NearDev, geo.hpp)
+    std::vector<int> pair_ptr, pair_tgt, pair_src, pair_col;
+    std::vector<long long> tgt_off;
+    std::vector<int> tgt_cols;
 
 private:
     template <typename T>
@@ -104,6 +106,7 @@ private:
     void mark(int slot);
     void collect_profile();
     void apply_one(const double2* x, double2* y);
+    std::vector<double2> near_block(int p);   // planes x (ns x nt column-major) of pair p
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
     double norm(const double2* v, int n);
@@ -114,13 +117,11 @@ private:
     int* d_perm_ = nullptr;
     int* d_leaf_ptr_ = nullptr;
     int* d_leaf_of_ = nullptr;
+    int* d_leaf_order_ = nullptr;   // leaves by decreasing size (aggregation launch order)
     double* d_leaf_c_ = nullptr;
     RwgGeo* d_geo_ = nullptr;
     GaussRule gauss_{};
-    int* d_pair_ptr_ = nullptr;
-    int* d_pair_tgt_ = nullptr;
-    int* d_pair_src_ = nullptr;
-    long long* d_pair_off_ = nullptr;
+    NearDev nd_{};
     double2* d_Z_ = nullptr;
     // solver scratch
     double2* d_V_ = nullptr;

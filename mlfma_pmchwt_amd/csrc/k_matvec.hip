@@ -4,6 +4,8 @@
 // (consecutive k-space samples) touch consecutive addresses.  RWG functions are
 // stored in leaf-major order (internal index i); perm[i] is the reference index
 // used by the caller's x and y vectors.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "cplx.hpp"
 
@@ -20,43 +22,85 @@ __device__ __forceinline__ int xcd_remap(int lin, int total) {
 
 // ---------------------------------------------------------------- aggregation
 // Code/ProcessMLFMM.h:33-56 (mexp), J and M currents of PMCHWT in one pass.
+// One workgroup per leaf, threads over the K samples (blockDim = K rounded up
+// to a wavefront, at most 512), so that a workgroup streams whole pattern rows
+// [i][3][K] of its leaf from HBM.  Leaves are taken largest first.  The leaf's
+// x values are gathered into LDS in chunks of kAggChunk RWGs, which keeps the
+// dependent perm -> x loads out of the pattern stream.
+constexpr int kAggChunk = 128;
+
 template <int NC>
-__global__ __launch_bounds__(256) void k_aggregate(int K, int N, const int* __restrict__ leaf_ptr,
-                                                   const int* __restrict__ perm, const double2* __restrict__ Vs,
-                                                   const double2* __restrict__ x, double2* __restrict__ Sm) {
-    const int leaf = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= K) return;
-    double2 acc[NC][3];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[c][d] = make_double2(0.0, 0.0);
+__global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __restrict__ order,
+                                                   const int* __restrict__ leaf_ptr, const int* __restrict__ perm,
+                                                   const double2* __restrict__ Vs, const double2* __restrict__ x,
+                                                   double2* __restrict__ Sm) {
+    __shared__ double2 xl[NC][kAggChunk];
+    const int leaf = order[blockIdx.x];
     const int i0 = leaf_ptr[leaf], i1 = leaf_ptr[leaf + 1];
-    for (int i = i0; i < i1; ++i) {
-        const int r = perm[i];
-        const double2* v = Vs + (size_t)i * 3 * K + q;
-        const double2 v0 = v[0], v1 = v[K], v2 = v[2 * K];
+    for (int q0 = 0; q0 < K; q0 += blockDim.x) {
+        const int q = q0 + threadIdx.x;
+        const bool act = q < K;
+        double2 acc[NC][3];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const double2 xv = x[(size_t)c * N + r];
-            cfma(acc[c][0], v0, xv);
-            cfma(acc[c][1], v1, xv);
-            cfma(acc[c][2], v2, xv);
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) acc[c][d] = make_double2(0.0, 0.0);
+        for (int b = i0; b < i1; b += kAggChunk) {
+            const int n = min(kAggChunk, i1 - b);
+            __syncthreads();
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                const int r = perm[b + j];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) xl[c][j] = x[(size_t)c * N + r];
+            }
+            __syncthreads();
+            if (act) {
+                const double2* v = Vs + (size_t)b * 3 * K + q;
+                const size_t s3 = (size_t)3 * K;
+                int j = 0;
+                for (; j + 1 < n; j += 2) {
+                    const double2 a0 = v[0], a1 = v[K], a2 = v[2 * K];
+                    const double2 b0 = v[s3], b1 = v[s3 + K], b2 = v[s3 + 2 * K];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const double2 xa = xl[c][j], xb = xl[c][j + 1];
+                        cfma(acc[c][0], a0, xa);
+                        cfma(acc[c][1], a1, xa);
+                        cfma(acc[c][2], a2, xa);
+                        cfma(acc[c][0], b0, xb);
+                        cfma(acc[c][1], b1, xb);
+                        cfma(acc[c][2], b2, xb);
+                    }
+                    v += 2 * s3;
+                }
+                if (j < n) {
+                    const double2 a0 = v[0], a1 = v[K], a2 = v[2 * K];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const double2 xa = xl[c][j];
+                        cfma(acc[c][0], a0, xa);
+                        cfma(acc[c][1], a1, xa);
+                        cfma(acc[c][2], a2, xa);
+                    }
+                }
+            }
+        }
+        if (act) {
+            double2* s = Sm + (size_t)leaf * NC * 3 * K + q;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) s[(size_t)(c * 3 + d) * K] = acc[c][d];
         }
     }
-    double2* s = Sm + (size_t)leaf * NC * 3 * K + q;
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) s[(size_t)(c * 3 + d) * K] = acc[c][d];
 }
 
-void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const int* leaf_ptr, const int* perm,
-                      const double2* Vs, const double2* x, double2* Sm) {
-    dim3 grid((K + 255) / 256, nleaf);
-    if (ncur == 1) k_aggregate<1><<<grid, 256, 0, st>>>(K, N, leaf_ptr, perm, Vs, x, Sm);
-    else k_aggregate<2><<<grid, 256, 0, st>>>(K, N, leaf_ptr, perm, Vs, x, Sm);
+void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const int* order, const int* leaf_ptr,
+                      const int* perm, const double2* Vs, const double2* x, double2* Sm) {
+    if (nleaf == 0) return;
+    const int threads = std::min(512, (K + 63) / 64 * 64);
+    if (ncur == 1) k_aggregate<1><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
+    else k_aggregate<2><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
 }
 
 // ---------------------------------------------------------------- interpolation
@@ -187,35 +231,55 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
 
 // ---------------------------------------------------------------- disaggregation
 // Code/MLFMM.cpp:211-227: w_m += (k eta/4pi)(k/4pi) sum_k Vfmj_m . Bm_leaf.
-// One wavefront per RWG row, lanes over k, then a wavefront reduction.
-__global__ __launch_bounds__(256) void k_disagg_pec(int K, const int* __restrict__ leaf_ptr,
+// Each wavefront owns kDisRows consecutive RWG rows (internal order) and
+// streams their patterns together, lanes over k, then one wavefront reduction
+// per row.  Rows run over the whole RWG range, so small and large leaves load
+// the GPU alike.
+constexpr int kDisRows = 4;
+
+__global__ __launch_bounds__(256) void k_disagg_pec(int N, int K, const int* __restrict__ leaf_of,
                                                     const int* __restrict__ perm, const double2* __restrict__ Vf,
                                                     const double2* __restrict__ Bm, double2 c,
                                                     double2* __restrict__ y) {
-    const int leaf = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const double2* B = Bm + (size_t)leaf * 3 * K;
-    for (int i = leaf_ptr[leaf] + wave; i < leaf_ptr[leaf + 1]; i += 4) {
-        const double2* V = Vf + (size_t)i * 3 * K;
-        double2 s = make_double2(0.0, 0.0);
-        for (int q = lane; q < K; q += 64) {
-            cfma(s, V[q], B[q]);
-            cfma(s, V[K + q], B[K + q]);
-            cfma(s, V[2 * K + q], B[2 * K + q]);
+    const int r0 = (blockIdx.x * 4 + wave) * kDisRows;
+    if (r0 >= N) return;
+    const double2* V[kDisRows];
+    const double2* B[kDisRows];
+    double2 s[kDisRows];
+#pragma unroll
+    for (int u = 0; u < kDisRows; ++u) {
+        const int i = min(r0 + u, N - 1);   // rows past N repeat row N-1 and are not written
+        V[u] = Vf + (size_t)i * 3 * K;
+        B[u] = Bm + (size_t)leaf_of[i] * 3 * K;
+        s[u] = make_double2(0.0, 0.0);
+    }
+    for (int q = lane; q < K; q += 64) {
+#pragma unroll
+        for (int u = 0; u < kDisRows; ++u) {
+            cfma(s[u], V[u][q], B[u][q]);
+            cfma(s[u], V[u][K + q], B[u][K + q]);
+            cfma(s[u], V[u][2 * K + q], B[u][2 * K + q]);
         }
-        s = wave_sum(s);
-        if (lane == 0) {
-            const int r = perm[i];
-            double2 t = y[r];
-            cfma(t, c, s);
-            y[r] = t;
-        }
+    }
+#pragma unroll
+    for (int u = 0; u < kDisRows; ++u) s[u] = wave_sum(s[u]);
+    if (lane < kDisRows && r0 + lane < N) {
+        double2 v = s[0];
+#pragma unroll
+        for (int u = 1; u < kDisRows; ++u)
+            if (lane == u) v = s[u];
+        const int r = perm[r0 + lane];
+        double2 t = y[r];
+        cfma(t, c, v);
+        y[r] = t;
     }
 }
 
-void launch_disagg_pec(hipStream_t st, int nleaf, int K, const int* leaf_ptr, const int* perm, const double2* Vf,
+void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const double2* Vf,
                        const double2* Bm, double2 c, double2* y) {
-    k_disagg_pec<<<nleaf, 256, 0, st>>>(K, leaf_ptr, perm, Vf, Bm, c, y);
+    const int rows_per_block = 4 * kDisRows;
+    k_disagg_pec<<<(N + rows_per_block - 1) / rows_per_block, 256, 0, st>>>(N, K, leaf_of, perm, Vf, Bm, c, y);
 }
 
 // Code/MLFMM.cpp:285-351.  The K integral sum (-khat x V).B with khat = -kp is
@@ -250,122 +314,168 @@ __device__ __forceinline__ void far_sums(int K, const double2* __restrict__ V, c
     }
 }
 
-__global__ __launch_bounds__(256) void k_disagg_pmchwt(const int* __restrict__ leaf_ptr, const int* __restrict__ perm,
+// One wavefront per RWG row over the whole RWG range (four rows per block).
+__global__ __launch_bounds__(256) void k_disagg_pmchwt(const int* __restrict__ leaf_of, const int* __restrict__ perm,
                                                        PmchwtFarArgs a, double2* __restrict__ y) {
-    const int leaf = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wave;
+    if (i >= a.N) return;
+    const int leaf = leaf_of[i];
     const double2* BJ1 = a.B1 + (size_t)leaf * 6 * a.K1;
     const double2* BM1 = BJ1 + 3 * (size_t)a.K1;
     const double2* BJ2 = a.B2 + (size_t)leaf * 6 * a.K2;
     const double2* BM2 = BJ2 + 3 * (size_t)a.K2;
-    for (int i = leaf_ptr[leaf] + wave; i < leaf_ptr[leaf + 1]; i += 4) {
-        double2 L1J = {0, 0}, K1J = {0, 0}, L1M = {0, 0}, K1M = {0, 0};
-        double2 L2J = {0, 0}, K2J = {0, 0}, L2M = {0, 0}, K2M = {0, 0};
-        far_sums(a.K1, a.Vf1 + (size_t)i * 3 * a.K1, BJ1, BM1, a.kp1, lane, L1J, K1J, L1M, K1M);
-        far_sums(a.K2, a.Vf2 + (size_t)i * 3 * a.K2, BJ2, BM2, a.kp2, lane, L2J, K2J, L2M, K2M);
-        L1J = wave_sum(L1J);
-        K1J = wave_sum(K1J);
-        L1M = wave_sum(L1M);
-        K1M = wave_sum(K1M);
-        L2J = wave_sum(L2J);
-        K2J = wave_sum(K2J);
-        L2M = wave_sum(L2M);
-        K2M = wave_sum(K2M);
-        if (lane == 0) {
-            const int r = perm[i];
-            double2 wj = y[r], wm = y[a.N + r];
-            // Code/MLFMM.cpp:339-349
-            wj = cadd(wj, cmul(a.c1, L1J));
-            wj = csub(wj, cmul(a.c1, K1M));
-            wm = cadd(wm, cmul(a.c1, K1J));
-            wm = cadd(wm, cmul(a.c1, L1M));
-            wj = cadd(wj, cmul(cmul(a.c2, a.etai), L2J));
-            wj = csub(wj, cmul(a.c2, K2M));
-            wm = cadd(wm, cmul(a.c2, K2J));
-            wm = cadd(wm, cmul(cmul(a.c2, L2M), a.inv_etai));
-            y[r] = wj;
-            y[a.N + r] = wm;
-        }
+    double2 L1J = {0, 0}, K1J = {0, 0}, L1M = {0, 0}, K1M = {0, 0};
+    double2 L2J = {0, 0}, K2J = {0, 0}, L2M = {0, 0}, K2M = {0, 0};
+    far_sums(a.K1, a.Vf1 + (size_t)i * 3 * a.K1, BJ1, BM1, a.kp1, lane, L1J, K1J, L1M, K1M);
+    far_sums(a.K2, a.Vf2 + (size_t)i * 3 * a.K2, BJ2, BM2, a.kp2, lane, L2J, K2J, L2M, K2M);
+    L1J = wave_sum(L1J);
+    K1J = wave_sum(K1J);
+    L1M = wave_sum(L1M);
+    K1M = wave_sum(K1M);
+    L2J = wave_sum(L2J);
+    K2J = wave_sum(K2J);
+    L2M = wave_sum(L2M);
+    K2M = wave_sum(K2M);
+    if (lane == 0) {
+        const int r = perm[i];
+        double2 wj = y[r], wm = y[a.N + r];
+        // Code/MLFMM.cpp:339-349
+        wj = cadd(wj, cmul(a.c1, L1J));
+        wj = csub(wj, cmul(a.c1, K1M));
+        wm = cadd(wm, cmul(a.c1, K1J));
+        wm = cadd(wm, cmul(a.c1, L1M));
+        wj = cadd(wj, cmul(cmul(a.c2, a.etai), L2J));
+        wj = csub(wj, cmul(a.c2, K2M));
+        wm = cadd(wm, cmul(a.c2, K2J));
+        wm = cadd(wm, cmul(cmul(a.c2, L2M), a.inv_etai));
+        y[r] = wj;
+        y[a.N + r] = wm;
     }
 }
 
-void launch_disagg_pmchwt(hipStream_t st, int nleaf, const int* leaf_ptr, const int* perm, const PmchwtFarArgs& a,
-                          double2* y) {
-    k_disagg_pmchwt<<<nleaf, 256, 0, st>>>(leaf_ptr, perm, a, y);
+void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, const PmchwtFarArgs& a, double2* y) {
+    k_disagg_pmchwt<<<(a.N + 3) / 4, 256, 0, st>>>(leaf_of, perm, a, y);
 }
 
 // ---------------------------------------------------------------- near field
-// Code/MLFMM.cpp:228-231 (PEC) and :354-366 (PMCHWT) on leaf-pair blocks.  A
-// block is stored column-major (consecutive rows of one source column are
-// contiguous) so that a wavefront reads 64 consecutive entries.  PMCHWT blocks
-// hold three planes z11, zK, z22 with z12 = -zK, z21 = zK (Code/Zmartix.h:611-614).
-template <int P>
-__global__ __launch_bounds__(256) void k_near(int N, int max_leaf, const int* __restrict__ leaf_ptr,
-                                              const int* __restrict__ perm, const int* __restrict__ pair_ptr,
-                                              const int* __restrict__ pair_src,
-                                              const long long* __restrict__ pair_off, const double2* __restrict__ Z,
+// Code/MLFMM.cpp:228-231 (PEC) and :354-366 (PMCHWT).  All near blocks of one
+// target leaf form a single column-major nt x C_t matrix per plane (PMCHWT:
+// z11, zK, z22 with z12 = -zK, z21 = zK, Code/Zmartix.h:611-614), so one
+// workgroup streams one contiguous range of HBM.  The rows are taken in slices
+// of `span` rows; the 256 threads form G = 256 / span column groups and group g
+// takes the columns c = g (mod G), so a wavefront reads runs of `span`
+// consecutive entries and few lanes idle whatever nt is.  The group partials
+// are summed in LDS in a fixed order.  x is gathered into LDS CH columns at a
+// time.
+template <int P, int CH>
+__global__ __launch_bounds__(256) void k_near(NearDev nd, const double2* __restrict__ Z,
                                               const double2* __restrict__ x, double2* __restrict__ y) {
-    extern __shared__ double2 xs[];   // [P == 3 ? 2 : 1][max_leaf]
-    const int t = blockIdx.x;
-    const int i0 = leaf_ptr[t], nt = leaf_ptr[t + 1] - i0;
-    for (int r0 = 0; r0 < nt; r0 += blockDim.x) {
-        const int row = r0 + threadIdx.x;
+    constexpr int NX = (P == 3) ? 2 : 1;
+    __shared__ double2 xs[NX][CH];
+    __shared__ double2 red[NX][256];
+    const int t = nd.order[blockIdx.x];
+    const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
+    if (nt == 0) return;
+    const int C = nd.tgt_cols[t];
+    const int* cref = nd.colref + nd.col_ptr[t];
+    const int plane = C * nt;   // < 2^31: one target's block row
+    const double2* Zt = Z + nd.tgt_off[t];
+    // Rows are covered in `passes` slices of `span` rows; pick the slicing
+    // with the fewest column steps per thread, passes * ceil(C / G), among
+    // slices of at least 32 rows (512-byte runs) or the whole leaf.
+    const int pmin = (nt + 255) / 256;
+    int passes = pmin, best = 0x7fffffff;
+    for (int p = pmin; p < pmin + 8; ++p) {
+        const int sp = (nt + p - 1) / p, gp = 256 / sp;
+        if (p > pmin && sp < 32) break;
+        const int cost = p * ((C + gp - 1) / gp);
+        if (cost < best) best = cost, passes = p;
+    }
+    const int span = (nt + passes - 1) / passes;
+    const int G = 256 / span;
+    const int g = threadIdx.x / span, rl = threadIdx.x - g * span;
+    for (int r0 = 0; r0 < nt; r0 += span) {
+        const int row = r0 + rl;
+        const bool act = g < G && row < nt;
         double2 aJ = {0, 0}, aM = {0, 0};
-        for (int p = pair_ptr[t]; p < pair_ptr[t + 1]; ++p) {
-            const int s = pair_src[p];
-            const int j0 = leaf_ptr[s], ns = leaf_ptr[s + 1] - j0;
+        for (int c0 = 0; c0 < C; c0 += CH) {
+            const int cl = min(CH, C - c0);
             __syncthreads();
-            for (int j = threadIdx.x; j < ns; j += blockDim.x) {
-                const int r = perm[j0 + j];
-                xs[j] = x[r];
-                if (P == 3) xs[max_leaf + j] = x[N + r];
+            for (int c = threadIdx.x; c < cl; c += 256) {
+                const int r = cref[c0 + c];
+                xs[0][c] = x[r];
+                if (P == 3) xs[NX - 1][c] = x[nd.N + r];
             }
             __syncthreads();
-            if (row < nt) {
-                const double2* z = Z + pair_off[p] + row;
+            if (act) {
+                const double2* z = Zt + c0 * nt + row;
+                const int step = G * nt;
+                int c = g;
                 if (P == 1) {
-                    int j = 0;
-                    for (; j + 3 < ns; j += 4) {
-                        const double2 z0 = z[(size_t)j * nt], z1 = z[(size_t)(j + 1) * nt];
-                        const double2 z2 = z[(size_t)(j + 2) * nt], z3 = z[(size_t)(j + 3) * nt];
-                        cfma(aJ, z0, xs[j]);
-                        cfma(aJ, z1, xs[j + 1]);
-                        cfma(aJ, z2, xs[j + 2]);
-                        cfma(aJ, z3, xs[j + 3]);
+                    for (; c + 3 * G < cl; c += 4 * G) {
+                        const int o = c * nt;
+                        const double2 z0 = z[o], z1 = z[o + step], z2 = z[o + 2 * step], z3 = z[o + 3 * step];
+                        cfma(aJ, z0, xs[0][c]);
+                        cfma(aJ, z1, xs[0][c + G]);
+                        cfma(aJ, z2, xs[0][c + 2 * G]);
+                        cfma(aJ, z3, xs[0][c + 3 * G]);
                     }
-                    for (; j < ns; ++j) cfma(aJ, z[(size_t)j * nt], xs[j]);
+                    for (; c < cl; c += G) cfma(aJ, z[c * nt], xs[0][c]);
                 } else {
-                    const double2* zk = z + (size_t)ns * nt;
-                    const double2* zm = z + (size_t)2 * ns * nt;
-                    for (int j = 0; j < ns; ++j) {
-                        const double2 a11 = z[(size_t)j * nt], ak = zk[(size_t)j * nt], a22 = zm[(size_t)j * nt];
-                        const double2 xJ = xs[j], xM = xs[max_leaf + j];
-                        cfma(aJ, a11, xJ);
-                        cfms(aJ, ak, xM);
-                        cfma(aM, ak, xJ);
-                        cfma(aM, a22, xM);
+                    for (; c + G < cl; c += 2 * G) {
+                        const int o = c * nt;
+                        const double2 p0 = z[o], k0 = z[o + plane], m0 = z[o + 2 * plane];
+                        const double2 p1 = z[o + step], k1 = z[o + step + plane], m1 = z[o + step + 2 * plane];
+                        const double2 xJ0 = xs[0][c], xM0 = xs[NX - 1][c];
+                        const double2 xJ1 = xs[0][c + G], xM1 = xs[NX - 1][c + G];
+                        cfma(aJ, p0, xJ0);
+                        cfms(aJ, k0, xM0);
+                        cfma(aM, k0, xJ0);
+                        cfma(aM, m0, xM0);
+                        cfma(aJ, p1, xJ1);
+                        cfms(aJ, k1, xM1);
+                        cfma(aM, k1, xJ1);
+                        cfma(aM, m1, xM1);
+                    }
+                    if (c < cl) {
+                        const int o = c * nt;
+                        const double2 p0 = z[o], k0 = z[o + plane], m0 = z[o + 2 * plane];
+                        const double2 xJ0 = xs[0][c], xM0 = xs[NX - 1][c];
+                        cfma(aJ, p0, xJ0);
+                        cfms(aJ, k0, xM0);
+                        cfma(aM, k0, xJ0);
+                        cfma(aM, m0, xM0);
                     }
                 }
             }
         }
-        if (row < nt) {
-            const int r = perm[i0 + row];
+        if (G > 1) {   // fold the column groups of this row slice
+            red[0][threadIdx.x] = aJ;
+            if (P == 3) red[NX - 1][threadIdx.x] = aM;
+            __syncthreads();
+            if (threadIdx.x < span && r0 + threadIdx.x < nt) {
+                double2 sJ = red[0][threadIdx.x], sM = red[NX - 1][threadIdx.x];
+                for (int h = 1; h < G; ++h) {
+                    sJ = cadd(sJ, red[0][h * span + threadIdx.x]);
+                    if (P == 3) sM = cadd(sM, red[NX - 1][h * span + threadIdx.x]);
+                }
+                const int r = nd.perm[i0 + r0 + threadIdx.x];
+                y[r] = sJ;
+                if (P == 3) y[nd.N + r] = sM;
+            }
+        } else if (act) {   // G == 1: threads past span hold no row
+            const int r = nd.perm[i0 + row];
             y[r] = aJ;
-            if (P == 3) y[N + r] = aM;
+            if (P == 3) y[nd.N + r] = aM;
         }
     }
 }
 
-void launch_near(hipStream_t st, int planes, int nleaf, int N, const int* leaf_ptr, const int* perm,
-                 const int* pair_ptr, const int* pair_src, const long long* pair_off, const double2* Z, int max_leaf,
-                 const double2* x, double2* y) {
-    if (planes == 1) {
-        size_t sh = (size_t)max_leaf * sizeof(double2);
-        k_near<1><<<nleaf, 256, sh, st>>>(N, max_leaf, leaf_ptr, perm, pair_ptr, pair_src, pair_off, Z, x, y);
-    } else {
-        size_t sh = (size_t)2 * max_leaf * sizeof(double2);
-        k_near<3><<<nleaf, 256, sh, st>>>(N, max_leaf, leaf_ptr, perm, pair_ptr, pair_src, pair_off, Z, x, y);
-    }
+void launch_near(hipStream_t st, const NearDev& nd, const double2* Z, const double2* x, double2* y) {
+    if (nd.nleaf == 0) return;
+    if (nd.planes == 1) k_near<1, 1024><<<nd.nleaf, 256, 0, st>>>(nd, Z, x, y);
+    else k_near<3, 512><<<nd.nleaf, 256, 0, st>>>(nd, Z, x, y);
 }
 
 }  // namespace mlfma

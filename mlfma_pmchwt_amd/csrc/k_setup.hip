@@ -240,18 +240,18 @@ __device__ void pair_ops(const RwgGeo& F, const RwgGeo& S, const GaussRule& g, c
 }  // namespace
 
 // ---------------------------------------------------------------- near field
-// One workgroup per (target leaf, source leaf) block; entries column-major.
+// One workgroup per (target leaf, source leaf) block, written into its
+// columns of the target's block row (layout of NearDev, geo.hpp).
 // Code/Zmartix.h:188-278 (EFIE), :372-465 (CFIE), :541-643 (PMCHWT).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_near_fill(const int* __restrict__ pair_tgt, const int* __restrict__ pair_src,
-                                                   const long long* __restrict__ pair_off,
-                                                   const int* __restrict__ leaf_ptr, const RwgGeo* __restrict__ geo,
-                                                   GaussRule g, NearConsts c, double2* __restrict__ Z) {
+__global__ __launch_bounds__(256) void k_near_fill(NearDev nd, const RwgGeo* __restrict__ geo, GaussRule g,
+                                                   NearConsts c, double2* __restrict__ Z) {
     const int p = blockIdx.x;
-    const int t = pair_tgt[p], s = pair_src[p];
-    const int i0 = leaf_ptr[t], nt = leaf_ptr[t + 1] - i0;
-    const int j0 = leaf_ptr[s], ns = leaf_ptr[s + 1] - j0;
-    double2* out = Z + pair_off[p];
+    const int t = nd.pair_tgt[p], s = nd.pair_src[p];
+    const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
+    const int j0 = nd.leaf_ptr[s], ns = nd.leaf_ptr[s + 1] - j0;
+    const size_t plane = (size_t)nd.tgt_cols[t] * nt;
+    double2* out = Z + nd.tgt_off[t] + (size_t)nd.pair_col[p] * nt;
     for (int e = threadIdx.x; e < nt * ns; e += blockDim.x) {
         const int row = e % nt, col = e / nt;
         const RwgGeo& F = geo[i0 + row];
@@ -264,8 +264,8 @@ __global__ __launch_bounds__(256) void k_near_fill(const int* __restrict__ pair_
             const double2 zk = cadd(K[0], K[1]);
             const double2 z22 = cadd(L[0], cmul(c.inv_etai, L[1]));
             out[(size_t)col * nt + row] = z11;
-            out[(size_t)(ns + col) * nt + row] = zk;
-            out[(size_t)(2 * ns + col) * nt + row] = z22;
+            out[plane + (size_t)col * nt + row] = zk;
+            out[2 * plane + (size_t)col * nt + row] = z22;
         } else if (MODE == 0) {
             double2 L, K, M;
             pair_ops<1, true, false, false>(F, S, g, &c.k1, &L, &K, &M);
@@ -280,13 +280,12 @@ __global__ __launch_bounds__(256) void k_near_fill(const int* __restrict__ pair_
     }
 }
 
-void launch_near_fill(hipStream_t st, int mode, int npairs, const int* pair_tgt, const int* pair_src,
-                      const long long* pair_off, const int* leaf_ptr, const RwgGeo* geo, const GaussRule& g,
+void launch_near_fill(hipStream_t st, int mode, const NearDev& nd, const RwgGeo* geo, const GaussRule& g,
                       const NearConsts& c, double2* Z) {
-    if (npairs == 0) return;
-    if (mode == 2) k_near_fill<2><<<npairs, 256, 0, st>>>(pair_tgt, pair_src, pair_off, leaf_ptr, geo, g, c, Z);
-    else if (mode == 1) k_near_fill<1><<<npairs, 256, 0, st>>>(pair_tgt, pair_src, pair_off, leaf_ptr, geo, g, c, Z);
-    else k_near_fill<0><<<npairs, 256, 0, st>>>(pair_tgt, pair_src, pair_off, leaf_ptr, geo, g, c, Z);
+    if (nd.npairs == 0) return;
+    if (mode == 2) k_near_fill<2><<<nd.npairs, 256, 0, st>>>(nd, geo, g, c, Z);
+    else if (mode == 1) k_near_fill<1><<<nd.npairs, 256, 0, st>>>(nd, geo, g, c, Z);
+    else k_near_fill<0><<<nd.npairs, 256, 0, st>>>(nd, geo, g, c, Z);
 }
 
 // Single-entry evaluation (tests): out[0..3] = L(k1), L(k2), K(k1), K(k2); out[4] = MFIE(k1)

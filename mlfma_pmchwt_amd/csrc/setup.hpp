@@ -18,8 +18,7 @@ struct ExcitationArgs {
 };
 
 // mode 0 EFIE, 1 CFIE, 2 PMCHWT; one workgroup per leaf-pair block.
-void launch_near_fill(hipStream_t st, int mode, int npairs, const int* pair_tgt, const int* pair_src,
-                      const long long* pair_off, const int* leaf_ptr, const RwgGeo* geo, const GaussRule& g,
+void launch_near_fill(hipStream_t st, int mode, const NearDev& nd, const RwgGeo* geo, const GaussRule& g,
                       const NearConsts& c, double2* Z);
 void launch_pair_probe(hipStream_t st, const RwgGeo* geo, int f, int s, const GaussRule& g, const NearConsts& c,
                        double2* out);

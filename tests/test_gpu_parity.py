@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 CASES = {
     # name: (mesh args, ie, eps)
     "efie_ml": ((2, 1.5), api.EFIE, (1.0, 0.0)),
+    "efie_1lvl": ((2, 0.5), api.EFIE, (1.0, 0.0)),
     "cfie_ml": ((2, 1.5), api.CFIE, (1.0, 0.0)),
     "pmchwt_1lvl": ((2, 0.5), api.PMCHWT, (4.0, 0.0)),
     "pmchwt_lossy_ml": ((2, 1.5), api.PMCHWT, (2.5, -0.1)),
@@ -136,7 +137,9 @@ def test_excitation(name):
         assert relerr(op.excitation(pol, th, ph), orc.rhs(pol, th, ph)) <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["efie_ml", "pmchwt_1lvl"])
+# EFIE on the 1.5-lambda sphere of 480 RWGs (edges near lambda/2) stagnates
+# near 1e-2 in the oracle as well, so the solver cases are the ones that converge.
+@pytest.mark.parametrize("name", ["efie_1lvl", "cfie_ml", "pmchwt_1lvl"])
 def test_gmres_and_rcs(name):
     _, op, orc = pair(name)
     b = op.excitation(0.0, 0.0, 0.0)
