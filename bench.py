@@ -116,31 +116,51 @@ def main():
     mesh = geometry.geodesic_sphere(args.frequency, args.radius)
     log(f"mesh: {mesh.n_faces} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    op = api.Operator(mesh, integral_equation=api.CFIE, cfie_alpha=args.alpha, verbose=1)
+    op = api.Operator(mesh, integral_equation=api.CFIE, cfie_alpha=args.alpha, far_precision=api.FP32,
+                      near_precision=api.FP32, verbose=1)
     setup_s = time.perf_counter() - t0
     op.k0 = 2 * op.grid(1, op.max_level)[0] ** 2
-    log(f"setup {setup_s:.2f} s: N={op.n_rwg} levels={op.max_level + 1} leaves={op.n_leaves} "
-        f"near={op.near_entries} device={op.device_bytes / 1e9:.2f} GB")
+    log(f"setup {setup_s:.2f} s (FP64 and FP32 tables): N={op.n_rwg} levels={op.max_level + 1} "
+        f"leaves={op.n_leaves} near={op.near_entries} device={op.device_bytes / 1e9:.2f} GB")
 
     gen = torch.Generator(device="cuda")
     gen.manual_seed(12345)
     x = torch.randn(op.n, dtype=torch.complex128, device="cuda", generator=gen)
-    y = torch.empty_like(x)
-    for _ in range(args.warmup):
-        op.apply_device(x.data_ptr(), y.data_ptr())
-    torch.cuda.synchronize()
+    ys = {}
 
-    op.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        op.apply_device(x.data_ptr(), y.data_ptr())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = op.profile()
-    op.set_profiling(False)
+    def timed(prec):
+        op.set_apply_precision(prec)
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            op.apply_device(x.data_ptr(), y.data_ptr())
+        torch.cuda.synchronize()
+        op.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            op.apply_device(x.data_ptr(), y.data_ptr())
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = op.profile()
+        op.set_profiling(False)
+        ys[prec] = y
+        return elapsed, prof
+
+    def solve_with(prec):
+        op.set_apply_precision(prec)
+        b = op.excitation(0.0, 0.0, 0.0)
+        xs, st = op.solve(b, restart=30, tol_rel=1e-3)
+        return {"solver": "GMRES(30)", "tol_rel": 1e-3, "iterations": st["iterations"], "matvecs": st["matvecs"],
+                "fp64_matvecs": st["fp64_matvecs"], "converged": st["converged"],
+                "true_rel_residual": st["rel_residual"], "seconds": round(st["seconds"], 4)}
+
+    elapsed, prof = timed(api.FP64)
     ms_step = 1e3 * elapsed / args.steps
-    log(f"{args.steps} matvecs in {elapsed:.3f} s -> {ms_step:.3f} ms/matvec")
+    log(f"FP64: {args.steps} matvecs in {elapsed:.3f} s -> {ms_step:.3f} ms/matvec")
+    elapsed32, prof32 = timed(api.FP32)
+    ms32 = 1e3 * elapsed32 / args.steps
+    err32 = float(torch.linalg.vector_norm(ys[api.FP32] - ys[api.FP64]) / torch.linalg.vector_norm(ys[api.FP64]))
+    log(f"FP32 mode (far field and near storage): {ms32:.3f} ms/matvec, rel. difference to FP64 {err32:.2e}")
 
     groups = prof["groups"]
     per_launch = {g: v["ms"] / max(v["launches"], 1) for g, v in groups.items()}
@@ -151,15 +171,17 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": nbytes, "ms_per_launch": round(per_launch[dom], 4)}
     stage_ms = {g: round(v["ms"] / max(prof["matvecs"], 1), 4) for g, v in groups.items()}
+    stage_ms32 = {g: round(v["ms"] / max(prof32["matvecs"], 1), 4) for g, v in prof32["groups"].items()}
 
-    solve = None
+    solve = solve32 = None
     if not args.no_solve:
-        b = op.excitation(0.0, 0.0, 0.0)
-        xs, st = op.solve(b, restart=30, tol_rel=1e-3)
-        solve = {"solver": "GMRES(30)", "tol_rel": 1e-3, "iterations": st["iterations"], "matvecs": st["matvecs"],
-                 "converged": st["converged"], "true_rel_residual": st["rel_residual"],
-                 "seconds": round(st["seconds"], 4)}
-        log(f"solve: {solve}")
+        solve = solve_with(api.FP64)
+        log(f"solve FP64: {solve}")
+        solve32 = solve_with(api.FP32)
+        log(f"solve FP32 mode + FP64 residual: {solve32}")
+    fp32 = {"mode": "far field FP32 storage+accumulation, near entries FP32 storage (FP64 accumulation)",
+            "value": round(args.steps / elapsed32, 3), "unit": "matvec/s", "ms_per_step": round(ms32, 4),
+            "rel_diff_vs_fp64": err32, "stage_ms_per_matvec": stage_ms32, "solve": solve32}
 
     cpu = None
     if not args.no_cpu:
@@ -185,6 +207,7 @@ def main():
         "stage_ms_per_matvec": stage_ms,
         "roofline": roofline,
         "solve": solve,
+        "fp32_mode": fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

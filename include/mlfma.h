@@ -39,8 +39,12 @@ typedef struct mlfma_handle mlfma_handle;
 #define MLFMA_PMCHWT 2
 
 /* Arithmetic of the far-field stages (aggregation, interpolation, translation,
- * anterpolation, disaggregation).  Near field, translation-operator generation
- * and Krylov steps are FP64 in both modes. */
+ * anterpolation, disaggregation): storage and accumulation.  Near field,
+ * translation-operator generation, patterns (computed in FP64, then rounded)
+ * and Krylov steps are FP64 in both modes.  With MLFMA_FP32 the FP64 far
+ * field is built as well: it gives the true residual of every solve
+ * (restarted GMRES then refines an FP32-operator solution against the FP64
+ * operator). */
 #define MLFMA_FP64 0
 #define MLFMA_FP32 1
 
@@ -53,6 +57,9 @@ typedef struct mlfma_params {
     double cfie_alpha;      /* reference compiles 0.2 (Code/PhyConstant.h:15) */
     double leaf_coeff;      /* finest box = leaf_coeff * lambda0; reference 0.3 (Code/PhyConstant.h:14) */
     int far_precision;      /* MLFMA_FP64 or MLFMA_FP32 */
+    int near_precision;     /* storage of the near-field entries used by FP32-mode matvecs: MLFMA_FP64
+                               or MLFMA_FP32 (entries computed in FP64, rounded once; accumulation
+                               FP64; the FP64 copy is kept for true residuals) */
     int tf_full_order;      /* 0: reference series length min(L, ceil|kR|) (Code/AlgoMLFMM.h:263); 1: L */
     int max_rhs;            /* largest batch a single apply/solve will be given */
     int device;             /* HIP device ordinal */
@@ -98,6 +105,12 @@ int mlfma_apply_batch(mlfma_handle* h, int n, int nrhs, const double* x, double*
  * the call returns after it completes. */
 int mlfma_apply_device(mlfma_handle* h, int n, int nrhs, const void* x_dev, void* y_dev);
 
+/* Far-field precision used by mlfma_apply* and by the Krylov iterations of
+ * mlfma_solve: MLFMA_FP64 always, MLFMA_FP32 when the handle was created with
+ * far_precision = MLFMA_FP32 (its default then).  No reference counterpart:
+ * the reference's operator is FP64 only. */
+int mlfma_set_apply_precision(mlfma_handle* h, int precision);
+
 /* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)
  * and MCGS::CGS::solve (Code/CGS.h:72-234) as driven by MLFMM::matrix_solver
  * (Code/MLFMM.cpp:370-397): x holds the initial guess on entry and the
@@ -120,6 +133,7 @@ typedef struct mlfma_solve_stats {
     int converged;
     double rel_residual;      /* true ||b - A x|| / ||b|| with the FP64 operator */
     double seconds;
+    int fp64_matvecs;         /* of matvecs, those with the FP64 operator (true residuals) */
 } mlfma_solve_stats;
 
 int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const double* rhs, double* x,

@@ -28,7 +28,8 @@ class Params(C.Structure):
     _fields_ = [("integral_equation", C.c_int), ("gauss_points", C.c_int), ("frequency_hz", C.c_double),
                 ("eps_r_re", C.c_double), ("eps_r_im", C.c_double), ("mu_r_re", C.c_double),
                 ("mu_r_im", C.c_double), ("cfie_alpha", C.c_double), ("leaf_coeff", C.c_double),
-                ("far_precision", C.c_int), ("tf_full_order", C.c_int), ("max_rhs", C.c_int),
+                ("far_precision", C.c_int), ("near_precision", C.c_int), ("tf_full_order", C.c_int),
+                ("max_rhs", C.c_int),
                 ("device", C.c_int), ("verbose", C.c_int)]
 
 
@@ -39,7 +40,7 @@ class SolveParams(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("converged", C.c_int),
-                ("rel_residual", C.c_double), ("seconds", C.c_double)]
+                ("rel_residual", C.c_double), ("seconds", C.c_double), ("fp64_matvecs", C.c_int)]
 
 
 _lib = None
@@ -64,6 +65,7 @@ def library() -> C.CDLL:
         lib.mlfma_apply.argtypes = [vp, C.c_int, vp, vp]
         lib.mlfma_apply_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_apply_device.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_set_apply_precision.argtypes = [vp, C.c_int]
         lib.mlfma_solve.argtypes = [vp, C.POINTER(SolveParams), C.c_int, vp, vp, C.POINTER(SolveStats)]
         lib.mlfma_excitation.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         lib.mlfma_rcs.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
@@ -151,6 +153,10 @@ class Operator:
     def apply_device(self, x_ptr: int, y_ptr: int, nrhs: int = 1):
         self._check(library().mlfma_apply_device(self._h, self.n, nrhs, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
 
+    def set_apply_precision(self, precision: int):
+        """FP64 or FP32 far field for apply* and the Krylov iterations of solve."""
+        self._check(library().mlfma_set_apply_precision(self._h, int(precision)))
+
     def solve(self, rhs: np.ndarray, x0: np.ndarray | None = None, solver=GMRES, itr_max=None, restart=30,
               tol_rel=1e-3, tol_abs=0.0, precond=0, check=False):
         rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
@@ -161,7 +167,7 @@ class Operator:
         if rc not in (0, 5) or (check and rc != 0):
             self._check(rc)
         return x, {"iterations": st.iterations, "matvecs": st.matvecs, "converged": bool(st.converged),
-                   "rel_residual": st.rel_residual, "seconds": st.seconds}
+                   "rel_residual": st.rel_residual, "seconds": st.seconds, "fp64_matvecs": st.fp64_matvecs}
 
     def excitation(self, pol=0.0, theta=0.0, phi=0.0) -> np.ndarray:
         v = np.empty(self.n, dtype=np.complex128)

@@ -24,6 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", f"-I{ROOT / 'include'}", "-W
 # Device code: gfx950 only; packed-FP32 instructions disabled.
 DEVICE = ["-x", "hip", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+# Host-only translation units (hipcc would otherwise add a default-target device pass).
+HOST = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def _sources():
@@ -40,7 +42,7 @@ def _compile(src: Path, incremental: bool) -> Path:
         newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in _headers()])
         if obj.stat().st_mtime >= newest:
             return obj
-    cmd = [HIPCC] + COMMON + (DEVICE if src.suffix == ".hip" else []) + ["-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC] + COMMON + (DEVICE if src.suffix == ".hip" else HOST) + ["-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

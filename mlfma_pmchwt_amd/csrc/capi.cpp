@@ -64,6 +64,7 @@ int check_params(const mlfma_params* p, std::string& why) {
     if (!(p->frequency_hz > 0)) return why = "frequency must be positive", MLFMA_ERR_ARG;
     if (!(p->leaf_coeff > 0)) return why = "leaf_coeff must be positive", MLFMA_ERR_ARG;
     if (p->far_precision != MLFMA_FP64 && p->far_precision != MLFMA_FP32) return why = "bad far_precision", MLFMA_ERR_ARG;
+    if (p->near_precision != MLFMA_FP64 && p->near_precision != MLFMA_FP32) return why = "bad near_precision", MLFMA_ERR_ARG;
     if (p->max_rhs < 1) return why = "max_rhs must be >= 1", MLFMA_ERR_ARG;
     const int g = p->gauss_points;
     if (g != 1 && g != 3 && g != 4 && g != 6 && g != 7 && g != 12 && g != 13 && g != 16)
@@ -85,6 +86,7 @@ void mlfma_params_default(mlfma_params* p) {
     p->cfie_alpha = 0.2;
     p->leaf_coeff = 0.3;
     p->far_precision = MLFMA_FP64;
+    p->near_precision = MLFMA_FP64;
     p->tf_full_order = 0;
     p->max_rhs = 1;
     p->device = 0;
@@ -152,6 +154,11 @@ int mlfma_apply_device(mlfma_handle* h, int n, int nrhs, const void* x_dev, void
         e.apply_device(D2(x_dev), D2(y_dev), nrhs);
         MLFMA_HIP(hipStreamSynchronize(e.st));
     });
+}
+
+int mlfma_set_apply_precision(mlfma_handle* h, int precision) {
+    if (!h) return MLFMA_ERR_ARG;
+    return guarded(h, [&] { h->eng->set_apply_precision(precision); });
 }
 
 int mlfma_apply_batch(mlfma_handle* h, int n, int nrhs, const double* x, double* y) {

@@ -1,4 +1,4 @@
-// Complex FP64 arithmetic on double2 (x = re, y = im) and wavefront reductions.
+// Complex arithmetic on double2 / float2 (x = re, y = im) and wavefront reductions.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -29,6 +29,28 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
     return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 
+// FP32 complex (float2), same operations; accumulation stays in FP32.
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ void cfma(float2& acc, float2 a, float2 b) {
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+}
+__device__ __forceinline__ void cfms(float2& acc, float2 a, float2 b) {
+    acc.x -= a.x * b.x - a.y * b.y;
+    acc.y -= a.x * b.y + a.y * b.x;
+}
+
+// Conversions between the FP64 solver vectors and the far-field precision.
+__device__ __forceinline__ void narrow(double2 a, double2& b) { b = a; }
+__device__ __forceinline__ void narrow(double2 a, float2& b) { b = make_float2((float)a.x, (float)a.y); }
+__device__ __forceinline__ double2 widen(double2 a) { return a; }
+__device__ __forceinline__ double2 widen(float2 a) { return make_double2(a.x, a.y); }
+
 // Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -36,6 +58,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 __device__ __forceinline__ double2 wave_sum(double2 v) {
+    v.x = wave_sum(v.x);
+    v.y = wave_sum(v.y);
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float2 wave_sum(float2 v) {
     v.x = wave_sum(v.x);
     v.y = wave_sum(v.y);
     return v;

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "setup.hpp"
@@ -56,6 +57,7 @@ void plane_wave(double pol_deg, double th_deg, double ph_deg, double* k, double*
 
 Engine::Engine(const mlfma_params& p) : P(p) {
     ie = p.integral_equation;
+    apply_prec = p.far_precision;
     nw = (ie == MLFMA_PMCHWT) ? 2 : 1;
     ncur = nw;
     MLFMA_HIP(hipSetDevice(p.device));
@@ -195,17 +197,52 @@ void Engine::build_wave(int w) {
     WaveDev& D = wd_[w];
     const int L = tree.max_level;
     D.K.assign(L + 1, 0);
-    D.Sm.assign(L + 1, nullptr);
-    D.Bm.assign(L + 1, nullptr);
-    D.TFw.assign(L + 1, nullptr);
     D.tap.assign(L + 1, nullptr);
-    D.tw.assign(L + 1, nullptr);
     D.t_ptr.assign(L + 1, nullptr);
     D.t_kp.assign(L + 1, nullptr);
-    D.t_w.assign(L + 1, nullptr);
-    D.sh_up.assign(L + 1, nullptr);
-    D.sh_dn.assign(L + 1, nullptr);
     for (int l = 2; l <= L; ++l) D.K[l] = T.grid[l].K();
+    for (int c = 3; c <= L; ++c) {
+        const InterpTable& it = T.interp[c];
+        D.tap[c] = upload(it.tap);
+        D.t_ptr[c] = upload(it.t_ptr);
+        D.t_kp[c] = upload(it.t_kp);
+    }
+    build_far<double>(w, T);
+    if (P.far_precision == MLFMA_FP32) build_far<float>(w, T);
+}
+
+template <typename R>
+FarDev<R>& Engine::far(int w) {
+    if constexpr (std::is_same<R, double>::value) return wd_[w].f64;
+    else return wd_[w].f32;
+}
+
+namespace {
+template <typename R>
+cplx_t<R> to_cx(cplx c) {
+    return cplx_t<R>{(R)c.real(), (R)c.imag()};
+}
+template <typename R>
+std::vector<R> to_real(const std::vector<double>& v) {
+    return std::vector<R>(v.begin(), v.end());
+}
+}  // namespace
+
+// Patterns are computed in FP64 on the device and narrowed for FP32; tables
+// are computed in FP64 on the host (translation operators, interpolation
+// weights, phase shifts) and rounded once.
+template <typename R>
+void Engine::build_far(int w, const WaveTables& T) {
+    WaveDev& D = wd_[w];
+    FarDev<R>& F = far<R>(w);
+    const int L = tree.max_level;
+    F.Sm.assign(L + 1, nullptr);
+    F.Bm.assign(L + 1, nullptr);
+    F.TFw.assign(L + 1, nullptr);
+    F.tw.assign(L + 1, nullptr);
+    F.t_w.assign(L + 1, nullptr);
+    F.sh_up.assign(L + 1, nullptr);
+    F.sh_dn.assign(L + 1, nullptr);
     const int K0 = D.K[L];
     std::vector<double> kp(3 * (size_t)K0);
     for (int q = 0; q < K0; ++q) {
@@ -213,35 +250,38 @@ void Engine::build_wave(int w) {
         kp[K0 + q] = T.grid[L].kp[q].y;
         kp[2 * K0 + q] = T.grid[L].kp[q].z;
     }
-    D.kp0 = upload(kp);
-    D.Vs = dalloc<double2>((size_t)N * 3 * K0);
-    D.Vf = dalloc<double2>((size_t)N * 3 * K0);
-    launch_patterns(st, ie == MLFMA_CFIE, N, K0, d_geo_, d_leaf_of_, d_leaf_c_, D.kp0, d2(T.k), gauss_, P.cfie_alpha,
-                    D.Vs, D.Vf);
+    F.kp0 = upload(to_real<R>(kp));
+    F.Vs = dalloc<cplx_t<R>>((size_t)N * 3 * K0);
+    F.Vf = dalloc<cplx_t<R>>((size_t)N * 3 * K0);
+    if constexpr (std::is_same<R, double>::value) {
+        launch_patterns(st, ie == MLFMA_CFIE, N, K0, d_geo_, d_leaf_of_, d_leaf_c_, F.kp0, d2(T.k), gauss_,
+                        P.cfie_alpha, F.Vs, F.Vf);
+    } else {
+        const FarDev<double>& G = wd_[w].f64;
+        launch_narrow(st, (size_t)N * 3 * K0, G.Vs, F.Vs);
+        launch_narrow(st, (size_t)N * 3 * K0, G.Vf, F.Vf);
+    }
     MLFMA_HIP(hipGetLastError());
     for (int l = 2; l <= L; ++l) {
         const size_t ws = (size_t)tree.lv[l].n * ncur * 3 * D.K[l];
-        D.Sm[l] = dalloc<double2>(ws);
-        D.Bm[l] = dalloc<double2>(ws);
+        F.Sm[l] = dalloc<cplx_t<R>>(ws);
+        F.Bm[l] = dalloc<cplx_t<R>>(ws);
         // quadrature weight w_theta * dphi folded into the translation operator
         // (the reference applies it inside m2l, Code/ProcessMLFMM.h:138-142)
-        std::vector<double2> tfw(T.tf[l].size());
+        std::vector<cplx_t<R>> tfw(T.tf[l].size());
         const int K = D.K[l];
-        for (size_t e = 0; e < tfw.size(); ++e) tfw[e] = d2(T.tf[l][e] * T.qw[l][e % K]);
-        D.TFw[l] = upload(tfw);
+        for (size_t e = 0; e < tfw.size(); ++e) tfw[e] = to_cx<R>(T.tf[l][e] * T.qw[l][e % K]);
+        F.TFw[l] = upload(tfw);
     }
     for (int c = 3; c <= L; ++c) {
         const InterpTable& it = T.interp[c];
-        D.tap[c] = upload(it.tap);
-        D.tw[c] = upload(it.w);
-        D.t_ptr[c] = upload(it.t_ptr);
-        D.t_kp[c] = upload(it.t_kp);
-        D.t_w[c] = upload(it.t_w);
-        std::vector<double2> su(T.shift_up[c].size()), sd(T.shift_down[c].size());
-        for (size_t e = 0; e < su.size(); ++e) su[e] = d2(T.shift_up[c][e]);
-        for (size_t e = 0; e < sd.size(); ++e) sd[e] = d2(T.shift_down[c][e]);
-        D.sh_up[c] = upload(su);
-        D.sh_dn[c] = upload(sd);
+        F.tw[c] = upload(to_real<R>(it.w));
+        F.t_w[c] = upload(to_real<R>(it.t_w));
+        std::vector<cplx_t<R>> su(T.shift_up[c].size()), sd(T.shift_down[c].size());
+        for (size_t e = 0; e < su.size(); ++e) su[e] = to_cx<R>(T.shift_up[c][e]);
+        for (size_t e = 0; e < sd.size(); ++e) sd[e] = to_cx<R>(T.shift_down[c][e]);
+        F.sh_up[c] = upload(su);
+        F.sh_dn[c] = upload(sd);
     }
 }
 
@@ -309,6 +349,11 @@ void Engine::build_near() {
     c.alpha = P.cfie_alpha;
     launch_near_fill(st, ie, nd_, d_geo_, gauss_, c, d_Z_);
     MLFMA_HIP(hipGetLastError());
+    if (P.near_precision == MLFMA_FP32) {
+        d_Z32_ = dalloc<float2>((size_t)near_entries);
+        launch_narrow(st, (size_t)near_entries, d_Z_, d_Z32_);
+        MLFMA_HIP(hipGetLastError());
+    }
 }
 
 // Copy of one leaf-pair block, planes x (ns x nt column-major).
@@ -325,25 +370,57 @@ std::vector<double2> Engine::near_block(int p) {
 }
 
 // mexp, m2m (fine -> coarse), then m2l with the gather-form l2l (coarse -> fine)
+template <typename R>
 void Engine::far_pass(int w, const double2* x) {
     WaveDev& D = wd_[w];
+    FarDev<R>& F = far<R>(w);
     const int L = tree.max_level;
     mark(6 * w + 0);
-    launch_aggregate(st, ncur, nleaf, D.K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_, D.Vs, x, D.Sm[L]);
+    launch_aggregate<R>(st, ncur, nleaf, D.K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_, F.Vs, x, F.Sm[L]);
     mark(6 * w + 1);
     mark(6 * w + 2);
     for (int l = L - 1; l >= 2; --l)
-        launch_m2m(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
-                   D.tap[l + 1], D.tw[l + 1], D.sh_up[l + 1], D.Sm[l + 1], D.Sm[l]);
+        launch_m2m<R>(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
+                      D.tap[l + 1], F.tw[l + 1], F.sh_up[l + 1], F.Sm[l + 1], F.Sm[l]);
     mark(6 * w + 3);
     mark(6 * w + 4);
     for (int l = 2; l <= L; ++l) {
         const bool up = l > 2;
-        launch_m2l_l2l(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, D.TFw[l], D.Sm[l],
-                       up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l], D.t_w[l], D.sh_dn[l],
-                       up ? D.Bm[l - 1] : nullptr, D.Bm[l]);
+        launch_m2l_l2l<R>(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, F.TFw[l],
+                          F.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l], F.t_w[l],
+                          F.sh_dn[l], up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
     }
     mark(6 * w + 5);
+}
+
+// y += far field (finest Bm of every wavenumber)
+template <typename R>
+void Engine::disaggregate(double2* y) {
+    const int L = tree.max_level;
+    if (ie != MLFMA_PMCHWT) {
+        // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
+        const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
+        const FarDev<R>& F = far<R>(0);
+        launch_disagg_pec<R>(st, N, wd_[0].K[L], d_leaf_of_, d_perm_, F.Vf, F.Bm[L], d2(c), y);
+    } else {
+        const FarDev<R>& F1 = far<R>(0);
+        const FarDev<R>& F2 = far<R>(1);
+        PmchwtFarArgs<R> a;
+        a.K1 = wd_[0].K[L];
+        a.K2 = wd_[1].K[L];
+        a.N = N;
+        a.Vf1 = F1.Vf;
+        a.Vf2 = F2.Vf;
+        a.B1 = F1.Bm[L];
+        a.B2 = F2.Bm[L];
+        a.kp1 = F1.kp0;
+        a.kp2 = F2.kp0;
+        a.c1 = d2(med.k1 * med.k1 / (16.0 * kPi * kPi));   // Code/MLFMM.cpp:252-253
+        a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
+        a.etai = d2(med.etai);
+        a.inv_etai = d2(1.0 / med.etai);
+        launch_disagg_pmchwt<R>(st, d_leaf_of_, d_perm_, a, y);
+    }
 }
 
 void Engine::mark(int slot) {
@@ -381,90 +458,75 @@ void Engine::collect_profile() {
     prof_apply += 1;
 }
 
-void Engine::apply_one(const double2* x, double2* y) {
-    const int L = tree.max_level;
-    for (int w = 0; w < nw; ++w) far_pass(w, x);
+void Engine::apply_one(const double2* x, double2* y, int prec) {
+    const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
+    for (int w = 0; w < nw; ++w) {
+        if (f32) far_pass<float>(w, x);
+        else far_pass<double>(w, x);
+    }
     mark(12);
-    launch_near(st, nd_, d_Z_, x, y);
+    if (prec == MLFMA_FP32 && d_Z32_) launch_near<float>(st, nd_, d_Z32_, x, y);
+    else launch_near<double>(st, nd_, d_Z_, x, y);
     mark(13);
     mark(14);
-    if (ie != MLFMA_PMCHWT) {
-        // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
-        const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
-        launch_disagg_pec(st, N, wd_[0].K[L], d_leaf_of_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), y);
-    } else {
-        PmchwtFarArgs a;
-        a.K1 = wd_[0].K[L];
-        a.K2 = wd_[1].K[L];
-        a.N = N;
-        a.Vf1 = wd_[0].Vf;
-        a.Vf2 = wd_[1].Vf;
-        a.B1 = wd_[0].Bm[L];
-        a.B2 = wd_[1].Bm[L];
-        a.kp1 = wd_[0].kp0;
-        a.kp2 = wd_[1].kp0;
-        a.c1 = d2(med.k1 * med.k1 / (16.0 * kPi * kPi));   // Code/MLFMM.cpp:252-253
-        a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
-        a.etai = d2(med.etai);
-        a.inv_etai = d2(1.0 / med.etai);
-        launch_disagg_pmchwt(st, d_leaf_of_, d_perm_, a, y);
-    }
+    if (f32) disaggregate<float>(y);
+    else disaggregate<double>(y);
     mark(15);
 }
 
 void Engine::apply_device(const double2* x, double2* y, int nrhs) {
     for (int r = 0; r < nrhs; ++r) {
-        apply_one(x + (size_t)r * nunk, y + (size_t)r * nunk);
+        apply_one(x + (size_t)r * nunk, y + (size_t)r * nunk, apply_prec);
         if (prof_on_) collect_profile();
     }
     MLFMA_HIP(hipGetLastError());
 }
 
+void Engine::set_apply_precision(int prec) {
+    if (prec != MLFMA_FP64 && prec != MLFMA_FP32) throw Error(MLFMA_ERR_ARG, "unknown precision");
+    if (prec == MLFMA_FP32 && P.far_precision != MLFMA_FP32 && P.near_precision != MLFMA_FP32)
+        throw Error(MLFMA_ERR_ARG, "no FP32 tables built (create with far_precision or near_precision = MLFMA_FP32)");
+    apply_prec = prec;
+}
+
+// Single stages of the FP64 operator (parity inspection).
 void Engine::stage(int which, const double2* dx, double2* dy) {
     const int L = tree.max_level;
     switch (which) {
         case MLFMA_STAGE_AGGREGATE:
             for (int w = 0; w < nw; ++w)
-                launch_aggregate(st, ncur, nleaf, wd_[w].K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_, wd_[w].Vs, dx, wd_[w].Sm[L]);
+                launch_aggregate<double>(st, ncur, nleaf, wd_[w].K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_,
+                                         wd_[w].f64.Vs, dx, wd_[w].f64.Sm[L]);
             break;
         case MLFMA_STAGE_M2M:
             for (int w = 0; w < nw; ++w) {
                 WaveDev& D = wd_[w];
+                const FarDev<double>& F = D.f64;
                 for (int l = L - 1; l >= 2; --l)
-                    launch_m2m(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
-                               D.tap[l + 1], D.tw[l + 1], D.sh_up[l + 1], D.Sm[l + 1], D.Sm[l]);
+                    launch_m2m<double>(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx,
+                                       lv_[l + 1].oct, D.tap[l + 1], F.tw[l + 1], F.sh_up[l + 1], F.Sm[l + 1],
+                                       F.Sm[l]);
             }
             break;
         case MLFMA_STAGE_M2L_L2L:
             for (int w = 0; w < nw; ++w) {
                 WaveDev& D = wd_[w];
+                const FarDev<double>& F = D.f64;
                 for (int l = 2; l <= L; ++l) {
                     const bool up = l > 2;
-                    launch_m2l_l2l(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, D.TFw[l],
-                                   D.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l],
-                                   D.t_w[l], D.sh_dn[l], up ? D.Bm[l - 1] : nullptr, D.Bm[l]);
+                    launch_m2l_l2l<double>(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec,
+                                           F.TFw[l], F.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct,
+                                           D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l], up ? F.Bm[l - 1] : nullptr,
+                                           F.Bm[l]);
                 }
             }
             break;
-        case MLFMA_STAGE_FAR: {
+        case MLFMA_STAGE_FAR:
             MLFMA_HIP(hipMemsetAsync(dy, 0, sizeof(double2) * nunk, st));
-            if (ie != MLFMA_PMCHWT) {
-                const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
-                launch_disagg_pec(st, N, wd_[0].K[L], d_leaf_of_, d_perm_, wd_[0].Vf, wd_[0].Bm[L], d2(c), dy);
-            } else {
-                PmchwtFarArgs a;
-                a.K1 = wd_[0].K[L], a.K2 = wd_[1].K[L], a.N = N;
-                a.Vf1 = wd_[0].Vf, a.Vf2 = wd_[1].Vf, a.B1 = wd_[0].Bm[L], a.B2 = wd_[1].Bm[L];
-                a.kp1 = wd_[0].kp0, a.kp2 = wd_[1].kp0;
-                a.c1 = d2(med.k1 * med.k1 / (16.0 * kPi * kPi));
-                a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
-                a.etai = d2(med.etai), a.inv_etai = d2(1.0 / med.etai);
-                launch_disagg_pmchwt(st, d_leaf_of_, d_perm_, a, dy);
-            }
+            disaggregate<double>(dy);
             break;
-        }
         case MLFMA_STAGE_NEAR:
-            launch_near(st, nd_, d_Z_, dx, dy);
+            launch_near<double>(st, nd_, d_Z_, dx, dy);
             break;
         default: throw Error(MLFMA_ERR_ARG, "unknown stage");
     }
@@ -506,19 +568,20 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
     }
     const bool pc = sp.precond != 0;
     if (pc) precond_setup();
-    int matvecs = 0, total = 0;
+    int matvecs = 0, fp64_matvecs = 0, total = 0;
     auto A = [&](const double2* in, double2* out) {
         if (pc) {
             precond_apply(in, d_t_);
-            apply_one(d_t_, out);
+            apply_one(d_t_, out, apply_prec);
         } else {
-            apply_one(in, out);
+            apply_one(in, out, apply_prec);
         }
         ++matvecs;
     };
     auto true_residual = [&]() {
-        apply_one(d_x, d_r_);
+        apply_one(d_x, d_r_, MLFMA_FP64);
         ++matvecs;
+        ++fp64_matvecs;
         launch_residual(st, n, d_b, d_r_);
         return norm(d_r_, n);
     };
@@ -653,7 +716,7 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
             if (pc) {
                 precond_apply(uq, tmp);
                 launch_axpy(st, n, d2(al), tmp, d_x, d_x);
-                apply_one(tmp, Q);
+                apply_one(tmp, Q, apply_prec);
                 ++matvecs;
             } else {
                 launch_axpy(st, n, d2(al), uq, d_x, d_x);
@@ -672,6 +735,7 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
     }
     stats.iterations = total;
     stats.matvecs = matvecs;
+    stats.fp64_matvecs = fp64_matvecs;
     stats.converged = converged ? 1 : 0;
     stats.rel_residual = err;
     stats.seconds = wall() - t0;
@@ -802,7 +866,7 @@ void Engine::get_workspace(int wn, int cur, int which, int level, int node, doub
         throw Error(MLFMA_ERR_ARG, "workspace index out of range");
     const WaveDev& D = wd_[wn - 1];
     const int K = D.K[level];
-    const double2* src = (which == 0 ? D.Sm[level] : D.Bm[level]) + ((size_t)node * ncur + cur) * 3 * K;
+    const double2* src = (which == 0 ? D.f64.Sm[level] : D.f64.Bm[level]) + ((size_t)node * ncur + cur) * 3 * K;
     std::vector<double2> h(3 * (size_t)K);
     MLFMA_HIP(hipMemcpy(h.data(), src, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
     for (int q = 0; q < K; ++q)
@@ -818,7 +882,7 @@ void Engine::get_pattern(int wn, int which, int rwg, double* out) {
     const int K = D.K[tree.max_level];
     const int i = inv_perm[rwg];
     std::vector<double2> h(3 * (size_t)K);
-    MLFMA_HIP(hipMemcpy(h.data(), (which == 0 ? D.Vs : D.Vf) + (size_t)i * 3 * K, h.size() * sizeof(double2),
+    MLFMA_HIP(hipMemcpy(h.data(), (which == 0 ? D.f64.Vs : D.f64.Vf) + (size_t)i * 3 * K, h.size() * sizeof(double2),
                         hipMemcpyDeviceToHost));
     for (int q = 0; q < K; ++q)
         for (int c = 0; c < 3; ++c) {

@@ -8,6 +8,7 @@
 
 #include "../../include/mlfma.h"
 #include "geo.hpp"
+#include "kernels.hpp"
 #include "model.hpp"
 
 namespace mlfma {
@@ -36,15 +37,22 @@ struct LevelDev {
     int* oct = nullptr;   // Morton octant of each node inside its parent
 };
 
+// Far-field tables and workspaces of one wavenumber in real type R.
+template <typename R>
+struct FarDev {
+    cplx_t<R>* Vs = nullptr;                 // [N][3][K0], internal RWG order
+    cplx_t<R>* Vf = nullptr;
+    R* kp0 = nullptr;                    // [3][K0] finest-level directions
+    std::vector<cplx_t<R>*> Sm, Bm, TFw;     // per tree level
+    std::vector<R*> tw, t_w;             // per child tree level
+    std::vector<cplx_t<R>*> sh_up, sh_dn;
+};
+
 struct WaveDev {
     std::vector<int> K;                  // per tree level
-    double2* Vs = nullptr;               // [N][3][K0], internal RWG order
-    double2* Vf = nullptr;
-    double* kp0 = nullptr;               // [3][K0] finest-level directions
-    std::vector<double2*> Sm, Bm, TFw;   // per tree level
     std::vector<int*> tap, t_ptr, t_kp;  // per child tree level
-    std::vector<double*> tw, t_w;
-    std::vector<double2*> sh_up, sh_dn;
+    FarDev<double> f64;                  // always built (true residuals, stage API)
+    FarDev<float> f32;                   // built when far_precision = MLFMA_FP32
 };
 
 class Engine {
@@ -56,8 +64,11 @@ public:
 
     void build(int np, const int* ids, const double* xyz, int nf, const int* fids, const int* fvid);
 
-    // y = A x for nrhs device vectors (column-major, n_unknowns each)
+    // y = A x for nrhs device vectors (column-major, n_unknowns each), far
+    // field in the apply precision
     void apply_device(const double2* x, double2* y, int nrhs);
+    void set_apply_precision(int prec);
+    int apply_prec = MLFMA_FP64;
     void stage(int which, const double2* dx, double2* dy);
     void solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, mlfma_solve_stats& st);
     void excitation(double pol, double th, double ph, double2* d_v);
@@ -102,10 +113,17 @@ private:
     void build_device_tree();
     void build_wave(int w);
     void build_near();
+    template <typename R>
+    FarDev<R>& far(int w);
+    template <typename R>
+    void build_far(int w, const WaveTables& T);
+    template <typename R>
     void far_pass(int w, const double2* x);
+    template <typename R>
+    void disaggregate(double2* y);
     void mark(int slot);
     void collect_profile();
-    void apply_one(const double2* x, double2* y);
+    void apply_one(const double2* x, double2* y, int prec);
     std::vector<double2> near_block(int p);   // planes x (ns x nt column-major) of pair p
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
@@ -123,6 +141,7 @@ private:
     GaussRule gauss_{};
     NearDev nd_{};
     double2* d_Z_ = nullptr;
+    float2* d_Z32_ = nullptr;   // FP32 copy (near_precision = MLFMA_FP32)
     // solver scratch
     double2* d_V_ = nullptr;
     int V_cols_ = 0;

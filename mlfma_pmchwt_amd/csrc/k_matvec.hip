@@ -1,9 +1,11 @@
-// FP64 MLFMA matvec kernels for gfx950 (64-wide wavefronts).
+// FP64 / FP32 MLFMA matvec kernels for gfx950 (64-wide wavefronts).
 //
 // Workspaces are laid out [box][cur][comp][K] so that consecutive threads
 // (consecutive k-space samples) touch consecutive addresses.  RWG functions are
 // stored in leaf-major order (internal index i); perm[i] is the reference index
-// used by the caller's x and y vectors.
+// used by the caller's x and y vectors.  The far-field kernels are templates on
+// the real type R of the patterns, tables and workspaces (double or float);
+// they accumulate in R.  Solver vectors x, y and the near field stay FP64.
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -29,41 +31,42 @@ __device__ __forceinline__ int xcd_remap(int lin, int total) {
 // dependent perm -> x loads out of the pattern stream.
 constexpr int kAggChunk = 128;
 
-template <int NC>
+template <typename R, int NC>
 __global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __restrict__ order,
                                                    const int* __restrict__ leaf_ptr, const int* __restrict__ perm,
-                                                   const double2* __restrict__ Vs, const double2* __restrict__ x,
-                                                   double2* __restrict__ Sm) {
-    __shared__ double2 xl[NC][kAggChunk];
+                                                   const cplx_t<R>* __restrict__ Vs, const double2* __restrict__ x,
+                                                   cplx_t<R>* __restrict__ Sm) {
+    using C = cplx_t<R>;
+    __shared__ C xl[NC][kAggChunk];
     const int leaf = order[blockIdx.x];
     const int i0 = leaf_ptr[leaf], i1 = leaf_ptr[leaf + 1];
     for (int q0 = 0; q0 < K; q0 += blockDim.x) {
         const int q = q0 + threadIdx.x;
         const bool act = q < K;
-        double2 acc[NC][3];
+        C acc[NC][3];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) acc[c][d] = make_double2(0.0, 0.0);
+            for (int d = 0; d < 3; ++d) acc[c][d] = C{0, 0};
         for (int b = i0; b < i1; b += kAggChunk) {
             const int n = min(kAggChunk, i1 - b);
             __syncthreads();
             for (int j = threadIdx.x; j < n; j += blockDim.x) {
                 const int r = perm[b + j];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) xl[c][j] = x[(size_t)c * N + r];
+                for (int c = 0; c < NC; ++c) narrow(x[(size_t)c * N + r], xl[c][j]);
             }
             __syncthreads();
             if (act) {
-                const double2* v = Vs + (size_t)b * 3 * K + q;
+                const C* v = Vs + (size_t)b * 3 * K + q;
                 const size_t s3 = (size_t)3 * K;
                 int j = 0;
                 for (; j + 1 < n; j += 2) {
-                    const double2 a0 = v[0], a1 = v[K], a2 = v[2 * K];
-                    const double2 b0 = v[s3], b1 = v[s3 + K], b2 = v[s3 + 2 * K];
+                    const C a0 = v[0], a1 = v[K], a2 = v[2 * K];
+                    const C b0 = v[s3], b1 = v[s3 + K], b2 = v[s3 + 2 * K];
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
-                        const double2 xa = xl[c][j], xb = xl[c][j + 1];
+                        const C xa = xl[c][j], xb = xl[c][j + 1];
                         cfma(acc[c][0], a0, xa);
                         cfma(acc[c][1], a1, xa);
                         cfma(acc[c][2], a2, xa);
@@ -74,10 +77,10 @@ __global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __re
                     v += 2 * s3;
                 }
                 if (j < n) {
-                    const double2 a0 = v[0], a1 = v[K], a2 = v[2 * K];
+                    const C a0 = v[0], a1 = v[K], a2 = v[2 * K];
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
-                        const double2 xa = xl[c][j];
+                        const C xa = xl[c][j];
                         cfma(acc[c][0], a0, xa);
                         cfma(acc[c][1], a1, xa);
                         cfma(acc[c][2], a2, xa);
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __re
             }
         }
         if (act) {
-            double2* s = Sm + (size_t)leaf * NC * 3 * K + q;
+            C* s = Sm + (size_t)leaf * NC * 3 * K + q;
 #pragma unroll
             for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -95,72 +98,77 @@ __global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __re
     }
 }
 
+template <typename R>
 void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const int* order, const int* leaf_ptr,
-                      const int* perm, const double2* Vs, const double2* x, double2* Sm) {
+                      const int* perm, const cplx_t<R>* Vs, const double2* x, cplx_t<R>* Sm) {
     if (nleaf == 0) return;
     const int threads = std::min(512, (K + 63) / 64 * 64);
-    if (ncur == 1) k_aggregate<1><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
-    else k_aggregate<2><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
+    if (ncur == 1) k_aggregate<R, 1><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
+    else k_aggregate<R, 2><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
 }
 
 // ---------------------------------------------------------------- interpolation
 // Code/ProcessMLFMM.h:61-107 (m2m): 3x3 Lagrange taps into the child grid, then
 // the octant phase shift, summed over the children of each parent.
-template <int NC>
+template <typename R, int NC>
 __global__ __launch_bounds__(256) void k_m2m(int Kp, int Kc, const int* __restrict__ son_ptr,
                                              const int* __restrict__ son_idx, const int* __restrict__ oct_child,
-                                             const int* __restrict__ tap, const double* __restrict__ w,
-                                             const double2* __restrict__ shup, const double2* __restrict__ Smc,
-                                             double2* __restrict__ Smp) {
+                                             const int* __restrict__ tap, const R* __restrict__ w,
+                                             const cplx_t<R>* __restrict__ shup, const cplx_t<R>* __restrict__ Smc,
+                                             cplx_t<R>* __restrict__ Smp) {
+    using C = cplx_t<R>;
     const int p = blockIdx.y;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= Kp) return;
     int t[9];
-    double ww[9];
+    R ww[9];
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
         t[s] = tap[(size_t)s * Kp + q];
         ww[s] = w[(size_t)s * Kp + q];
     }
-    double2 acc[NC][3];
+    C acc[NC][3];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) acc[c][d] = make_double2(0.0, 0.0);
+        for (int d = 0; d < 3; ++d) acc[c][d] = C{0, 0};
     for (int j = son_ptr[p]; j < son_ptr[p + 1]; ++j) {
         const int c = son_idx[j];
-        const double2 e = shup[(size_t)oct_child[c] * Kp + q];
-        const double2* S = Smc + (size_t)c * NC * 3 * Kc;
+        const C e = shup[(size_t)oct_child[c] * Kp + q];
+        const C* S = Smc + (size_t)c * NC * 3 * Kc;
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double2* s = S + (size_t)(cc * 3 + d) * Kc;
-                double2 r = make_double2(0.0, 0.0);
+                const C* s = S + (size_t)(cc * 3 + d) * Kc;
+                C r = C{0, 0};
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
-                    const double2 f = s[t[k]];
+                    const C f = s[t[k]];
                     r.x += f.x * ww[k];
                     r.y += f.y * ww[k];
                 }
                 cfma(acc[cc][d], e, r);
             }
     }
-    double2* o = Smp + (size_t)p * NC * 3 * Kp + q;
+    C* o = Smp + (size_t)p * NC * 3 * Kp + q;
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
         for (int d = 0; d < 3; ++d) o[(size_t)(cc * 3 + d) * Kp] = acc[cc][d];
 }
 
+template <typename R>
 void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int* son_ptr, const int* son_idx,
-                const int* octant_child, const int* tap, const double* w, const double2* shift_up,
-                const double2* Sm_child, double2* Sm_parent) {
+                const int* octant_child, const int* tap, const R* w, const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child,
+                cplx_t<R>* Sm_parent) {
     dim3 grid((Kp + 255) / 256, nparent);
     if (ncur == 1)
-        k_m2m<1><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child, Sm_parent);
+        k_m2m<R, 1><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child,
+                                          Sm_parent);
     else
-        k_m2m<2><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child, Sm_parent);
+        k_m2m<R, 2><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child,
+                                          Sm_parent);
 }
 
 // ---------------------------------------------------------------- translation + anterpolation
@@ -168,26 +176,27 @@ void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int
 // :152-202 (l2l) in gather form: each child sample sums the parent samples whose
 // 9-tap stencil contains it (transposed table, ascending parent sample), so no
 // atomics and a fixed summation order.
-template <int NC>
+template <typename R, int NC>
 __global__ __launch_bounds__(256) void k_m2l_l2l(int nbox, int K, int ktiles, const int* __restrict__ dr_ptr,
                                                  const int* __restrict__ dr_idx, const int* __restrict__ dr_vec,
-                                                 const double2* __restrict__ TFw, const double2* __restrict__ Sm,
+                                                 const cplx_t<R>* __restrict__ TFw, const cplx_t<R>* __restrict__ Sm,
                                                  int Kp, const int* __restrict__ parent, const int* __restrict__ oct,
                                                  const int* __restrict__ t_ptr, const int* __restrict__ t_kp,
-                                                 const double* __restrict__ t_w, const double2* __restrict__ shdn,
-                                                 const double2* __restrict__ Bmp, double2* __restrict__ Bm) {
+                                                 const R* __restrict__ t_w, const cplx_t<R>* __restrict__ shdn,
+                                                 const cplx_t<R>* __restrict__ Bmp, cplx_t<R>* __restrict__ Bm) {
+    using C = cplx_t<R>;
     const int lin = xcd_remap(blockIdx.x, nbox * ktiles);
     const int b = lin / ktiles;
     const int q = (lin % ktiles) * blockDim.x + threadIdx.x;
     if (q >= K) return;
-    double2 acc[NC][3];
+    C acc[NC][3];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) acc[c][d] = make_double2(0.0, 0.0);
+        for (int d = 0; d < 3; ++d) acc[c][d] = C{0, 0};
     for (int j = dr_ptr[b]; j < dr_ptr[b + 1]; ++j) {
-        const double2 T = TFw[(size_t)dr_vec[j] * K + q];
-        const double2* S = Sm + (size_t)dr_idx[j] * NC * 3 * K + q;
+        const C T = TFw[(size_t)dr_vec[j] * K + q];
+        const C* S = Sm + (size_t)dr_idx[j] * NC * 3 * K + q;
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
@@ -195,11 +204,11 @@ __global__ __launch_bounds__(256) void k_m2l_l2l(int nbox, int K, int ktiles, co
     }
     if (Bmp != nullptr) {
         const int pb = parent[b];
-        const double2* sh = shdn + (size_t)oct[b] * Kp;
-        const double2* P = Bmp + (size_t)pb * NC * 3 * Kp;
+        const C* sh = shdn + (size_t)oct[b] * Kp;
+        const C* P = Bmp + (size_t)pb * NC * 3 * Kp;
         for (int j = t_ptr[q]; j < t_ptr[q + 1]; ++j) {
             const int kp = t_kp[j];
-            double2 e = sh[kp];
+            C e = sh[kp];
             e.x *= t_w[j];
             e.y *= t_w[j];
 #pragma unroll
@@ -208,25 +217,26 @@ __global__ __launch_bounds__(256) void k_m2l_l2l(int nbox, int K, int ktiles, co
                 for (int d = 0; d < 3; ++d) cfma(acc[cc][d], e, P[(size_t)(cc * 3 + d) * Kp + kp]);
         }
     }
-    double2* o = Bm + (size_t)b * NC * 3 * K + q;
+    C* o = Bm + (size_t)b * NC * 3 * K + q;
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
         for (int d = 0; d < 3; ++d) o[(size_t)(cc * 3 + d) * K] = acc[cc][d];
 }
 
+template <typename R>
 void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr, const int* dr_idx,
-                    const int* dr_vec, const double2* TFw, const double2* Sm, int Kp, const int* parent,
-                    const int* octant, const int* t_ptr, const int* t_kp, const double* t_w,
-                    const double2* shift_down, const double2* Bm_parent, double2* Bm) {
+                    const int* dr_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp, const int* parent,
+                    const int* octant, const int* t_ptr, const int* t_kp, const R* t_w, const cplx_t<R>* shift_down,
+                    const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
     const int kt = (K + 255) / 256;
     dim3 grid(kt * nbox);
     if (ncur == 1)
-        k_m2l_l2l<1><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant, t_ptr,
-                                           t_kp, t_w, shift_down, Bm_parent, Bm);
+        k_m2l_l2l<R, 1><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant,
+                                              t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm);
     else
-        k_m2l_l2l<2><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant, t_ptr,
-                                           t_kp, t_w, shift_down, Bm_parent, Bm);
+        k_m2l_l2l<R, 2><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant,
+                                              t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm);
 }
 
 // ---------------------------------------------------------------- disaggregation
@@ -237,22 +247,24 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
 // the GPU alike.
 constexpr int kDisRows = 4;
 
+template <typename R>
 __global__ __launch_bounds__(256) void k_disagg_pec(int N, int K, const int* __restrict__ leaf_of,
-                                                    const int* __restrict__ perm, const double2* __restrict__ Vf,
-                                                    const double2* __restrict__ Bm, double2 c,
+                                                    const int* __restrict__ perm, const cplx_t<R>* __restrict__ Vf,
+                                                    const cplx_t<R>* __restrict__ Bm, double2 c,
                                                     double2* __restrict__ y) {
+    using C = cplx_t<R>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r0 = (blockIdx.x * 4 + wave) * kDisRows;
     if (r0 >= N) return;
-    const double2* V[kDisRows];
-    const double2* B[kDisRows];
-    double2 s[kDisRows];
+    const C* V[kDisRows];
+    const C* B[kDisRows];
+    C s[kDisRows];
 #pragma unroll
     for (int u = 0; u < kDisRows; ++u) {
         const int i = min(r0 + u, N - 1);   // rows past N repeat row N-1 and are not written
         V[u] = Vf + (size_t)i * 3 * K;
         B[u] = Bm + (size_t)leaf_of[i] * 3 * K;
-        s[u] = make_double2(0.0, 0.0);
+        s[u] = C{0, 0};
     }
     for (int q = lane; q < K; q += 64) {
 #pragma unroll
@@ -265,33 +277,36 @@ __global__ __launch_bounds__(256) void k_disagg_pec(int N, int K, const int* __r
 #pragma unroll
     for (int u = 0; u < kDisRows; ++u) s[u] = wave_sum(s[u]);
     if (lane < kDisRows && r0 + lane < N) {
-        double2 v = s[0];
+        C v = s[0];
 #pragma unroll
         for (int u = 1; u < kDisRows; ++u)
             if (lane == u) v = s[u];
         const int r = perm[r0 + lane];
         double2 t = y[r];
-        cfma(t, c, v);
+        cfma(t, c, widen(v));
         y[r] = t;
     }
 }
 
-void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const double2* Vf,
-                       const double2* Bm, double2 c, double2* y) {
+template <typename R>
+void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const cplx_t<R>* Vf,
+                       const cplx_t<R>* Bm, double2 c, double2* y) {
     const int rows_per_block = 4 * kDisRows;
-    k_disagg_pec<<<(N + rows_per_block - 1) / rows_per_block, 256, 0, st>>>(N, K, leaf_of, perm, Vf, Bm, c, y);
+    k_disagg_pec<R><<<(N + rows_per_block - 1) / rows_per_block, 256, 0, st>>>(N, K, leaf_of, perm, Vf, Bm, c, y);
 }
 
 // Code/MLFMM.cpp:285-351.  The K integral sum (-khat x V).B with khat = -kp is
 // evaluated as sum V.(kp x B).
-__device__ __forceinline__ void far_sums(int K, const double2* __restrict__ V, const double2* __restrict__ BJ,
-                                         const double2* __restrict__ BM, const double* __restrict__ kp, int lane,
-                                         double2& LJ, double2& KJ, double2& LM, double2& KM) {
+template <typename R>
+__device__ __forceinline__ void far_sums(int K, const cplx_t<R>* __restrict__ V, const cplx_t<R>* __restrict__ BJ,
+                                         const cplx_t<R>* __restrict__ BM, const R* __restrict__ kp, int lane,
+                                         cplx_t<R>& LJ, cplx_t<R>& KJ, cplx_t<R>& LM, cplx_t<R>& KM) {
+    using C = cplx_t<R>;
     for (int q = lane; q < K; q += 64) {
-        const double kx = kp[q], ky = kp[K + q], kz = kp[2 * K + q];
-        const double2 v0 = V[q], v1 = V[K + q], v2 = V[2 * K + q];
-        const double2 j0 = BJ[q], j1 = BJ[K + q], j2 = BJ[2 * K + q];
-        const double2 m0 = BM[q], m1 = BM[K + q], m2 = BM[2 * K + q];
+        const R kx = kp[q], ky = kp[K + q], kz = kp[2 * K + q];
+        const C v0 = V[q], v1 = V[K + q], v2 = V[2 * K + q];
+        const C j0 = BJ[q], j1 = BJ[K + q], j2 = BJ[2 * K + q];
+        const C m0 = BM[q], m1 = BM[K + q], m2 = BM[2 * K + q];
         cfma(LJ, v0, j0);
         cfma(LJ, v1, j1);
         cfma(LJ, v2, j2);
@@ -299,12 +314,12 @@ __device__ __forceinline__ void far_sums(int K, const double2* __restrict__ V, c
         cfma(LM, v1, m1);
         cfma(LM, v2, m2);
         // kp x B
-        const double2 cj0 = csub(cscale(j2, ky), cscale(j1, kz));
-        const double2 cj1 = csub(cscale(j0, kz), cscale(j2, kx));
-        const double2 cj2 = csub(cscale(j1, kx), cscale(j0, ky));
-        const double2 cm0 = csub(cscale(m2, ky), cscale(m1, kz));
-        const double2 cm1 = csub(cscale(m0, kz), cscale(m2, kx));
-        const double2 cm2 = csub(cscale(m1, kx), cscale(m0, ky));
+        const C cj0 = csub(cscale(j2, ky), cscale(j1, kz));
+        const C cj1 = csub(cscale(j0, kz), cscale(j2, kx));
+        const C cj2 = csub(cscale(j1, kx), cscale(j0, ky));
+        const C cm0 = csub(cscale(m2, ky), cscale(m1, kz));
+        const C cm1 = csub(cscale(m0, kz), cscale(m2, kx));
+        const C cm2 = csub(cscale(m1, kx), cscale(m0, ky));
         cfma(KJ, v0, cj0);
         cfma(KJ, v1, cj1);
         cfma(KJ, v2, cj2);
@@ -315,20 +330,22 @@ __device__ __forceinline__ void far_sums(int K, const double2* __restrict__ V, c
 }
 
 // One wavefront per RWG row over the whole RWG range (four rows per block).
+template <typename R>
 __global__ __launch_bounds__(256) void k_disagg_pmchwt(const int* __restrict__ leaf_of, const int* __restrict__ perm,
-                                                       PmchwtFarArgs a, double2* __restrict__ y) {
+                                                       PmchwtFarArgs<R> a, double2* __restrict__ y) {
+    using C = cplx_t<R>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wave;
     if (i >= a.N) return;
     const int leaf = leaf_of[i];
-    const double2* BJ1 = a.B1 + (size_t)leaf * 6 * a.K1;
-    const double2* BM1 = BJ1 + 3 * (size_t)a.K1;
-    const double2* BJ2 = a.B2 + (size_t)leaf * 6 * a.K2;
-    const double2* BM2 = BJ2 + 3 * (size_t)a.K2;
-    double2 L1J = {0, 0}, K1J = {0, 0}, L1M = {0, 0}, K1M = {0, 0};
-    double2 L2J = {0, 0}, K2J = {0, 0}, L2M = {0, 0}, K2M = {0, 0};
-    far_sums(a.K1, a.Vf1 + (size_t)i * 3 * a.K1, BJ1, BM1, a.kp1, lane, L1J, K1J, L1M, K1M);
-    far_sums(a.K2, a.Vf2 + (size_t)i * 3 * a.K2, BJ2, BM2, a.kp2, lane, L2J, K2J, L2M, K2M);
+    const C* BJ1 = a.B1 + (size_t)leaf * 6 * a.K1;
+    const C* BM1 = BJ1 + 3 * (size_t)a.K1;
+    const C* BJ2 = a.B2 + (size_t)leaf * 6 * a.K2;
+    const C* BM2 = BJ2 + 3 * (size_t)a.K2;
+    C L1J = {0, 0}, K1J = {0, 0}, L1M = {0, 0}, K1M = {0, 0};
+    C L2J = {0, 0}, K2J = {0, 0}, L2M = {0, 0}, K2M = {0, 0};
+    far_sums<R>(a.K1, a.Vf1 + (size_t)i * 3 * a.K1, BJ1, BM1, a.kp1, lane, L1J, K1J, L1M, K1M);
+    far_sums<R>(a.K2, a.Vf2 + (size_t)i * 3 * a.K2, BJ2, BM2, a.kp2, lane, L2J, K2J, L2M, K2M);
     L1J = wave_sum(L1J);
     K1J = wave_sum(K1J);
     L1M = wave_sum(L1M);
@@ -341,59 +358,77 @@ __global__ __launch_bounds__(256) void k_disagg_pmchwt(const int* __restrict__ l
         const int r = perm[i];
         double2 wj = y[r], wm = y[a.N + r];
         // Code/MLFMM.cpp:339-349
-        wj = cadd(wj, cmul(a.c1, L1J));
-        wj = csub(wj, cmul(a.c1, K1M));
-        wm = cadd(wm, cmul(a.c1, K1J));
-        wm = cadd(wm, cmul(a.c1, L1M));
-        wj = cadd(wj, cmul(cmul(a.c2, a.etai), L2J));
-        wj = csub(wj, cmul(a.c2, K2M));
-        wm = cadd(wm, cmul(a.c2, K2J));
-        wm = cadd(wm, cmul(cmul(a.c2, L2M), a.inv_etai));
+        wj = cadd(wj, cmul(a.c1, widen(L1J)));
+        wj = csub(wj, cmul(a.c1, widen(K1M)));
+        wm = cadd(wm, cmul(a.c1, widen(K1J)));
+        wm = cadd(wm, cmul(a.c1, widen(L1M)));
+        wj = cadd(wj, cmul(cmul(a.c2, a.etai), widen(L2J)));
+        wj = csub(wj, cmul(a.c2, widen(K2M)));
+        wm = cadd(wm, cmul(a.c2, widen(K2J)));
+        wm = cadd(wm, cmul(cmul(a.c2, widen(L2M)), a.inv_etai));
         y[r] = wj;
         y[a.N + r] = wm;
     }
 }
 
-void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, const PmchwtFarArgs& a, double2* y) {
-    k_disagg_pmchwt<<<(a.N + 3) / 4, 256, 0, st>>>(leaf_of, perm, a, y);
+template <typename R>
+void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, const PmchwtFarArgs<R>& a,
+                          double2* y) {
+    k_disagg_pmchwt<R><<<(a.N + 3) / 4, 256, 0, st>>>(leaf_of, perm, a, y);
 }
+
+__global__ void k_narrow(size_t n, const double2* __restrict__ in, float2* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        narrow(in[i], out[i]);
+}
+
+void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    k_narrow<<<(unsigned)blocks, 256, 0, st>>>(n, in, out);
+}
+
+#define MLFMA_FAR_INSTANTIATE(R)                                                                                    \
+    template void launch_aggregate<R>(hipStream_t, int, int, int, int, const int*, const int*, const int*,         \
+                                      const cplx_t<R>*, const double2*, cplx_t<R>*);                                      \
+    template void launch_m2m<R>(hipStream_t, int, int, int, int, const int*, const int*, const int*, const int*,   \
+                                const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);                                     \
+    template void launch_m2l_l2l<R>(hipStream_t, int, int, int, const int*, const int*, const int*, const cplx_t<R>*,  \
+                                    const cplx_t<R>*, int, const int*, const int*, const int*, const int*, const R*,   \
+                                    const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);                                           \
+    template void launch_disagg_pec<R>(hipStream_t, int, int, const int*, const int*, const cplx_t<R>*, const cplx_t<R>*,  \
+                                       double2, double2*);                                                        \
+    template void launch_disagg_pmchwt<R>(hipStream_t, const int*, const int*, const PmchwtFarArgs<R>&, double2*);
+MLFMA_FAR_INSTANTIATE(double)
+MLFMA_FAR_INSTANTIATE(float)
+#undef MLFMA_FAR_INSTANTIATE
 
 // ---------------------------------------------------------------- near field
 // Code/MLFMM.cpp:228-231 (PEC) and :354-366 (PMCHWT).  All near blocks of one
 // target leaf form a single column-major nt x C_t matrix per plane (PMCHWT:
 // z11, zK, z22 with z12 = -zK, z21 = zK, Code/Zmartix.h:611-614), so one
-// workgroup streams one contiguous range of HBM.  The rows are taken in slices
-// of `span` rows; the 256 threads form G = 256 / span column groups and group g
-// takes the columns c = g (mod G), so a wavefront reads runs of `span`
-// consecutive entries and few lanes idle whatever nt is.  The group partials
-// are summed in LDS in a fixed order.  x is gathered into LDS CH columns at a
-// time.
-template <int P, int CH>
-__global__ __launch_bounds__(256) void k_near(NearDev nd, const double2* __restrict__ Z,
-                                              const double2* __restrict__ x, double2* __restrict__ y) {
+// workgroup streams one contiguous range of HBM.  For nt < BS the BS threads
+// form G = BS / nt column groups of nt rows and group g takes the columns
+// c = g (mod G): thread tid reads entry c * nt + tid, so the workgroup reads
+// consecutive addresses; U columns per thread are in flight at once.  The
+// group partials are summed in LDS in a fixed order.  x is gathered into LDS
+// CH columns at a time.  Entries are stored in S (double or float) and
+// accumulated in FP64.
+template <typename S, int P, int BS, int U, int CH>
+__global__ __launch_bounds__(BS) void k_near(NearDev nd, const cplx_t<S>* __restrict__ Z,
+                                             const double2* __restrict__ x, double2* __restrict__ y) {
     constexpr int NX = (P == 3) ? 2 : 1;
     __shared__ double2 xs[NX][CH];
-    __shared__ double2 red[NX][256];
+    __shared__ double2 red[NX][BS];
     const int t = nd.order[blockIdx.x];
     const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
     if (nt == 0) return;
     const int C = nd.tgt_cols[t];
     const int* cref = nd.colref + nd.col_ptr[t];
     const int plane = C * nt;   // < 2^31: one target's block row
-    const double2* Zt = Z + nd.tgt_off[t];
-    // Rows are covered in `passes` slices of `span` rows; pick the slicing
-    // with the fewest column steps per thread, passes * ceil(C / G), among
-    // slices of at least 32 rows (512-byte runs) or the whole leaf.
-    const int pmin = (nt + 255) / 256;
-    int passes = pmin, best = 0x7fffffff;
-    for (int p = pmin; p < pmin + 8; ++p) {
-        const int sp = (nt + p - 1) / p, gp = 256 / sp;
-        if (p > pmin && sp < 32) break;
-        const int cost = p * ((C + gp - 1) / gp);
-        if (cost < best) best = cost, passes = p;
-    }
-    const int span = (nt + passes - 1) / passes;
-    const int G = 256 / span;
+    const cplx_t<S>* Zt = Z + nd.tgt_off[t];
+    const int span = nt < BS ? nt : BS;
+    const int G = BS / span;
     const int g = threadIdx.x / span, rl = threadIdx.x - g * span;
     for (int r0 = 0; r0 < nt; r0 += span) {
         const int row = r0 + rl;
@@ -402,69 +437,71 @@ __global__ __launch_bounds__(256) void k_near(NearDev nd, const double2* __restr
         for (int c0 = 0; c0 < C; c0 += CH) {
             const int cl = min(CH, C - c0);
             __syncthreads();
-            for (int c = threadIdx.x; c < cl; c += 256) {
+            for (int c = threadIdx.x; c < cl; c += BS) {
                 const int r = cref[c0 + c];
                 xs[0][c] = x[r];
                 if (P == 3) xs[NX - 1][c] = x[nd.N + r];
             }
             __syncthreads();
             if (act) {
-                const double2* z = Zt + c0 * nt + row;
+                const cplx_t<S>* z = Zt + c0 * nt + row;
                 const int step = G * nt;
                 int c = g;
                 if (P == 1) {
-                    for (; c + 3 * G < cl; c += 4 * G) {
-                        const int o = c * nt;
-                        const double2 z0 = z[o], z1 = z[o + step], z2 = z[o + 2 * step], z3 = z[o + 3 * step];
-                        cfma(aJ, z0, xs[0][c]);
-                        cfma(aJ, z1, xs[0][c + G]);
-                        cfma(aJ, z2, xs[0][c + 2 * G]);
-                        cfma(aJ, z3, xs[0][c + 3 * G]);
+                    for (; c + (U - 1) * G < cl; c += U * G) {
+                        double2 zz[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) zz[u] = widen(z[c * nt + u * step]);
+#pragma unroll
+                        for (int u = 0; u < U; ++u) cfma(aJ, zz[u], xs[0][c + u * G]);
                     }
-                    for (; c < cl; c += G) cfma(aJ, z[c * nt], xs[0][c]);
+                    for (; c < cl; c += G) cfma(aJ, widen(z[c * nt]), xs[0][c]);
                 } else {
-                    for (; c + G < cl; c += 2 * G) {
-                        const int o = c * nt;
-                        const double2 p0 = z[o], k0 = z[o + plane], m0 = z[o + 2 * plane];
-                        const double2 p1 = z[o + step], k1 = z[o + step + plane], m1 = z[o + step + 2 * plane];
-                        const double2 xJ0 = xs[0][c], xM0 = xs[NX - 1][c];
-                        const double2 xJ1 = xs[0][c + G], xM1 = xs[NX - 1][c + G];
-                        cfma(aJ, p0, xJ0);
-                        cfms(aJ, k0, xM0);
-                        cfma(aM, k0, xJ0);
-                        cfma(aM, m0, xM0);
-                        cfma(aJ, p1, xJ1);
-                        cfms(aJ, k1, xM1);
-                        cfma(aM, k1, xJ1);
-                        cfma(aM, m1, xM1);
+                    for (; c + (U - 1) * G < cl; c += U * G) {
+                        double2 p[U], k[U], m[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int o = c * nt + u * step;
+                            p[u] = widen(z[o]);
+                            k[u] = widen(z[o + plane]);
+                            m[u] = widen(z[o + 2 * plane]);
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const double2 xJ = xs[0][c + u * G], xM = xs[NX - 1][c + u * G];
+                            cfma(aJ, p[u], xJ);
+                            cfms(aJ, k[u], xM);
+                            cfma(aM, k[u], xJ);
+                            cfma(aM, m[u], xM);
+                        }
                     }
-                    if (c < cl) {
+                    for (; c < cl; c += G) {
                         const int o = c * nt;
-                        const double2 p0 = z[o], k0 = z[o + plane], m0 = z[o + 2 * plane];
-                        const double2 xJ0 = xs[0][c], xM0 = xs[NX - 1][c];
-                        cfma(aJ, p0, xJ0);
-                        cfms(aJ, k0, xM0);
-                        cfma(aM, k0, xJ0);
-                        cfma(aM, m0, xM0);
+                        const double2 p0 = widen(z[o]), k0 = widen(z[o + plane]), m0 = widen(z[o + 2 * plane]);
+                        const double2 xJ = xs[0][c], xM = xs[NX - 1][c];
+                        cfma(aJ, p0, xJ);
+                        cfms(aJ, k0, xM);
+                        cfma(aM, k0, xJ);
+                        cfma(aM, m0, xM);
                     }
                 }
             }
         }
-        if (G > 1) {   // fold the column groups of this row slice
+        if (G > 1) {   // nt < BS: a single row pass, fold the column groups
             red[0][threadIdx.x] = aJ;
             if (P == 3) red[NX - 1][threadIdx.x] = aM;
             __syncthreads();
-            if (threadIdx.x < span && r0 + threadIdx.x < nt) {
+            if (threadIdx.x < nt) {
                 double2 sJ = red[0][threadIdx.x], sM = red[NX - 1][threadIdx.x];
                 for (int h = 1; h < G; ++h) {
-                    sJ = cadd(sJ, red[0][h * span + threadIdx.x]);
-                    if (P == 3) sM = cadd(sM, red[NX - 1][h * span + threadIdx.x]);
+                    sJ = cadd(sJ, red[0][h * nt + threadIdx.x]);
+                    if (P == 3) sM = cadd(sM, red[NX - 1][h * nt + threadIdx.x]);
                 }
-                const int r = nd.perm[i0 + r0 + threadIdx.x];
+                const int r = nd.perm[i0 + threadIdx.x];
                 y[r] = sJ;
                 if (P == 3) y[nd.N + r] = sM;
             }
-        } else if (act) {   // G == 1: threads past span hold no row
+        } else if (act) {   // G == 1: threads past nt hold no row
             const int r = nd.perm[i0 + row];
             y[r] = aJ;
             if (P == 3) y[nd.N + r] = aM;
@@ -472,10 +509,13 @@ __global__ __launch_bounds__(256) void k_near(NearDev nd, const double2* __restr
     }
 }
 
-void launch_near(hipStream_t st, const NearDev& nd, const double2* Z, const double2* x, double2* y) {
+template <typename S>
+void launch_near(hipStream_t st, const NearDev& nd, const cplx_t<S>* Z, const double2* x, double2* y) {
     if (nd.nleaf == 0) return;
-    if (nd.planes == 1) k_near<1, 1024><<<nd.nleaf, 256, 0, st>>>(nd, Z, x, y);
-    else k_near<3, 512><<<nd.nleaf, 256, 0, st>>>(nd, Z, x, y);
+    if (nd.planes == 1) k_near<S, 1, 512, 8, 1024><<<nd.nleaf, 512, 0, st>>>(nd, Z, x, y);
+    else k_near<S, 3, 512, 4, 512><<<nd.nleaf, 512, 0, st>>>(nd, Z, x, y);
 }
+template void launch_near<double>(hipStream_t, const NearDev&, const double2*, const double2*, double2*);
+template void launch_near<float>(hipStream_t, const NearDev&, const float2*, const double2*, double2*);
 
 }  // namespace mlfma

@@ -1,4 +1,6 @@
-// Device-side helpers and kernel launch declarations (FP64 complex = double2).
+// Device-side helpers and kernel launch declarations of the matvec.
+// Solver vectors are FP64 complex (double2); the far field (patterns, tables,
+// workspaces) is stored and accumulated in R = double or float.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -6,44 +8,69 @@
 
 namespace mlfma {
 
+template <typename R>
+struct cx_of;
+template <>
+struct cx_of<double> {
+    using type = double2;
+};
+template <>
+struct cx_of<float> {
+    using type = float2;
+};
+template <typename R>
+using cplx_t = typename cx_of<R>::type;
+
 // Launch wrappers; every one enqueues on `st` and returns immediately.
 // Workspace layout of one tree level: [box][cur][comp][K] complex, cur < ncur.
 
 // Sm_leaf[b][cur][c][k] = sum_{i in leaf b} Vs[i][c][k] * x[cur*N + perm[i]]
 // (order: the leaves in launch order, largest first)
+template <typename R>
 void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const int* order, const int* leaf_ptr,
-                      const int* perm, const double2* Vs, const double2* x, double2* Sm);
+                      const int* perm, const cplx_t<R>* Vs, const double2* x, cplx_t<R>* Sm);
 
 // Sm_parent[p][..][q] = sum_sons shift_up[oct][q] * sum_9 w * Sm_child[son][..][tap]
+template <typename R>
 void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int* son_ptr, const int* son_idx,
-                const int* octant_child, const int* tap, const double* w, const double2* shift_up,
-                const double2* Sm_child, double2* Sm_parent);
+                const int* octant_child, const int* tap, const R* w, const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child,
+                cplx_t<R>* Sm_parent);
 
 // Bm[b][..][q] = sum_dr TFw[vec][q] * Sm[dr][..][q]
 //              + (parent present) sum_{(kp,w) in T[q]} w * shift_down[oct][kp] * Bm_parent[par][..][kp]
+template <typename R>
 void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr, const int* dr_idx,
-                    const int* dr_vec, const double2* TFw, const double2* Sm, int Kp, const int* parent,
-                    const int* octant, const int* t_ptr, const int* t_kp, const double* t_w,
-                    const double2* shift_down, const double2* Bm_parent, double2* Bm);
+                    const int* dr_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp, const int* parent,
+                    const int* octant, const int* t_ptr, const int* t_kp, const R* t_w, const cplx_t<R>* shift_down,
+                    const cplx_t<R>* Bm_parent, cplx_t<R>* Bm);
 
 // PEC: y[perm[i]] += c * sum_k Vf[i].B_leaf[k], leaf = leaf_of[i]
-void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const double2* Vf,
-                       const double2* Bm, double2 c, double2* y);
+template <typename R>
+void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const cplx_t<R>* Vf,
+                       const cplx_t<R>* Bm, double2 c, double2* y);
 
 // PMCHWT, both wavenumbers: L and K forms against B_J, B_M (Code/MLFMM.cpp:314-351)
+template <typename R>
 struct PmchwtFarArgs {
     int K1, K2, N;
-    const double2* Vf1;
-    const double2* Vf2;
-    const double2* B1;   // finest Bm for k1: [leaf][2][3][K1]
-    const double2* B2;
-    const double* kp1;   // [3][K1] directions
-    const double* kp2;
+    const cplx_t<R>* Vf1;
+    const cplx_t<R>* Vf2;
+    const cplx_t<R>* B1;   // finest Bm for k1: [leaf][2][3][K1]
+    const cplx_t<R>* B2;
+    const R* kp1;      // [3][K1] directions
+    const R* kp2;
     double2 c1, c2, etai, inv_etai;
 };
-void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, const PmchwtFarArgs& a, double2* y);
+template <typename R>
+void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, const PmchwtFarArgs<R>& a,
+                          double2* y);
 
-// Near-field product y = Z_near x (layout in geo.hpp NearDev).
-void launch_near(hipStream_t st, const NearDev& nd, const double2* Z, const double2* x, double2* y);
+// Near-field product y = Z_near x (layout in geo.hpp NearDev), entries stored
+// in S = double or float, accumulated in FP64.
+template <typename S>
+void launch_near(hipStream_t st, const NearDev& nd, const cplx_t<S>* Z, const double2* x, double2* y);
+
+// FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
+void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
 
 }  // namespace mlfma
