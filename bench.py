@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--batch", type=int, default=32, help="right-hand sides of the batched-matvec measurement")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -117,7 +118,7 @@ def main():
     log(f"mesh: {mesh.n_faces} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     op = api.Operator(mesh, integral_equation=api.CFIE, cfie_alpha=args.alpha, far_precision=api.FP32,
-                      near_precision=api.FP32, verbose=1)
+                      near_precision=api.FP32, max_rhs=args.batch, verbose=1)
     setup_s = time.perf_counter() - t0
     op.k0 = 2 * op.grid(1, op.max_level)[0] ** 2
     log(f"setup {setup_s:.2f} s (FP64 and FP32 tables): N={op.n_rwg} levels={op.max_level + 1} "
@@ -161,6 +162,26 @@ def main():
     ms32 = 1e3 * elapsed32 / args.steps
     err32 = float(torch.linalg.vector_norm(ys[api.FP32] - ys[api.FP64]) / torch.linalg.vector_norm(ys[api.FP64]))
     log(f"FP32 mode (far field and near storage): {ms32:.3f} ms/matvec, rel. difference to FP64 {err32:.2e}")
+
+    batched = None
+    if args.batch > 1:
+        Xb = torch.randn(args.batch, op.n, dtype=torch.complex128, device="cuda", generator=gen)
+        Yb = torch.empty_like(Xb)
+        batched = {"rhs": args.batch}
+        for prec, name in ((api.FP64, "fp64"), (api.FP32, "fp32_mode")):
+            op.set_apply_precision(prec)
+            op.apply_device(Xb.data_ptr(), Yb.data_ptr(), args.batch)
+            torch.cuda.synchronize()
+            reps = max(1, args.steps // 4)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                op.apply_device(Xb.data_ptr(), Yb.data_ptr(), args.batch)
+            torch.cuda.synchronize()
+            ms_b = 1e3 * (time.perf_counter() - t0) / reps
+            batched[name] = {"ms_per_batch": round(ms_b, 3), "rhs_matvec_per_s": round(1e3 * args.batch / ms_b, 2)}
+            log(f"batched {args.batch} RHS {name}: {ms_b:.2f} ms per batch -> "
+                f"{1e3 * args.batch / ms_b:.1f} RHS-matvecs/s")
+        del Xb, Yb
 
     groups = prof["groups"]
     per_launch = {g: v["ms"] / max(v["launches"], 1) for g, v in groups.items()}
@@ -208,6 +229,7 @@ def main():
         "roofline": roofline,
         "solve": solve,
         "fp32_mode": fp32,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -139,6 +139,16 @@ typedef struct mlfma_solve_stats {
 int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const double* rhs, double* x,
                 mlfma_solve_stats* stats);
 
+/* nrhs (<= 256) independent GMRES solves in lockstep, sharing one batched
+ * matvec per Arnoldi step (each pass through the operator's tables serves up
+ * to 8 right-hand sides).  rhs / x are column-major (vector j at 2*n*j
+ * doubles); stats has nrhs entries; returns MLFMA_ERR_NOT_CONVERGED if any
+ * right-hand side did not converge.  Replaces the sequential per-incidence
+ * MLFMM::matrix_solver calls of the monostatic sweeps (Code/RCS.h:272-325,
+ * 428-497) and of batch mode (Code/BatchRunner.h:422-484); precond must be 0. */
+int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int nrhs, const double* rhs, double* x,
+                      mlfma_solve_stats* stats);
+
 /* Plane-wave excitation for the handle's equation; replaces RHS::computeV_E,
  * computeV_C and computeV_PMCHWT (Code/Vm.h:10-203) with EMSource::initPW
  * (Code/EMSource.cpp:34-72).  pol 0 = "V" (E along -theta_hat), 90 = "H". */

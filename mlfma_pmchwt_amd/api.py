@@ -67,6 +67,7 @@ def library() -> C.CDLL:
         lib.mlfma_apply_device.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_set_apply_precision.argtypes = [vp, C.c_int]
         lib.mlfma_solve.argtypes = [vp, C.POINTER(SolveParams), C.c_int, vp, vp, C.POINTER(SolveStats)]
+        lib.mlfma_solve_batch.argtypes = [vp, C.POINTER(SolveParams), C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_excitation.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         lib.mlfma_rcs.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
         lib.mlfma_stage.argtypes = [vp, C.c_int, vp, vp]
@@ -168,6 +169,24 @@ class Operator:
             self._check(rc)
         return x, {"iterations": st.iterations, "matvecs": st.matvecs, "converged": bool(st.converged),
                    "rel_residual": st.rel_residual, "seconds": st.seconds, "fp64_matvecs": st.fp64_matvecs}
+
+    def solve_batch(self, rhs: np.ndarray, x0: np.ndarray | None = None, itr_max=None, restart=30, tol_rel=1e-3,
+                    tol_abs=0.0):
+        """GMRES for the columns of rhs (n x R) in lockstep with batched matvecs."""
+        rhs = np.asfortranarray(rhs, dtype=np.complex128)
+        if rhs.ndim != 2 or rhs.shape[0] != self.n:
+            raise ValueError("rhs must be n x R")
+        R = rhs.shape[1]
+        x = np.zeros_like(rhs, order="F") if x0 is None else np.asfortranarray(x0, dtype=np.complex128).copy(order="F")
+        sp = SolveParams(GMRES, itr_max if itr_max is not None else 4 * self.n, restart, tol_rel, tol_abs, 0)
+        st = (SolveStats * R)()
+        rc = library().mlfma_solve_batch(self._h, C.byref(sp), self.n, R, rhs.ctypes.data_as(C.c_void_p),
+                                         x.ctypes.data_as(C.c_void_p), st)
+        if rc not in (0, 5):
+            self._check(rc)
+        return x, [{"iterations": s.iterations, "matvecs": s.matvecs, "converged": bool(s.converged),
+                    "rel_residual": s.rel_residual, "seconds": s.seconds, "fp64_matvecs": s.fp64_matvecs}
+                   for s in st]
 
     def excitation(self, pol=0.0, theta=0.0, phi=0.0) -> np.ndarray:
         v = np.empty(self.n, dtype=np.complex128)

@@ -2,6 +2,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -195,6 +196,32 @@ int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const doub
         MLFMA_HIP(hipMemcpy(x, h->io.y, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost));
         if (stats) *stats = s;
         conv = s.converged;
+    });
+    if (rc != MLFMA_OK) return rc;
+    return conv ? MLFMA_OK : MLFMA_ERR_NOT_CONVERGED;
+}
+
+int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int nrhs, const double* rhs, double* x,
+                      mlfma_solve_stats* stats) {
+    if (!h) return MLFMA_ERR_ARG;
+    int conv = 1;
+    int rc = guarded(h, [&] {
+        mlfma::Engine& e = *h->eng;
+        if (!sp || n != e.nunk || nrhs < 1 || nrhs > 256 || !rhs || !x)
+            throw mlfma::Error(MLFMA_ERR_ARG, "invalid batched solve arguments");
+        if (sp->solver != MLFMA_SOLVER_GMRES) throw mlfma::Error(MLFMA_ERR_ARG, "batched solve supports GMRES only");
+        if (sp->itr_max < 0 || sp->restart < 1 || sp->tol_rel < 0) throw mlfma::Error(MLFMA_ERR_ARG, "bad solver parameters");
+        const size_t cnt = (size_t)n * nrhs;
+        h->io.ensure(cnt);
+        MLFMA_HIP(hipMemcpy(h->io.x, rhs, cnt * sizeof(double2), hipMemcpyHostToDevice));
+        MLFMA_HIP(hipMemcpy(h->io.y, x, cnt * sizeof(double2), hipMemcpyHostToDevice));
+        std::vector<mlfma_solve_stats> s(nrhs);
+        e.solve_device(*sp, h->io.x, h->io.y, nrhs, s.data());
+        MLFMA_HIP(hipMemcpy(x, h->io.y, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+        for (int r = 0; r < nrhs; ++r) {
+            if (stats) stats[r] = s[r];
+            conv = conv && s[r].converged;
+        }
     });
     if (rc != MLFMA_OK) return rc;
     return conv ? MLFMA_OK : MLFMA_ERR_NOT_CONVERGED;

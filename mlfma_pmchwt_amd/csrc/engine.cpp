@@ -263,7 +263,7 @@ void Engine::build_far(int w, const WaveTables& T) {
     }
     MLFMA_HIP(hipGetLastError());
     for (int l = 2; l <= L; ++l) {
-        const size_t ws = (size_t)tree.lv[l].n * ncur * 3 * D.K[l];
+        const size_t ws = (size_t)tree.lv[l].n * ncur * rb_max<R>() * 3 * D.K[l];
         F.Sm[l] = dalloc<cplx_t<R>>(ws);
         F.Bm[l] = dalloc<cplx_t<R>>(ws);
         // quadrature weight w_theta * dphi folded into the translation operator
@@ -369,39 +369,53 @@ std::vector<double2> Engine::near_block(int p) {
     return out;
 }
 
-// mexp, m2m (fine -> coarse), then m2l with the gather-form l2l (coarse -> fine)
+// Right-hand sides per pass of the batched matvec: nc = rb * ncur workspace
+// currents, at most 4 in FP64 and 8 in FP32 (register budget of the
+// translation and disaggregation kernels), and at most max_rhs.
 template <typename R>
-void Engine::far_pass(int w, const double2* x) {
+int Engine::rb_max() const {
+    const int cap = (std::is_same<R, double>::value ? 4 : 8) / ncur;
+    int rb = 1;
+    while (rb * 2 <= cap && rb * 2 <= P.max_rhs) rb *= 2;
+    return rb;
+}
+
+// mexp, m2m (fine -> coarse), then m2l with the gather-form l2l (coarse -> fine)
+// for right-hand sides r0 .. r0 + rb - 1 of RHS-fastest vectors with stride ldr.
+template <typename R>
+void Engine::far_pass(int w, const double2* x, int ldr, int r0, int rb) {
     WaveDev& D = wd_[w];
     FarDev<R>& F = far<R>(w);
     const int L = tree.max_level;
+    const int nc = ncur * rb;
     mark(6 * w + 0);
-    launch_aggregate<R>(st, ncur, nleaf, D.K[L], N, d_leaf_order_, d_leaf_ptr_, d_perm_, F.Vs, x, F.Sm[L]);
+    launch_aggregate_b<R>(st, ncur, rb, nleaf, D.K[L], N, ldr, r0, d_leaf_order_, d_leaf_ptr_, d_perm_, F.Vs, x,
+                          F.Sm[L]);
     mark(6 * w + 1);
     mark(6 * w + 2);
     for (int l = L - 1; l >= 2; --l)
-        launch_m2m<R>(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
+        launch_m2m<R>(st, nc, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
                       D.tap[l + 1], F.tw[l + 1], F.sh_up[l + 1], F.Sm[l + 1], F.Sm[l]);
     mark(6 * w + 3);
     mark(6 * w + 4);
     for (int l = 2; l <= L; ++l) {
         const bool up = l > 2;
-        launch_m2l_l2l<R>(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, F.TFw[l],
+        launch_m2l_l2l<R>(st, nc, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, F.TFw[l],
                           F.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l], F.t_w[l],
                           F.sh_dn[l], up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
     }
     mark(6 * w + 5);
 }
 
-// y += far field (finest Bm of every wavenumber)
+// y += far field (finest Bm of every wavenumber), same right-hand-side range
 template <typename R>
-void Engine::disaggregate(double2* y) {
+void Engine::disaggregate(double2* y, int ldr, int r0, int rb) {
     const int L = tree.max_level;
     if (ie != MLFMA_PMCHWT) {
         // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
         const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
         const FarDev<R>& F = far<R>(0);
-        launch_disagg_pec<R>(st, N, wd_[0].K[L], d_leaf_of_, d_perm_, F.Vf, F.Bm[L], d2(c), y);
+        launch_disagg_pec_b<R>(st, rb, N, wd_[0].K[L], ldr, r0, d_leaf_of_, d_perm_, F.Vf, F.Bm[L], d2(c), y);
     } else {
         const FarDev<R>& F1 = far<R>(0);
         const FarDev<R>& F2 = far<R>(1);
@@ -419,7 +433,7 @@ void Engine::disaggregate(double2* y) {
         a.c2 = d2(med.k2 * med.k2 / (16.0 * kPi * kPi));
         a.etai = d2(med.etai);
         a.inv_etai = d2(1.0 / med.etai);
-        launch_disagg_pmchwt<R>(st, d_leaf_of_, d_perm_, a, y);
+        launch_disagg_pmchwt_b<R>(st, rb, ldr, r0, d_leaf_of_, d_perm_, a, y);
     }
 }
 
@@ -435,7 +449,7 @@ void Engine::set_profiling(bool on) {
     prof_apply = 0;
 }
 
-void Engine::collect_profile() {
+void Engine::collect_profile(int count) {
     MLFMA_HIP(hipEventSynchronize(ev_[15]));
     auto span = [&](int a) {
         float ms = 0;
@@ -455,31 +469,82 @@ void Engine::collect_profile() {
     prof_ms[4] += span(14);
     prof_launch[3] += 1;
     prof_launch[4] += 1;
-    prof_apply += 1;
+    prof_apply += count;
 }
 
-void Engine::apply_one(const double2* x, double2* y, int prec) {
+// One pass of the matvec for right-hand sides r0 .. r0 + rb - 1 of RHS-fastest
+// vectors with stride ldr (single vector: ldr = 1, r0 = 0, rb = 1).
+void Engine::apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec) {
     const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
     for (int w = 0; w < nw; ++w) {
-        if (f32) far_pass<float>(w, x);
-        else far_pass<double>(w, x);
+        if (f32) far_pass<float>(w, x, ldr, r0, rb);
+        else far_pass<double>(w, x, ldr, r0, rb);
     }
     mark(12);
-    if (prec == MLFMA_FP32 && d_Z32_) launch_near<float>(st, nd_, d_Z32_, x, y);
-    else launch_near<double>(st, nd_, d_Z_, x, y);
+    if (prec == MLFMA_FP32 && d_Z32_) launch_near_b<float>(st, rb, ldr, r0, nd_, d_Z32_, x, y);
+    else launch_near_b<double>(st, rb, ldr, r0, nd_, d_Z_, x, y);
     mark(13);
     mark(14);
-    if (f32) disaggregate<float>(y);
-    else disaggregate<double>(y);
+    if (f32) disaggregate<float>(y, ldr, r0, rb);
+    else disaggregate<double>(y, ldr, r0, rb);
     mark(15);
 }
 
+void Engine::apply_one(const double2* x, double2* y, int prec) { apply_pass(x, y, 1, 0, 1, prec); }
+
+// Y = A X for R right-hand sides, RHS-fastest (element (u, r) at u * R + r).
+void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
+    const int cap = (prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32) ? rb_max<float>() : rb_max<double>();
+    for (int r0 = 0; r0 < R;) {
+        int rb = 1;
+        while (rb * 2 <= cap && r0 + rb * 2 <= R) rb *= 2;
+        apply_pass(X, Y, R, r0, rb, prec);
+        if (prof_on_) collect_profile(rb);
+        r0 += rb;
+    }
+}
+
+// Column-major device vectors (vector j at x + j * nunk), as the C ABI passes them.
 void Engine::apply_device(const double2* x, double2* y, int nrhs) {
-    for (int r = 0; r < nrhs; ++r) {
-        apply_one(x + (size_t)r * nunk, y + (size_t)r * nunk, apply_prec);
-        if (prof_on_) collect_profile();
+    if (nrhs == 1) {
+        apply_one(x, y, apply_prec);
+        if (prof_on_) collect_profile(1);
+    } else {
+        ensure_block(nrhs);
+        launch_transpose(st, nrhs, nunk, x, d_bx_);
+        apply_block(d_bx_, d_by_, nrhs, apply_prec);
+        launch_transpose(st, nunk, nrhs, d_by_, y);
     }
     MLFMA_HIP(hipGetLastError());
+}
+
+void Engine::ensure_block(int R) {
+    if (R > 256) throw Error(MLFMA_ERR_ARG, "at most 256 right-hand sides per call");
+    if (R <= block_cap_) return;
+    dfree(d_bx_);
+    dfree(d_by_);
+    d_bx_ = dalloc<double2>((size_t)nunk * R);
+    d_by_ = dalloc<double2>((size_t)nunk * R);
+    block_cap_ = R;
+}
+
+void Engine::dfree(void* p) {
+    if (!p) return;
+    auto it = std::find(allocs_.begin(), allocs_.end(), p);
+    if (it != allocs_.end()) allocs_.erase(it);
+    MLFMA_HIP(hipStreamSynchronize(st));
+    MLFMA_HIP(hipFree(p));
+}
+
+// Column-major device blocks (vector j at b + j * nunk), as the C ABI passes them.
+void Engine::solve_device(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, int R,
+                          mlfma_solve_stats* stats) {
+    ensure_block(R);
+    launch_transpose(st, R, nunk, d_b, d_bx_);
+    launch_transpose(st, R, nunk, d_x, d_by_);
+    solve_block(sp, d_bx_, d_by_, R, stats);
+    launch_transpose(st, nunk, R, d_by_, d_x);
+    MLFMA_HIP(hipStreamSynchronize(st));
 }
 
 void Engine::set_apply_precision(int prec) {
@@ -523,7 +588,7 @@ void Engine::stage(int which, const double2* dx, double2* dy) {
             break;
         case MLFMA_STAGE_FAR:
             MLFMA_HIP(hipMemsetAsync(dy, 0, sizeof(double2) * nunk, st));
-            disaggregate<double>(dy);
+            disaggregate<double>(dy, 1, 0, 1);
             break;
         case MLFMA_STAGE_NEAR:
             launch_near<double>(st, nd_, d_Z_, dx, dy);
@@ -739,6 +804,199 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
     stats.converged = converged ? 1 : 0;
     stats.rel_residual = err;
     stats.seconds = wall() - t0;
+}
+
+// Restarted GMRES(m) for R right-hand sides in lockstep: one batched matvec
+// per Arnoldi step serves every right-hand side, while the Gram-Schmidt
+// coefficients, Givens rotations and convergence tests stay per right-hand
+// side (the same recurrences as Engine::solve).  Block vectors are
+// RHS-fastest.  A right-hand side that has converged inside a cycle keeps its
+// Krylov column count fixed; its later columns are not used.  Krylov steps run
+// with the apply precision, every restart starts from the true FP64 residual.
+void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R,
+                         mlfma_solve_stats* stats) {
+    const double t0 = wall();
+    const int n = nunk;
+    const int m = std::max(1, sp.restart);
+    const double eps = 1e-16;
+    if (sp.solver != MLFMA_SOLVER_GMRES) throw Error(MLFMA_ERR_ARG, "batched solve supports GMRES only");
+    if (sp.precond != 0) throw Error(MLFMA_ERR_ARG, "batched solve: preconditioner not supported");
+    if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "batched solve: 1..256 right-hand sides");
+    const size_t nR = (size_t)n * R;
+    if (bV_cols_ < m + 1 || bV_R_ != R) {
+        for (void* p : {(void*)d_bV_, (void*)d_bW_, (void*)d_bRr_, (void*)d_bpart_, (void*)d_bsmall_, (void*)d_bscale_})
+            dfree(p);
+        bV_cols_ = m + 1;
+        bV_R_ = R;
+        d_bV_ = dalloc<double2>(nR * (m + 1));
+        d_bW_ = dalloc<double2>(nR);
+        d_bRr_ = dalloc<double2>(nR);
+        d_bpart_ = dalloc<double2>((size_t)(m + 1) * kRedBlocks * R);
+        d_bsmall_ = dalloc<double2>((size_t)(m + 1) * R);
+        d_bscale_ = dalloc<double>(R);
+    }
+    int matvecs = 0, fp64_matvecs = 0;
+    std::vector<int> iters(R, 0);
+    auto norms = [&](const double2* v, std::vector<double>& out) {
+        launch_norm2_partial_b(st, n, R, v, d_bpart_, kRedBlocks);
+        launch_reduce_partials_b(st, 1, R, kRedBlocks, d_bpart_, d_bsmall_);
+        std::vector<double2> h(R);
+        MLFMA_HIP(hipMemcpyAsync(h.data(), d_bsmall_, R * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        out.resize(R);
+        for (int r = 0; r < R; ++r) out[r] = std::sqrt(h[r].x);
+    };
+    auto dots = [&](int cnt, std::vector<cplx>& h) {   // h[j * R + r] = <V_j, W>_r
+        launch_multidot_b(st, n, R, cnt, d_bV_, nR, d_bW_, d_bpart_, kRedBlocks);
+        launch_reduce_partials_b(st, cnt, R, kRedBlocks, d_bpart_, d_bsmall_);
+        std::vector<double2> tmp((size_t)cnt * R);
+        MLFMA_HIP(hipMemcpyAsync(tmp.data(), d_bsmall_, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        h.resize(tmp.size());
+        for (size_t e = 0; e < tmp.size(); ++e) h[e] = cx(tmp[e]);
+    };
+    auto put = [&](const std::vector<cplx>& v) {
+        std::vector<double2> tmp(v.size());
+        for (size_t e = 0; e < v.size(); ++e) tmp[e] = d2(v[e]);
+        MLFMA_HIP(hipMemcpyAsync(d_bsmall_, tmp.data(), tmp.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+    };
+    auto put_scale = [&](const std::vector<double>& a) {
+        MLFMA_HIP(hipMemcpyAsync(d_bscale_, a.data(), R * sizeof(double), hipMemcpyHostToDevice, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+    };
+    auto true_residual = [&](std::vector<double>& rn) {
+        apply_block(d_X, d_bRr_, R, MLFMA_FP64);
+        matvecs += R;
+        fp64_matvecs += R;
+        launch_residual(st, (int)std::min<size_t>(nR, 0x7fffffff), d_B, d_bRr_);
+        norms(d_bRr_, rn);
+    };
+    if (nR > 0x7fffffff) throw Error(MLFMA_ERR_ARG, "batched solve: n * R exceeds 2^31");
+
+    std::vector<double> bnorm, rnorm, err(R);
+    norms(d_B, bnorm);
+    for (double& b : bnorm)
+        if (b < eps) b = 1.0;
+    true_residual(rnorm);
+    std::vector<char> done(R);
+    for (int r = 0; r < R; ++r) {
+        err[r] = rnorm[r] / bnorm[r];
+        if (!std::isfinite(err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite initial residual");
+        done[r] = err[r] < sp.tol_rel || err[r] < sp.tol_abs;
+    }
+    auto all_done = [&] { return std::all_of(done.begin(), done.end(), [](char c) { return c != 0; }); };
+    std::vector<cplx> H((size_t)R * (m + 1) * m), cs((size_t)R * m), sn((size_t)R * m), sv((size_t)R * (m + 1));
+    auto Hm = [&](int r, int i, int j) -> cplx& { return H[((size_t)r * m + j) * (m + 1) + i]; };
+    std::vector<cplx> h1, h2, yv;
+    std::vector<int> kused(R);
+    int total = 0;
+    for (int cyc = 1; !all_done() && total < sp.itr_max; ++cyc) {
+        std::vector<double> sc(R);
+        for (int r = 0; r < R; ++r) sc[r] = (done[r] || rnorm[r] == 0.0) ? 0.0 : 1.0 / rnorm[r];
+        put_scale(sc);
+        launch_scale_copy_b(st, n, R, d_bscale_, d_bRr_, d_bV_);
+        std::fill(H.begin(), H.end(), cplx(0, 0));
+        std::fill(sv.begin(), sv.end(), cplx(0, 0));
+        std::vector<char> live(R);
+        for (int r = 0; r < R; ++r) {
+            sv[(size_t)r * (m + 1)] = rnorm[r];
+            live[r] = !done[r];
+            kused[r] = 0;
+        }
+        for (int k = 0; k < m && total < sp.itr_max; ++k) {
+            if (std::none_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
+            ++total;
+            apply_block(d_bV_ + (size_t)k * nR, d_bW_, R, apply_prec);
+            matvecs += R;
+            dots(k + 1, h1);
+            put(h1);
+            launch_sub_combination_b(st, n, R, k + 1, d_bV_, nR, d_bsmall_, d_bW_);
+            dots(k + 1, h2);
+            put(h2);
+            launch_sub_combination_b(st, n, R, k + 1, d_bV_, nR, d_bsmall_, d_bW_);
+            std::vector<double> hn;
+            norms(d_bW_, hn);
+            std::vector<double> inv(R, 0.0);
+            for (int r = 0; r < R; ++r) {
+                if (!live[r]) continue;
+                kused[r] = k + 1;
+                ++iters[r];
+                for (int j = 0; j <= k; ++j) Hm(r, j, k) = h1[(size_t)j * R + r] + h2[(size_t)j * R + r];
+                if (!std::isfinite(hn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite Arnoldi vector");
+                Hm(r, k + 1, k) = hn[r];
+                const bool breakdown = hn[r] < eps * rnorm[r];
+                if (!breakdown) inv[r] = 1.0 / hn[r];
+                cplx* c_ = &cs[(size_t)r * m];
+                cplx* s_ = &sn[(size_t)r * m];
+                cplx* g = &sv[(size_t)r * (m + 1)];
+                // Code/GMRES.h:22-48, 144-152
+                for (int j = 0; j < k; ++j) {
+                    cplx tmp = std::conj(c_[j]) * Hm(r, j, k) + std::conj(s_[j]) * Hm(r, j + 1, k);
+                    Hm(r, j + 1, k) = -s_[j] * Hm(r, j, k) + c_[j] * Hm(r, j + 1, k);
+                    Hm(r, j, k) = tmp;
+                }
+                cplx a = Hm(r, k, k), b = Hm(r, k + 1, k);
+                if (std::abs(b) < eps) {
+                    c_[k] = 1.0, s_[k] = 0.0;
+                } else if (std::abs(a) < eps) {
+                    c_[k] = 0.0, s_[k] = 1.0;
+                } else {
+                    double rr = std::sqrt(std::real(a * std::conj(a) + b * std::conj(b)));
+                    c_[k] = a / rr, s_[k] = b / rr;
+                }
+                Hm(r, k, k) = std::conj(c_[k]) * a + std::conj(s_[k]) * b;
+                Hm(r, k + 1, k) = 0.0;
+                cplx tmp = std::conj(c_[k]) * g[k] + std::conj(s_[k]) * g[k + 1];
+                g[k + 1] = -s_[k] * g[k] + c_[k] * g[k + 1];
+                g[k] = tmp;
+                err[r] = std::abs(g[k + 1]) / bnorm[r];
+                if (breakdown || err[r] < sp.tol_rel || err[r] < sp.tol_abs) live[r] = 0;
+            }
+            if (k + 1 < m + 1) {
+                put_scale(inv);
+                launch_scale_copy_b(st, n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * nR);
+            }
+        }
+        // x_r += V y_r with y_r from the triangular Hessenberg of each right-hand side
+        int kmax = 0;
+        for (int r = 0; r < R; ++r) kmax = std::max(kmax, kused[r]);
+        yv.assign((size_t)std::max(kmax, 1) * R, cplx(0, 0));
+        for (int r = 0; r < R; ++r) {
+            const int mm = kused[r];
+            const cplx* g = &sv[(size_t)r * (m + 1)];
+            std::vector<cplx> y(mm);
+            for (int i = mm - 1; i >= 0; --i) {
+                y[i] = g[i];
+                for (int j = i + 1; j < mm; ++j) y[i] -= Hm(r, i, j) * y[j];
+                y[i] = (std::abs(Hm(r, i, i)) > eps) ? y[i] / Hm(r, i, i) : cplx(0, 0);
+            }
+            for (int i = 0; i < mm; ++i) yv[(size_t)i * R + r] = y[i];
+        }
+        if (kmax > 0) {
+            put(yv);
+            launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_X);
+        }
+        true_residual(rnorm);
+        for (int r = 0; r < R; ++r) {
+            err[r] = rnorm[r] / bnorm[r];
+            if (!std::isfinite(err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
+            if (err[r] < sp.tol_rel || err[r] < sp.tol_abs) done[r] = 1;
+        }
+        bool stalled = true;   // Code/GMRES.h:208-214, all right-hand sides
+        for (int r = 0; r < R; ++r)
+            if (!done[r] && !(err[r] > 1000.0 * sp.tol_rel)) stalled = false;
+        if (stalled && cyc > 8) break;
+    }
+    const double secs = wall() - t0;
+    for (int r = 0; r < R; ++r) {
+        stats[r].iterations = iters[r];
+        stats[r].matvecs = matvecs / R;
+        stats[r].fp64_matvecs = fp64_matvecs / R;
+        stats[r].converged = done[r] ? 1 : 0;
+        stats[r].rel_residual = err[r];
+        stats[r].seconds = secs;
+    }
 }
 
 // Near-field block-diagonal preconditioner: exact inverse of every leaf's self

@@ -71,6 +71,12 @@ public:
     int apply_prec = MLFMA_FP64;
     void stage(int which, const double2* dx, double2* dy);
     void solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, mlfma_solve_stats& st);
+    // R right-hand sides at once, RHS-fastest device blocks; stats[R]
+    void solve_block(const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R, mlfma_solve_stats* stats);
+    // same with column-major blocks (vector j at b + j * n_unknowns)
+    void solve_device(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, int R,
+                      mlfma_solve_stats* stats);
+    void ensure_block(int R);
     void excitation(double pol, double th, double ph, double2* d_v);
     void rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out);
 
@@ -110,6 +116,7 @@ private:
     T* dalloc(size_t count);
     template <typename T>
     T* upload(const std::vector<T>& v);
+    void dfree(void* p);
     void build_device_tree();
     void build_wave(int w);
     void build_near();
@@ -118,12 +125,16 @@ private:
     template <typename R>
     void build_far(int w, const WaveTables& T);
     template <typename R>
-    void far_pass(int w, const double2* x);
+    int rb_max() const;
     template <typename R>
-    void disaggregate(double2* y);
+    void far_pass(int w, const double2* x, int ldr, int r0, int rb);
+    template <typename R>
+    void disaggregate(double2* y, int ldr, int r0, int rb);
+    void apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec);
     void mark(int slot);
-    void collect_profile();
+    void collect_profile(int count);
     void apply_one(const double2* x, double2* y, int prec);
+    void apply_block(const double2* X, double2* Y, int R, int prec);
     std::vector<double2> near_block(int p);   // planes x (ns x nt column-major) of pair p
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
@@ -142,6 +153,10 @@ private:
     NearDev nd_{};
     double2* d_Z_ = nullptr;
     float2* d_Z32_ = nullptr;   // FP32 copy (near_precision = MLFMA_FP32)
+    // RHS-fastest block vectors of the batched matvec
+    double2* d_bx_ = nullptr;
+    double2* d_by_ = nullptr;
+    int block_cap_ = 0;
     // solver scratch
     double2* d_V_ = nullptr;
     int V_cols_ = 0;
@@ -151,6 +166,14 @@ private:
     double2* d_part_ = nullptr;
     double2* d_small_ = nullptr;
     double* d_rpart_ = nullptr;
+    // batched solver scratch
+    double2* d_bV_ = nullptr;
+    double2* d_bW_ = nullptr;
+    double2* d_bRr_ = nullptr;
+    double2* d_bpart_ = nullptr;
+    double2* d_bsmall_ = nullptr;
+    double* d_bscale_ = nullptr;
+    int bV_cols_ = 0, bV_R_ = 0;
     // profiling events: per wave 3 groups x 2, near 2, far 2
     bool prof_on_ = false;
     hipEvent_t ev_[16] = {};

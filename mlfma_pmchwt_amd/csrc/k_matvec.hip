@@ -27,17 +27,19 @@ __device__ __forceinline__ int xcd_remap(int lin, int total) {
 // One workgroup per leaf, threads over the K samples (blockDim = K rounded up
 // to a wavefront, at most 512), so that a workgroup streams whole pattern rows
 // [i][3][K] of its leaf from HBM.  Leaves are taken largest first.  The leaf's
-// x values are gathered into LDS in chunks of kAggChunk RWGs, which keeps the
-// dependent perm -> x loads out of the pattern stream.
-constexpr int kAggChunk = 128;
-
-template <typename R, int NC>
-__global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __restrict__ order,
+// x values are gathered into LDS in chunks, which keeps the dependent
+// perm -> x loads out of the pattern stream.  RB right-hand sides share each
+// pattern load: solver vectors are RHS-fastest (unknown u, rhs r at
+// u * ldr + r) and workspace current c = rr * NCUR + cur.
+template <typename R, int NCUR, int RB>
+__global__ __launch_bounds__(512) void k_aggregate(int K, int N, int ldr, int r0, const int* __restrict__ order,
                                                    const int* __restrict__ leaf_ptr, const int* __restrict__ perm,
                                                    const cplx_t<R>* __restrict__ Vs, const double2* __restrict__ x,
                                                    cplx_t<R>* __restrict__ Sm) {
     using C = cplx_t<R>;
-    __shared__ C xl[NC][kAggChunk];
+    constexpr int NC = NCUR * RB;
+    constexpr int CHUNK = NC <= 2 ? 128 : 64;
+    __shared__ C xl[NC][CHUNK];
     const int leaf = order[blockIdx.x];
     const int i0 = leaf_ptr[leaf], i1 = leaf_ptr[leaf + 1];
     for (int q0 = 0; q0 < K; q0 += blockDim.x) {
@@ -48,13 +50,13 @@ __global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __re
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int d = 0; d < 3; ++d) acc[c][d] = C{0, 0};
-        for (int b = i0; b < i1; b += kAggChunk) {
-            const int n = min(kAggChunk, i1 - b);
+        for (int b = i0; b < i1; b += CHUNK) {
+            const int n = min(CHUNK, i1 - b);
             __syncthreads();
-            for (int j = threadIdx.x; j < n; j += blockDim.x) {
-                const int r = perm[b + j];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) narrow(x[(size_t)c * N + r], xl[c][j]);
+            for (int e = threadIdx.x; e < n * NC; e += blockDim.x) {
+                const int j = e / NC, c = e - j * NC;
+                const int rr = c / NCUR, cur = c - rr * NCUR;
+                narrow(x[((size_t)cur * N + perm[b + j]) * ldr + r0 + rr], xl[c][j]);
             }
             __syncthreads();
             if (act) {
@@ -99,12 +101,29 @@ __global__ __launch_bounds__(512) void k_aggregate(int K, int N, const int* __re
 }
 
 template <typename R>
-void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const int* order, const int* leaf_ptr,
-                      const int* perm, const cplx_t<R>* Vs, const double2* x, cplx_t<R>* Sm) {
+void launch_aggregate_b(hipStream_t st, int ncur, int rb, int nleaf, int K, int N, int ldr, int r0, const int* order,
+                        const int* leaf_ptr, const int* perm, const cplx_t<R>* Vs, const double2* x,
+                        cplx_t<R>* Sm) {
     if (nleaf == 0) return;
     const int threads = std::min(512, (K + 63) / 64 * 64);
-    if (ncur == 1) k_aggregate<R, 1><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
-    else k_aggregate<R, 2><<<nleaf, threads, 0, st>>>(K, N, order, leaf_ptr, perm, Vs, x, Sm);
+#define AGG(NCUR, RB) k_aggregate<R, NCUR, RB><<<nleaf, threads, 0, st>>>(K, N, ldr, r0, order, leaf_ptr, perm, Vs, x, Sm)
+    if (ncur == 1) {
+        if (rb == 1) AGG(1, 1);
+        else if (rb == 2) AGG(1, 2);
+        else if (rb == 4) AGG(1, 4);
+        else AGG(1, 8);
+    } else {
+        if (rb == 1) AGG(2, 1);
+        else if (rb == 2) AGG(2, 2);
+        else AGG(2, 4);
+    }
+#undef AGG
+}
+
+template <typename R>
+void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const int* order, const int* leaf_ptr,
+                      const int* perm, const cplx_t<R>* Vs, const double2* x, cplx_t<R>* Sm) {
+    launch_aggregate_b<R>(st, ncur, 1, nleaf, K, N, 1, 0, order, leaf_ptr, perm, Vs, x, Sm);
 }
 
 // ---------------------------------------------------------------- interpolation
@@ -163,12 +182,12 @@ void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int
                 const int* octant_child, const int* tap, const R* w, const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child,
                 cplx_t<R>* Sm_parent) {
     dim3 grid((Kp + 255) / 256, nparent);
-    if (ncur == 1)
-        k_m2m<R, 1><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child,
-                                          Sm_parent);
-    else
-        k_m2m<R, 2><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child,
-                                          Sm_parent);
+#define M2M(NC) k_m2m<R, NC><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child, Sm_parent)
+    if (ncur == 1) M2M(1);
+    else if (ncur == 2) M2M(2);
+    else if (ncur == 4) M2M(4);
+    else M2M(8);
+#undef M2M
 }
 
 // ---------------------------------------------------------------- translation + anterpolation
@@ -231,150 +250,199 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
                     const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
     const int kt = (K + 255) / 256;
     dim3 grid(kt * nbox);
-    if (ncur == 1)
-        k_m2l_l2l<R, 1><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant,
-                                              t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm);
-    else
-        k_m2l_l2l<R, 2><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant,
-                                              t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm);
+#define M2L(NC) k_m2l_l2l<R, NC><<<grid, 256, 0, st>>>(nbox, K, kt, dr_ptr, dr_idx, dr_vec, TFw, Sm, Kp, parent, octant, t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm)
+    if (ncur == 1) M2L(1);
+    else if (ncur == 2) M2L(2);
+    else if (ncur == 4) M2L(4);
+    else M2L(8);
+#undef M2L
 }
 
 // ---------------------------------------------------------------- disaggregation
 // Code/MLFMM.cpp:211-227: w_m += (k eta/4pi)(k/4pi) sum_k Vfmj_m . Bm_leaf.
-// Each wavefront owns kDisRows consecutive RWG rows (internal order) and
-// streams their patterns together, lanes over k, then one wavefront reduction
-// per row.  Rows run over the whole RWG range, so small and large leaves load
-// the GPU alike.
-constexpr int kDisRows = 4;
-
-template <typename R>
-__global__ __launch_bounds__(256) void k_disagg_pec(int N, int K, const int* __restrict__ leaf_of,
+// Each wavefront owns ROWS consecutive RWG rows (internal order) and streams
+// their patterns together, lanes over k, then one wavefront reduction per row
+// and right-hand side.  Rows run over the whole RWG range, so small and large
+// leaves load the GPU alike.  With RB right-hand sides each pattern load
+// serves RB workspaces (current rr of the leaf).
+template <typename R, int ROWS, int RB>
+__global__ __launch_bounds__(256) void k_disagg_pec(int N, int K, int ldr, int r0, const int* __restrict__ leaf_of,
                                                     const int* __restrict__ perm, const cplx_t<R>* __restrict__ Vf,
                                                     const cplx_t<R>* __restrict__ Bm, double2 c,
                                                     double2* __restrict__ y) {
     using C = cplx_t<R>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r0 = (blockIdx.x * 4 + wave) * kDisRows;
-    if (r0 >= N) return;
-    const C* V[kDisRows];
-    const C* B[kDisRows];
-    C s[kDisRows];
+    const int i00 = (blockIdx.x * 4 + wave) * ROWS;
+    if (i00 >= N) return;
+    const C* V[ROWS];
+    const C* B[ROWS];
+    C s[ROWS][RB];
 #pragma unroll
-    for (int u = 0; u < kDisRows; ++u) {
-        const int i = min(r0 + u, N - 1);   // rows past N repeat row N-1 and are not written
+    for (int u = 0; u < ROWS; ++u) {
+        const int i = min(i00 + u, N - 1);   // rows past N repeat row N-1 and are not written
         V[u] = Vf + (size_t)i * 3 * K;
-        B[u] = Bm + (size_t)leaf_of[i] * 3 * K;
-        s[u] = C{0, 0};
+        B[u] = Bm + (size_t)leaf_of[i] * RB * 3 * K;
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) s[u][rr] = C{0, 0};
     }
     for (int q = lane; q < K; q += 64) {
 #pragma unroll
-        for (int u = 0; u < kDisRows; ++u) {
-            cfma(s[u], V[u][q], B[u][q]);
-            cfma(s[u], V[u][K + q], B[u][K + q]);
-            cfma(s[u], V[u][2 * K + q], B[u][2 * K + q]);
+        for (int u = 0; u < ROWS; ++u) {
+            const C v0 = V[u][q], v1 = V[u][K + q], v2 = V[u][2 * K + q];
+#pragma unroll
+            for (int rr = 0; rr < RB; ++rr) {
+                const C* b = B[u] + (size_t)rr * 3 * K;
+                cfma(s[u][rr], v0, b[q]);
+                cfma(s[u][rr], v1, b[K + q]);
+                cfma(s[u][rr], v2, b[2 * K + q]);
+            }
         }
     }
 #pragma unroll
-    for (int u = 0; u < kDisRows; ++u) s[u] = wave_sum(s[u]);
-    if (lane < kDisRows && r0 + lane < N) {
-        C v = s[0];
+    for (int u = 0; u < ROWS; ++u)
 #pragma unroll
-        for (int u = 1; u < kDisRows; ++u)
-            if (lane == u) v = s[u];
-        const int r = perm[r0 + lane];
-        double2 t = y[r];
-        cfma(t, c, widen(v));
-        y[r] = t;
+        for (int rr = 0; rr < RB; ++rr) s[u][rr] = wave_sum(s[u][rr]);
+    // lane u * RB + rr writes row u, right-hand side rr
+    if (lane < ROWS * RB) {
+        const int u = lane / RB, rr = lane - u * RB;
+        C v = s[0][0];
+#pragma unroll
+        for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+            for (int q = 0; q < RB; ++q)
+                if (uu == u && q == rr) v = s[uu][q];
+        if (i00 + u < N) {
+            double2* yp = y + (size_t)perm[i00 + u] * ldr + r0 + rr;
+            double2 t = *yp;
+            cfma(t, c, widen(v));
+            *yp = t;
+        }
     }
+}
+
+template <typename R>
+void launch_disagg_pec_b(hipStream_t st, int rb, int N, int K, int ldr, int r0, const int* leaf_of, const int* perm,
+                         const cplx_t<R>* Vf, const cplx_t<R>* Bm, double2 c, double2* y) {
+#define DIS(ROWS, RB)                                                                                      \
+    k_disagg_pec<R, ROWS, RB><<<(N + 4 * ROWS - 1) / (4 * ROWS), 256, 0, st>>>(N, K, ldr, r0, leaf_of, perm, Vf, \
+                                                                               Bm, c, y)
+    if (rb == 1) DIS(4, 1);
+    else if (rb == 2) DIS(2, 2);
+    else if (rb == 4) DIS(1, 4);
+    else DIS(1, 8);
+#undef DIS
 }
 
 template <typename R>
 void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const cplx_t<R>* Vf,
                        const cplx_t<R>* Bm, double2 c, double2* y) {
-    const int rows_per_block = 4 * kDisRows;
-    k_disagg_pec<R><<<(N + rows_per_block - 1) / rows_per_block, 256, 0, st>>>(N, K, leaf_of, perm, Vf, Bm, c, y);
+    launch_disagg_pec_b<R>(st, 1, N, K, 1, 0, leaf_of, perm, Vf, Bm, c, y);
 }
 
 // Code/MLFMM.cpp:285-351.  The K integral sum (-khat x V).B with khat = -kp is
-// evaluated as sum V.(kp x B).
-template <typename R>
-__device__ __forceinline__ void far_sums(int K, const cplx_t<R>* __restrict__ V, const cplx_t<R>* __restrict__ BJ,
-                                         const cplx_t<R>* __restrict__ BM, const R* __restrict__ kp, int lane,
-                                         cplx_t<R>& LJ, cplx_t<R>& KJ, cplx_t<R>& LM, cplx_t<R>& KM) {
+// evaluated as sum V.(kp x B).  Sums of RB right-hand sides: s[rr] = {LJ, KJ, LM, KM}.
+template <typename R, int RB>
+__device__ __forceinline__ void far_sums(int K, const cplx_t<R>* __restrict__ V, const cplx_t<R>* __restrict__ B,
+                                         const R* __restrict__ kp, int lane, cplx_t<R> (&s)[RB][4]) {
     using C = cplx_t<R>;
     for (int q = lane; q < K; q += 64) {
         const R kx = kp[q], ky = kp[K + q], kz = kp[2 * K + q];
         const C v0 = V[q], v1 = V[K + q], v2 = V[2 * K + q];
-        const C j0 = BJ[q], j1 = BJ[K + q], j2 = BJ[2 * K + q];
-        const C m0 = BM[q], m1 = BM[K + q], m2 = BM[2 * K + q];
-        cfma(LJ, v0, j0);
-        cfma(LJ, v1, j1);
-        cfma(LJ, v2, j2);
-        cfma(LM, v0, m0);
-        cfma(LM, v1, m1);
-        cfma(LM, v2, m2);
-        // kp x B
-        const C cj0 = csub(cscale(j2, ky), cscale(j1, kz));
-        const C cj1 = csub(cscale(j0, kz), cscale(j2, kx));
-        const C cj2 = csub(cscale(j1, kx), cscale(j0, ky));
-        const C cm0 = csub(cscale(m2, ky), cscale(m1, kz));
-        const C cm1 = csub(cscale(m0, kz), cscale(m2, kx));
-        const C cm2 = csub(cscale(m1, kx), cscale(m0, ky));
-        cfma(KJ, v0, cj0);
-        cfma(KJ, v1, cj1);
-        cfma(KJ, v2, cj2);
-        cfma(KM, v0, cm0);
-        cfma(KM, v1, cm1);
-        cfma(KM, v2, cm2);
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) {
+            const C* BJ = B + (size_t)rr * 6 * K;
+            const C* BM = BJ + (size_t)3 * K;
+            const C j0 = BJ[q], j1 = BJ[K + q], j2 = BJ[2 * K + q];
+            const C m0 = BM[q], m1 = BM[K + q], m2 = BM[2 * K + q];
+            cfma(s[rr][0], v0, j0);
+            cfma(s[rr][0], v1, j1);
+            cfma(s[rr][0], v2, j2);
+            cfma(s[rr][2], v0, m0);
+            cfma(s[rr][2], v1, m1);
+            cfma(s[rr][2], v2, m2);
+            // kp x B
+            const C cj0 = csub(cscale(j2, ky), cscale(j1, kz));
+            const C cj1 = csub(cscale(j0, kz), cscale(j2, kx));
+            const C cj2 = csub(cscale(j1, kx), cscale(j0, ky));
+            const C cm0 = csub(cscale(m2, ky), cscale(m1, kz));
+            const C cm1 = csub(cscale(m0, kz), cscale(m2, kx));
+            const C cm2 = csub(cscale(m1, kx), cscale(m0, ky));
+            cfma(s[rr][1], v0, cj0);
+            cfma(s[rr][1], v1, cj1);
+            cfma(s[rr][1], v2, cj2);
+            cfma(s[rr][3], v0, cm0);
+            cfma(s[rr][3], v1, cm1);
+            cfma(s[rr][3], v2, cm2);
+        }
     }
 }
 
 // One wavefront per RWG row over the whole RWG range (four rows per block).
-template <typename R>
-__global__ __launch_bounds__(256) void k_disagg_pmchwt(const int* __restrict__ leaf_of, const int* __restrict__ perm,
-                                                       PmchwtFarArgs<R> a, double2* __restrict__ y) {
+template <typename R, int RB>
+__global__ __launch_bounds__(256) void k_disagg_pmchwt(int ldr, int r0, const int* __restrict__ leaf_of,
+                                                       const int* __restrict__ perm, PmchwtFarArgs<R> a,
+                                                       double2* __restrict__ y) {
     using C = cplx_t<R>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wave;
     if (i >= a.N) return;
     const int leaf = leaf_of[i];
-    const C* BJ1 = a.B1 + (size_t)leaf * 6 * a.K1;
-    const C* BM1 = BJ1 + 3 * (size_t)a.K1;
-    const C* BJ2 = a.B2 + (size_t)leaf * 6 * a.K2;
-    const C* BM2 = BJ2 + 3 * (size_t)a.K2;
-    C L1J = {0, 0}, K1J = {0, 0}, L1M = {0, 0}, K1M = {0, 0};
-    C L2J = {0, 0}, K2J = {0, 0}, L2M = {0, 0}, K2M = {0, 0};
-    far_sums<R>(a.K1, a.Vf1 + (size_t)i * 3 * a.K1, BJ1, BM1, a.kp1, lane, L1J, K1J, L1M, K1M);
-    far_sums<R>(a.K2, a.Vf2 + (size_t)i * 3 * a.K2, BJ2, BM2, a.kp2, lane, L2J, K2J, L2M, K2M);
-    L1J = wave_sum(L1J);
-    K1J = wave_sum(K1J);
-    L1M = wave_sum(L1M);
-    K1M = wave_sum(K1M);
-    L2J = wave_sum(L2J);
-    K2J = wave_sum(K2J);
-    L2M = wave_sum(L2M);
-    K2M = wave_sum(K2M);
-    if (lane == 0) {
+    C s1[RB][4], s2[RB][4];
+#pragma unroll
+    for (int rr = 0; rr < RB; ++rr)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) s1[rr][f] = s2[rr][f] = C{0, 0};
+    far_sums<R, RB>(a.K1, a.Vf1 + (size_t)i * 3 * a.K1, a.B1 + (size_t)leaf * RB * 6 * a.K1, a.kp1, lane, s1);
+    far_sums<R, RB>(a.K2, a.Vf2 + (size_t)i * 3 * a.K2, a.B2 + (size_t)leaf * RB * 6 * a.K2, a.kp2, lane, s2);
+#pragma unroll
+    for (int rr = 0; rr < RB; ++rr)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            s1[rr][f] = wave_sum(s1[rr][f]);
+            s2[rr][f] = wave_sum(s2[rr][f]);
+        }
+    if (lane < RB) {
+        C t1[4], t2[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) t1[f] = s1[0][f], t2[f] = s2[0][f];
+#pragma unroll
+        for (int rr = 1; rr < RB; ++rr)
+            if (lane == rr)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) t1[f] = s1[rr][f], t2[f] = s2[rr][f];
+        const double2 L1J = widen(t1[0]), K1J = widen(t1[1]), L1M = widen(t1[2]), K1M = widen(t1[3]);
+        const double2 L2J = widen(t2[0]), K2J = widen(t2[1]), L2M = widen(t2[2]), K2M = widen(t2[3]);
         const int r = perm[i];
-        double2 wj = y[r], wm = y[a.N + r];
+        double2* pj = y + (size_t)r * ldr + r0 + lane;
+        double2* pm = y + ((size_t)a.N + r) * ldr + r0 + lane;
+        double2 wj = *pj, wm = *pm;
         // Code/MLFMM.cpp:339-349
-        wj = cadd(wj, cmul(a.c1, widen(L1J)));
-        wj = csub(wj, cmul(a.c1, widen(K1M)));
-        wm = cadd(wm, cmul(a.c1, widen(K1J)));
-        wm = cadd(wm, cmul(a.c1, widen(L1M)));
-        wj = cadd(wj, cmul(cmul(a.c2, a.etai), widen(L2J)));
-        wj = csub(wj, cmul(a.c2, widen(K2M)));
-        wm = cadd(wm, cmul(a.c2, widen(K2J)));
-        wm = cadd(wm, cmul(cmul(a.c2, widen(L2M)), a.inv_etai));
-        y[r] = wj;
-        y[a.N + r] = wm;
+        wj = cadd(wj, cmul(a.c1, L1J));
+        wj = csub(wj, cmul(a.c1, K1M));
+        wm = cadd(wm, cmul(a.c1, K1J));
+        wm = cadd(wm, cmul(a.c1, L1M));
+        wj = cadd(wj, cmul(cmul(a.c2, a.etai), L2J));
+        wj = csub(wj, cmul(a.c2, K2M));
+        wm = cadd(wm, cmul(a.c2, K2J));
+        wm = cadd(wm, cmul(cmul(a.c2, L2M), a.inv_etai));
+        *pj = wj;
+        *pm = wm;
     }
+}
+
+template <typename R>
+void launch_disagg_pmchwt_b(hipStream_t st, int rb, int ldr, int r0, const int* leaf_of, const int* perm,
+                            const PmchwtFarArgs<R>& a, double2* y) {
+    const int blocks = (a.N + 3) / 4;
+    if (rb == 1) k_disagg_pmchwt<R, 1><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
+    else if (rb == 2) k_disagg_pmchwt<R, 2><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
+    else k_disagg_pmchwt<R, 4><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
 }
 
 template <typename R>
 void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, const PmchwtFarArgs<R>& a,
                           double2* y) {
-    k_disagg_pmchwt<R><<<(a.N + 3) / 4, 256, 0, st>>>(leaf_of, perm, a, y);
+    launch_disagg_pmchwt_b<R>(st, 1, 1, 0, leaf_of, perm, a, y);
 }
 
 __global__ void k_narrow(size_t n, const double2* __restrict__ in, float2* __restrict__ out) {
@@ -398,7 +466,13 @@ void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
                                     const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);                                           \
     template void launch_disagg_pec<R>(hipStream_t, int, int, const int*, const int*, const cplx_t<R>*, const cplx_t<R>*,  \
                                        double2, double2*);                                                        \
-    template void launch_disagg_pmchwt<R>(hipStream_t, const int*, const int*, const PmchwtFarArgs<R>&, double2*);
+    template void launch_disagg_pmchwt<R>(hipStream_t, const int*, const int*, const PmchwtFarArgs<R>&, double2*); \
+    template void launch_aggregate_b<R>(hipStream_t, int, int, int, int, int, int, int, const int*, const int*,     \
+                                        const int*, const cplx_t<R>*, const double2*, cplx_t<R>*);                \
+    template void launch_disagg_pec_b<R>(hipStream_t, int, int, int, int, int, const int*, const int*,             \
+                                         const cplx_t<R>*, const cplx_t<R>*, double2, double2*);                  \
+    template void launch_disagg_pmchwt_b<R>(hipStream_t, int, int, int, const int*, const int*,                    \
+                                            const PmchwtFarArgs<R>&, double2*);
 MLFMA_FAR_INSTANTIATE(double)
 MLFMA_FAR_INSTANTIATE(float)
 #undef MLFMA_FAR_INSTANTIATE
@@ -517,5 +591,142 @@ void launch_near(hipStream_t st, const NearDev& nd, const cplx_t<S>* Z, const do
 }
 template void launch_near<double>(hipStream_t, const NearDev&, const double2*, const double2*, double2*);
 template void launch_near<float>(hipStream_t, const NearDev&, const float2*, const double2*, double2*);
+
+// Batched near product: RB right-hand sides per pass over the entries (vectors
+// RHS-fastest, element (u, r) at u * ldr + r).  Same column-group scheme as
+// k_near with 256 threads; the x chunk is staged as [column][rhs] and each
+// thread keeps RB (PMCHWT: 2 RB) FP64 accumulators.
+template <typename S, int P, int RB, int U, int CH>
+__global__ __launch_bounds__(256) void k_near_b(NearDev nd, int ldr, int r0, const cplx_t<S>* __restrict__ Z,
+                                                const double2* __restrict__ x, double2* __restrict__ y) {
+    constexpr int NX = (P == 3) ? 2 : 1;
+    constexpr int BS = 256;
+    __shared__ double2 xs[NX][CH][RB];
+    __shared__ double2 red[NX][BS];
+    const int t = nd.order[blockIdx.x];
+    const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
+    if (nt == 0) return;
+    const int C = nd.tgt_cols[t];
+    const int* cref = nd.colref + nd.col_ptr[t];
+    const int plane = C * nt;
+    const cplx_t<S>* Zt = Z + nd.tgt_off[t];
+    const int span = nt < BS ? nt : BS;
+    const int G = BS / span;
+    const int g = threadIdx.x / span, rl = threadIdx.x - g * span;
+    for (int rs = 0; rs < nt; rs += span) {
+        const int row = rs + rl;
+        const bool act = g < G && row < nt;
+        double2 aJ[RB], aM[RB];
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) aJ[rr] = aM[rr] = make_double2(0.0, 0.0);
+        for (int c0 = 0; c0 < C; c0 += CH) {
+            const int cl = min(CH, C - c0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < cl * RB; e += BS) {
+                const int c = e / RB, rr = e - c * RB;
+                const int r = cref[c0 + c];
+                xs[0][c][rr] = x[(size_t)r * ldr + r0 + rr];
+                if (P == 3) xs[NX - 1][c][rr] = x[((size_t)nd.N + r) * ldr + r0 + rr];
+            }
+            __syncthreads();
+            if (act) {
+                const cplx_t<S>* z = Zt + c0 * nt + row;
+                const int step = G * nt;
+                int c = g;
+                for (; c + (U - 1) * G < cl; c += U * G) {
+                    double2 p[U], k[U], m[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int o = c * nt + u * step;
+                        p[u] = widen(z[o]);
+                        if (P == 3) {
+                            k[u] = widen(z[o + plane]);
+                            m[u] = widen(z[o + 2 * plane]);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int rr = 0; rr < RB; ++rr) {
+                            const double2 xJ = xs[0][c + u * G][rr];
+                            cfma(aJ[rr], p[u], xJ);
+                            if (P == 3) {
+                                const double2 xM = xs[NX - 1][c + u * G][rr];
+                                cfms(aJ[rr], k[u], xM);
+                                cfma(aM[rr], k[u], xJ);
+                                cfma(aM[rr], m[u], xM);
+                            }
+                        }
+                }
+                for (; c < cl; c += G) {
+                    const int o = c * nt;
+                    const double2 p0 = widen(z[o]);
+                    double2 k0 = p0, m0 = p0;
+                    if (P == 3) {
+                        k0 = widen(z[o + plane]);
+                        m0 = widen(z[o + 2 * plane]);
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < RB; ++rr) {
+                        const double2 xJ = xs[0][c][rr];
+                        cfma(aJ[rr], p0, xJ);
+                        if (P == 3) {
+                            const double2 xM = xs[NX - 1][c][rr];
+                            cfms(aJ[rr], k0, xM);
+                            cfma(aM[rr], k0, xJ);
+                            cfma(aM[rr], m0, xM);
+                        }
+                    }
+                }
+            }
+        }
+        // fold the column groups one right-hand side at a time
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) {
+            if (G > 1) {
+                __syncthreads();
+                red[0][threadIdx.x] = aJ[rr];
+                if (P == 3) red[NX - 1][threadIdx.x] = aM[rr];
+                __syncthreads();
+                if (threadIdx.x < nt) {
+                    double2 sJ = red[0][threadIdx.x], sM = red[NX - 1][threadIdx.x];
+                    for (int h = 1; h < G; ++h) {
+                        sJ = cadd(sJ, red[0][h * nt + threadIdx.x]);
+                        if (P == 3) sM = cadd(sM, red[NX - 1][h * nt + threadIdx.x]);
+                    }
+                    const size_t r = nd.perm[i0 + threadIdx.x];
+                    y[r * ldr + r0 + rr] = sJ;
+                    if (P == 3) y[((size_t)nd.N + r) * ldr + r0 + rr] = sM;
+                }
+            } else if (act) {
+                const size_t r = nd.perm[i0 + row];
+                y[r * ldr + r0 + rr] = aJ[rr];
+                if (P == 3) y[((size_t)nd.N + r) * ldr + r0 + rr] = aM[rr];
+            }
+        }
+    }
+}
+
+template <typename S>
+void launch_near_b(hipStream_t st, int rb, int ldr, int r0, const NearDev& nd, const cplx_t<S>* Z, const double2* x,
+                   double2* y) {
+    if (nd.nleaf == 0) return;
+    if (rb == 1) {
+        if (ldr == 1 && r0 == 0) launch_near<S>(st, nd, Z, x, y);
+        else if (nd.planes == 1) k_near_b<S, 1, 1, 8, 512><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+        else k_near_b<S, 3, 1, 4, 256><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+    } else if (nd.planes == 1) {
+        if (rb == 2) k_near_b<S, 1, 2, 8, 512><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+        else if (rb == 4) k_near_b<S, 1, 4, 4, 256><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+        else k_near_b<S, 1, 8, 4, 256><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+    } else {
+        if (rb == 2) k_near_b<S, 3, 2, 2, 256><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+        else k_near_b<S, 3, 4, 2, 128><<<nd.nleaf, 256, 0, st>>>(nd, ldr, r0, Z, x, y);
+    }
+}
+template void launch_near_b<double>(hipStream_t, int, int, int, const NearDev&, const double2*, const double2*,
+                                    double2*);
+template void launch_near_b<float>(hipStream_t, int, int, int, const NearDev&, const float2*, const double2*,
+                                   double2*);
 
 }  // namespace mlfma

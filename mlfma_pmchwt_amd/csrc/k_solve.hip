@@ -128,6 +128,150 @@ void launch_bidot_partial(hipStream_t st, int n, const double2* a, const double2
     k_bidot_partial<<<nblk, 256, 0, st>>>(n, a, b, partial);
 }
 
+// ---------------------------------------------------------------- batched (R right-hand sides)
+// Block vectors are RHS-fastest: element (u, r) at u * R + r, R <= 256.  A
+// block of 256 threads covers rows of floor(256 / R) unknowns; thread t owns
+// right-hand side t % R, and the per-r block sums are folded in a fixed order.
+namespace {
+__device__ __forceinline__ void fold_rows(double2 v, int R, double2* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    const int rows = blockDim.x / R;
+    if ((int)threadIdx.x < R) {
+        double2 s = red[threadIdx.x];
+        for (int q = 1; q < rows; ++q) s = cadd(s, red[q * R + threadIdx.x]);
+        red[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+}  // namespace
+
+// partial[(j * nblk + b) * R + r] = sum over block b of conj(V_j[u][r]) w[u][r]
+__global__ __launch_bounds__(256) void k_multidot_b(int n, int R, const double2* __restrict__ V, size_t ldv,
+                                                    const double2* __restrict__ w, double2* __restrict__ partial) {
+    __shared__ double2 red[256];
+    const int j = blockIdx.y;
+    const int rows = blockDim.x / R;
+    const int r = threadIdx.x % R, q = threadIdx.x / R;
+    const double2* v = V + (size_t)j * ldv;
+    double2 s = make_double2(0, 0);
+    if (q < rows)
+        for (int u = blockIdx.x * rows + q; u < n; u += gridDim.x * rows) {
+            const size_t e = (size_t)u * R + r;
+            s = cadd(s, cmulc(v[e], w[e]));
+        }
+    fold_rows(s, R, red);
+    if ((int)threadIdx.x < R) partial[((size_t)j * gridDim.x + blockIdx.x) * R + threadIdx.x] = red[threadIdx.x];
+}
+
+void launch_multidot_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* w,
+                       double2* partial, int nblk) {
+    k_multidot_b<<<dim3(nblk, m), 256, 0, st>>>(n, R, V, ldv, w, partial);
+}
+
+// out[j * R + r] = sum_b partial[(j * nblk + b) * R + r], b ascending
+__global__ void k_reduce_partials_b(int nblk, int R, const double2* __restrict__ partial, double2* __restrict__ out) {
+    const int j = blockIdx.x;
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        double2 s = make_double2(0, 0);
+        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[((size_t)j * nblk + b) * R + r]);
+        out[(size_t)j * R + r] = s;
+    }
+}
+
+void launch_reduce_partials_b(hipStream_t st, int m, int R, int nblk, const double2* partial, double2* out) {
+    k_reduce_partials_b<<<m, 256, 0, st>>>(nblk, R, partial, out);
+}
+
+// partial[b * R + r] = sum over block b of |x[u][r]|^2
+__global__ __launch_bounds__(256) void k_norm2_partial_b(int n, int R, const double2* __restrict__ x,
+                                                         double2* __restrict__ partial) {
+    __shared__ double2 red[256];
+    const int rows = blockDim.x / R;
+    const int r = threadIdx.x % R, q = threadIdx.x / R;
+    double2 s = make_double2(0, 0);
+    if (q < rows)
+        for (int u = blockIdx.x * rows + q; u < n; u += gridDim.x * rows) {
+            const double2 a = x[(size_t)u * R + r];
+            s.x += a.x * a.x + a.y * a.y;
+        }
+    fold_rows(s, R, red);
+    if ((int)threadIdx.x < R) partial[(size_t)blockIdx.x * R + threadIdx.x] = red[threadIdx.x];
+}
+
+void launch_norm2_partial_b(hipStream_t st, int n, int R, const double2* x, double2* partial, int nblk) {
+    k_norm2_partial_b<<<nblk, 256, 0, st>>>(n, R, x, partial);
+}
+
+// w[u][r] -= sum_j h[j * R + r] V_j[u][r]
+__global__ __launch_bounds__(256) void k_sub_combination_b(size_t nR, int R, int m, const double2* __restrict__ V,
+                                                           size_t ldv, const double2* __restrict__ h,
+                                                           double2* __restrict__ w) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nR) return;
+    const int r = (int)(e % R);
+    double2 a = w[e];
+    for (int j = 0; j < m; ++j) cfms(a, h[(size_t)j * R + r], V[(size_t)j * ldv + e]);
+    w[e] = a;
+}
+
+void launch_sub_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* h,
+                              double2* w) {
+    const size_t nR = (size_t)n * R;
+    k_sub_combination_b<<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, m, V, ldv, h, w);
+}
+
+// x[u][r] += sum_j y[j * R + r] V_j[u][r]
+__global__ __launch_bounds__(256) void k_add_combination_b(size_t nR, int R, int m, const double2* __restrict__ V,
+                                                           size_t ldv, const double2* __restrict__ y,
+                                                           double2* __restrict__ x) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nR) return;
+    const int r = (int)(e % R);
+    double2 a = x[e];
+    for (int j = 0; j < m; ++j) cfma(a, y[(size_t)j * R + r], V[(size_t)j * ldv + e]);
+    x[e] = a;
+}
+
+void launch_add_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* y,
+                              double2* x) {
+    const size_t nR = (size_t)n * R;
+    k_add_combination_b<<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, m, V, ldv, y, x);
+}
+
+// y[u][r] = a[r] * x[u][r]  (a real, one per right-hand side)
+__global__ void k_scale_copy_b(size_t nR, int R, const double* __restrict__ a, const double2* __restrict__ x,
+                               double2* __restrict__ y) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nR) y[e] = cscale(x[e], a[e % R]);
+}
+
+void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, double2* y) {
+    const size_t nR = (size_t)n * R;
+    k_scale_copy_b<<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, a, x, y);
+}
+
+// column-major (vector r at in + r * n) <-> RHS-fastest (element (u, r) at u * R + r)
+__global__ void k_transpose(int rows, int cols, const double2* __restrict__ in, double2* __restrict__ out) {
+    __shared__ double2 tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[k][threadIdx.x] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[threadIdx.x][k];
+    }
+}
+
+// out (cols x rows, row-major) = transpose of in (rows x cols, row-major)
+void launch_transpose(hipStream_t st, int rows, int cols, const double2* in, double2* out) {
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+    k_transpose<<<grid, dim3(32, 8), 0, st>>>(rows, cols, in, out);
+}
+
 // y_t = D_t^{-1} x_t on every leaf; D_t^{-1} dense column-major of order
 // nt (PEC) or 2 nt (PMCHWT, unknowns ordered J then M).
 __global__ __launch_bounds__(256) void k_block_diag(int planes, int N, int max_leaf, const int* __restrict__ leaf_ptr,

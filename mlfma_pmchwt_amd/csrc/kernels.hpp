@@ -70,6 +70,23 @@ void launch_disagg_pmchwt(hipStream_t st, const int* leaf_of, const int* perm, c
 template <typename S>
 void launch_near(hipStream_t st, const NearDev& nd, const cplx_t<S>* Z, const double2* x, double2* y);
 
+// Batched forms: rb right-hand sides per pass (rb in 1, 2, 4, 8 with
+// rb * ncur <= 8); solver vectors RHS-fastest, element (unknown u, rhs r) at
+// u * ldr + r; the pass covers r0 .. r0 + rb - 1; workspace current
+// c = rr * ncur + cur (m2m / m2l_l2l take nc = rb * ncur as their ncur).
+template <typename R>
+void launch_aggregate_b(hipStream_t st, int ncur, int rb, int nleaf, int K, int N, int ldr, int r0, const int* order,
+                        const int* leaf_ptr, const int* perm, const cplx_t<R>* Vs, const double2* x, cplx_t<R>* Sm);
+template <typename R>
+void launch_disagg_pec_b(hipStream_t st, int rb, int N, int K, int ldr, int r0, const int* leaf_of, const int* perm,
+                         const cplx_t<R>* Vf, const cplx_t<R>* Bm, double2 c, double2* y);
+template <typename R>
+void launch_disagg_pmchwt_b(hipStream_t st, int rb, int ldr, int r0, const int* leaf_of, const int* perm,
+                            const PmchwtFarArgs<R>& a, double2* y);
+template <typename S>
+void launch_near_b(hipStream_t st, int rb, int ldr, int r0, const NearDev& nd, const cplx_t<S>* Z, const double2* x,
+                   double2* y);
+
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
 

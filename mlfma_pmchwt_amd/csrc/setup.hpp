@@ -50,6 +50,20 @@ void launch_norm2_partial(hipStream_t st, int n, const double2* x, double* parti
 void launch_axpy(hipStream_t st, int n, double2 a, const double2* z, const double2* x, double2* y);
 // sum_i a_i b_i (bilinear) partials
 void launch_bidot_partial(hipStream_t st, int n, const double2* a, const double2* b, double2* partial, int nblk);
+// Batched Krylov kernels: R <= 256 right-hand sides, vectors RHS-fastest
+// (element (u, r) at u * R + r); per-r scalars indexed j * R + r.
+void launch_multidot_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* w,
+                       double2* partial, int nblk);
+void launch_reduce_partials_b(hipStream_t st, int m, int R, int nblk, const double2* partial, double2* out);
+void launch_norm2_partial_b(hipStream_t st, int n, int R, const double2* x, double2* partial, int nblk);
+void launch_sub_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* h,
+                              double2* w);
+void launch_add_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* y,
+                              double2* x);
+void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, double2* y);
+// out (cols x rows) = transpose of in (rows x cols), both row-major
+void launch_transpose(hipStream_t st, int rows, int cols, const double2* in, double2* out);
+
 // block-diagonal preconditioner: y_t = D_t^{-1} x_t per leaf (dense inverse blocks)
 void launch_block_diag(hipStream_t st, int planes, int nleaf, int N, const int* leaf_ptr, const int* perm,
                        const long long* inv_off, const double2* Dinv, int max_leaf, const double2* x, double2* y);

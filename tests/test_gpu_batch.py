@@ -1,0 +1,93 @@
+"""Batched right-hand sides: the batched matvec and the lockstep GMRES against
+the single-vector path and the oracle.
+
+The batched kernels change only the summation order of the near-field column
+groups, so a batched matvec matches the single-vector one to rounding
+(1e-13) and the oracle to the FP64 parity bar (1e-10).  Lockstep GMRES runs
+each right-hand side's own recurrences, so its iteration counts equal the
+single solves' and its solutions agree to far below the tolerance.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr, seeded_x
+from mlfma_pmchwt_amd import api, geometry
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "efie_ml": ((2, 1.5), api.EFIE, (1.0, 0.0)),
+    "cfie_ml": ((2, 1.5), api.CFIE, (1.0, 0.0)),
+    "pmchwt_lossy_ml": ((2, 1.5), api.PMCHWT, (2.5, -0.1)),
+    "pmchwt_1lvl": ((2, 0.5), api.PMCHWT, (4.0, 0.0)),
+}
+_cache = {}
+
+
+def pair(name):
+    if name not in _cache:
+        (s, r), ie, eps = CASES[name]
+        mesh = geometry.icosphere(s, r)
+        print(f"[case] {name}: icosphere s={s} r={r} ie={ie} eps={eps} batched", flush=True)
+        op = api.Operator(mesh, integral_equation=ie, eps_r_re=eps[0], eps_r_im=eps[1], max_rhs=64,
+                          far_precision=api.FP32, near_precision=api.FP32)
+        orc = pyoracle.Oracle(mesh, ie=ie, eps=eps)
+        _cache[name] = (mesh, op, orc)
+    return _cache[name]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("R", [1, 3, 8, 13])
+def test_batched_matvec(name, R):
+    _, op, orc = pair(name)
+    X = np.stack([seeded_x(op.n, 100 + j) for j in range(R)], axis=1)
+    for prec in (api.FP64, api.FP32):
+        op.set_apply_precision(prec)
+        Y = op.apply(X)
+        for j in range(R):
+            y1 = op.apply(np.ascontiguousarray(X[:, j]))
+            tol = 1e-13 if prec == api.FP64 else 1e-6
+            assert relerr(Y[:, j], y1) <= tol
+            if prec == api.FP64 and j in (0, R - 1):
+                assert relerr(Y[:, j], orc.apply(X[:, j])) <= 1e-10
+    op.set_apply_precision(api.FP64)
+
+
+@pytest.mark.parametrize("name", ["cfie_ml", "pmchwt_1lvl"])
+def test_batched_solve_matches_single(name):
+    _, op, orc = pair(name)
+    op.set_apply_precision(api.FP64)
+    inc = [(0.0, th, 0.0) for th in (0.0, 40.0, 90.0, 150.0)] + [(90.0, 70.0, 0.0)]
+    B = np.stack([op.excitation(*a) for a in inc], axis=1)
+    Xb, stb = op.solve_batch(B, tol_rel=1e-6, restart=30)
+    for j, a in enumerate(inc):
+        x1, st1 = op.solve(B[:, j], tol_rel=1e-6, restart=30)
+        print(f"[batch solve] {name} inc={a} it batch={stb[j]['iterations']} single={st1['iterations']} "
+              f"res={stb[j]['rel_residual']:.2e}")
+        assert stb[j]["converged"] and stb[j]["rel_residual"] < 1e-6
+        assert abs(stb[j]["iterations"] - st1["iterations"]) <= 1
+        assert relerr(Xb[:, j], x1) <= 1e-6
+        # true residual against the oracle operator
+        assert relerr(orc.apply(Xb[:, j]), B[:, j]) < 1.01e-6
+
+
+def test_batched_solve_fp32_mode():
+    _, op, orc = pair("cfie_ml")
+    op.set_apply_precision(api.FP32)
+    B = np.stack([op.excitation(0.0, th, 0.0) for th in np.linspace(0, 180, 9)], axis=1)
+    Xb, stb = op.solve_batch(B, tol_rel=1e-3, restart=30)
+    for j in range(B.shape[1]):
+        assert stb[j]["converged"] and stb[j]["rel_residual"] < 1e-3
+        assert relerr(orc.apply(Xb[:, j]), B[:, j]) < 1e-3
+    op.set_apply_precision(api.FP64)
+
+
+def test_batched_limits(sphere_small):
+    op = api.Operator(sphere_small, integral_equation=api.CFIE, max_rhs=4)
+    X = np.ones((op.n, 300), dtype=np.complex128)
+    with pytest.raises(RuntimeError):
+        op.apply(X)
+    B = np.ones((op.n, 2), dtype=np.complex128)
+    with pytest.raises(ValueError):
+        op.solve_batch(B[:1])
