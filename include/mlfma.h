@@ -124,7 +124,9 @@ typedef struct mlfma_solve_params {
     int restart;        /* GMRES(mr) */
     double tol_rel;     /* stop when ||r|| / ||b|| < tol_rel (or < tol_abs) */
     double tol_abs;
-    int precond;        /* 0 none (reference), 1 near-field block-diagonal */
+    int precond;        /* 0 none (reference), 1 near-field block-diagonal, 2 near-field polynomial
+                           (coefficients from mlfma_set_precond_poly, e.g. predicted by the network);
+                           mlfma_solve_batch accepts 0 and 2 */
 } mlfma_solve_params;
 
 typedef struct mlfma_solve_stats {
@@ -134,6 +136,7 @@ typedef struct mlfma_solve_stats {
     double rel_residual;      /* true ||b - A x|| / ||b|| with the FP64 operator */
     double seconds;
     int fp64_matvecs;         /* of matvecs, those with the FP64 operator (true residuals) */
+    int precond_fallbacks;    /* 1 if a stalling polynomial preconditioner was dropped mid-solve */
 } mlfma_solve_stats;
 
 int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const double* rhs, double* x,
@@ -148,6 +151,60 @@ int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const doub
  * 428-497) and of batch mode (Code/BatchRunner.h:422-484); precond must be 0. */
 int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int nrhs, const double* rhs, double* x,
                       mlfma_solve_stats* stats);
+
+/* ---- Learned near-field preconditioner (no reference counterpart; the
+ * reference solves unpreconditioned, Code/MLFMM.cpp:370-397).
+ *
+ * precond = 2 applies M^-1 r = sum_k alpha_k t_k with t_0 = D^-1 r and
+ * t_{k+1} = t_k - D^-1 Z_near t_k (D the operator's diagonal, Z_near the stored
+ * near field): one near-field product per degree.  Its coefficients are
+ * either fitted from the solver's own Krylov pairs (mlfma_precond_statistics)
+ * or predicted for an unseen problem by the coefficient network
+ * (mlfma_mlp_*) from mlfma_precond_features.  A solve drops a polynomial
+ * preconditioner that stalls (true residual not 10% lower after a restart
+ * cycle) and continues unpreconditioned. */
+#define MLFMA_PRECOND_FEATURES 20
+
+/* 20 problem features: spectral moments mu_k = z^H (D^-1 Z_near)^k z / n,
+ * k = 1..8 (8 real parts, 8 imaginary parts) for a seeded unit-modulus probe
+ * z, then eps_r (re, im), PMCHWT flag, log10(n_unknowns). */
+int mlfma_precond_features(mlfma_handle* h, float* features);
+
+/* Least-squares statistics of the coefficients from `steps` Arnoldi steps of
+ * the unpreconditioned operator started at rhs: pairs (r_k = A v_k, e_k = v_k),
+ * basis a_k = [t_0(r_k) .. t_degree(r_k)]; G = sum a_k^H a_k ((degree+1)^2
+ * complex, row-major), hv = sum a_k^H e_k, c = sum |e_k|^2. */
+int mlfma_precond_statistics(mlfma_handle* h, const double* rhs, int steps, int degree, double* G, double* hv,
+                             double* c);
+
+/* Sets the degree + 1 complex coefficients (interleaved re, im). */
+int mlfma_set_precond_poly(mlfma_handle* h, int degree, const double* alpha);
+
+/* y = M^-1 x for kind 1 (block-diagonal) or 2 (polynomial), host vectors. */
+int mlfma_precond_apply(mlfma_handle* h, int kind, int n, const double* x, double* y);
+
+/* The coefficient network: n_in -> width -> width -> n_out, tanh hidden
+ * layers, FP32, trained on the device.  Loss per sample s (features x_s,
+ * statistics G_s, h_s, c_s of one problem): with a = net(x_s) read as
+ * n_out / 2 complex coefficients (real parts first),
+ *   l_s = (a^H G_s a - 2 Re a^H h_s + c_s) / c_s,   loss = mean_s l_s. */
+typedef struct mlfma_mlp mlfma_mlp;
+int mlfma_mlp_create(int n_in, int width, int n_out, unsigned seed, mlfma_mlp** out);
+void mlfma_mlp_destroy(mlfma_mlp* m);
+const char* mlfma_mlp_last_error(const mlfma_mlp* m);
+/* parameter vector: W1 (width x n_in), b1, W2 (width x width), b2, W3 (n_out x width), b3 */
+int mlfma_mlp_num_params(const mlfma_mlp* m, long long* count);
+int mlfma_mlp_get_params(mlfma_mlp* m, float* params);
+int mlfma_mlp_set_params(mlfma_mlp* m, const float* params);
+/* inputs are standardised as (x - mean) / std before the first layer */
+int mlfma_mlp_set_normalization(mlfma_mlp* m, const float* mean, const float* stdv);
+int mlfma_mlp_predict(mlfma_mlp* m, int batch, const float* x, float* y);
+/* loss and its gradient with respect to the parameter vector (grad may be NULL) */
+int mlfma_mlp_loss_grad(mlfma_mlp* m, int batch, const float* x, const double* G, const double* h, const double* c,
+                        double* loss, float* grad);
+/* `steps` full-batch Adam steps (beta 0.9 / 0.999), loss per step in loss_history */
+int mlfma_mlp_train(mlfma_mlp* m, int batch, const float* x, const double* G, const double* h, const double* c,
+                    int steps, float lr, float* loss_history);
 
 /* Plane-wave excitation for the handle's equation; replaces RHS::computeV_E,
  * computeV_C and computeV_PMCHWT (Code/Vm.h:10-203) with EMSource::initPW

@@ -17,6 +17,8 @@ _LIB_PATH = Path(__file__).resolve().parent / "libmlfma.so"
 
 EFIE, CFIE, PMCHWT = 0, 1, 2
 FP64, FP32 = 0, 1
+PRECOND_NONE, PRECOND_BLOCK_DIAG, PRECOND_POLY = 0, 1, 2
+PRECOND_FEATURES = 20
 GMRES, CGS = 0, 1
 STAGE_AGGREGATE, STAGE_M2M, STAGE_M2L_L2L, STAGE_FAR, STAGE_NEAR = range(5)
 
@@ -40,7 +42,8 @@ class SolveParams(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("converged", C.c_int),
-                ("rel_residual", C.c_double), ("seconds", C.c_double), ("fp64_matvecs", C.c_int)]
+                ("rel_residual", C.c_double), ("seconds", C.c_double), ("fp64_matvecs", C.c_int),
+                ("precond_fallbacks", C.c_int)]
 
 
 _lib = None
@@ -66,6 +69,21 @@ def library() -> C.CDLL:
         lib.mlfma_apply_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_apply_device.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_set_apply_precision.argtypes = [vp, C.c_int]
+        lib.mlfma_precond_features.argtypes = [vp, vp]
+        lib.mlfma_precond_statistics.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        lib.mlfma_set_precond_poly.argtypes = [vp, C.c_int, vp]
+        lib.mlfma_precond_apply.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]
+        lib.mlfma_mlp_destroy.argtypes = [vp]
+        lib.mlfma_mlp_last_error.argtypes = [vp]
+        lib.mlfma_mlp_last_error.restype = C.c_char_p
+        lib.mlfma_mlp_num_params.argtypes = [vp, C.POINTER(C.c_longlong)]
+        lib.mlfma_mlp_get_params.argtypes = [vp, vp]
+        lib.mlfma_mlp_set_params.argtypes = [vp, vp]
+        lib.mlfma_mlp_set_normalization.argtypes = [vp, vp, vp]
+        lib.mlfma_mlp_predict.argtypes = [vp, C.c_int, vp, vp]
+        lib.mlfma_mlp_loss_grad.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.POINTER(C.c_double), vp]
+        lib.mlfma_mlp_train.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_float, vp]
         lib.mlfma_solve.argtypes = [vp, C.POINTER(SolveParams), C.c_int, vp, vp, C.POINTER(SolveStats)]
         lib.mlfma_solve_batch.argtypes = [vp, C.POINTER(SolveParams), C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_excitation.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
@@ -168,25 +186,52 @@ class Operator:
         if rc not in (0, 5) or (check and rc != 0):
             self._check(rc)
         return x, {"iterations": st.iterations, "matvecs": st.matvecs, "converged": bool(st.converged),
-                   "rel_residual": st.rel_residual, "seconds": st.seconds, "fp64_matvecs": st.fp64_matvecs}
+                   "rel_residual": st.rel_residual, "seconds": st.seconds, "fp64_matvecs": st.fp64_matvecs,
+                   "precond_fallbacks": st.precond_fallbacks}
 
     def solve_batch(self, rhs: np.ndarray, x0: np.ndarray | None = None, itr_max=None, restart=30, tol_rel=1e-3,
-                    tol_abs=0.0):
+                    tol_abs=0.0, precond=0):
         """GMRES for the columns of rhs (n x R) in lockstep with batched matvecs."""
         rhs = np.asfortranarray(rhs, dtype=np.complex128)
         if rhs.ndim != 2 or rhs.shape[0] != self.n:
             raise ValueError("rhs must be n x R")
         R = rhs.shape[1]
         x = np.zeros_like(rhs, order="F") if x0 is None else np.asfortranarray(x0, dtype=np.complex128).copy(order="F")
-        sp = SolveParams(GMRES, itr_max if itr_max is not None else 4 * self.n, restart, tol_rel, tol_abs, 0)
+        sp = SolveParams(GMRES, itr_max if itr_max is not None else 4 * self.n, restart, tol_rel, tol_abs, precond)
         st = (SolveStats * R)()
         rc = library().mlfma_solve_batch(self._h, C.byref(sp), self.n, R, rhs.ctypes.data_as(C.c_void_p),
                                          x.ctypes.data_as(C.c_void_p), st)
         if rc not in (0, 5):
             self._check(rc)
         return x, [{"iterations": s.iterations, "matvecs": s.matvecs, "converged": bool(s.converged),
-                    "rel_residual": s.rel_residual, "seconds": s.seconds, "fp64_matvecs": s.fp64_matvecs}
-                   for s in st]
+                    "rel_residual": s.rel_residual, "seconds": s.seconds, "fp64_matvecs": s.fp64_matvecs,
+                    "precond_fallbacks": s.precond_fallbacks} for s in st]
+
+    # ---------------------------------------------------------------- learned preconditioner
+    def precond_features(self) -> np.ndarray:
+        f = np.zeros(PRECOND_FEATURES, dtype=np.float32)
+        self._check(library().mlfma_precond_features(self._h, _p(f)))
+        return f
+
+    def precond_statistics(self, rhs: np.ndarray, steps: int = 30, degree: int = 6):
+        """(G, h, c) of the polynomial coefficients from `steps` Krylov pairs started at rhs."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+        m = degree + 1
+        G = np.zeros((m, m), dtype=np.complex128)
+        h = np.zeros(m, dtype=np.complex128)
+        c = np.zeros(1)
+        self._check(library().mlfma_precond_statistics(self._h, _p(rhs), steps, degree, _p(G), _p(h), _p(c)))
+        return G, h, float(c[0])
+
+    def set_precond_poly(self, alpha):
+        a = np.ascontiguousarray(alpha, dtype=np.complex128)
+        self._check(library().mlfma_set_precond_poly(self._h, a.size - 1, _p(a)))
+
+    def precond_apply(self, x: np.ndarray, kind: int = 2) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.empty_like(x)
+        self._check(library().mlfma_precond_apply(self._h, kind, x.size, _p(x), _p(y)))
+        return y
 
     def excitation(self, pol=0.0, theta=0.0, phi=0.0) -> np.ndarray:
         v = np.empty(self.n, dtype=np.complex128)
@@ -299,3 +344,76 @@ class Operator:
         vals = np.empty(n.value, dtype=np.complex128)
         self._check(library().mlfma_get_near_row(self._h, row, C.byref(n), _p(cols), _p(vals)))
         return cols, vals
+
+
+class Mlp:
+    """The coefficient network (mlfma_mlp): n_in -> width -> width -> n_out, tanh, FP32, on the GPU."""
+
+    def __init__(self, n_in: int = PRECOND_FEATURES, width: int = 256, n_out: int = 14, seed: int = 0):
+        lib = library()
+        h = C.c_void_p()
+        rc = lib.mlfma_mlp_create(n_in, width, n_out, seed, C.byref(h))
+        if rc != 0:
+            raise MlfmaError(rc, lib.mlfma_mlp_last_error(None).decode())
+        self._h = h
+        self.n_in, self.width, self.n_out = n_in, width, n_out
+        cnt = C.c_longlong()
+        lib.mlfma_mlp_num_params(self._h, C.byref(cnt))
+        self.n_params = int(cnt.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mlfma_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MlfmaError(rc, library().mlfma_mlp_last_error(self._h).decode())
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self._check(library().mlfma_mlp_get_params(self._h, _p(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self._check(library().mlfma_mlp_set_params(self._h, _p(p)))
+
+    def set_normalization(self, mean, std):
+        mean = np.ascontiguousarray(mean, dtype=np.float32)
+        std = np.ascontiguousarray(std, dtype=np.float32)
+        self._check(library().mlfma_mlp_set_normalization(self._h, _p(mean), _p(std)))
+
+    def predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        self._check(library().mlfma_mlp_predict(self._h, x.shape[0], _p(x), _p(y)))
+        return y
+
+    def loss_grad(self, x, G, h, c):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        G = np.ascontiguousarray(G, dtype=np.complex128)
+        h = np.ascontiguousarray(h, dtype=np.complex128)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        self._check(library().mlfma_mlp_loss_grad(self._h, x.shape[0], _p(x), _p(G), _p(h), _p(c), C.byref(loss),
+                                                  _p(g)))
+        return float(loss.value), g
+
+    def train(self, x, G, h, c, steps: int = 500, lr: float = 1e-3) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        G = np.ascontiguousarray(G, dtype=np.complex128)
+        h = np.ascontiguousarray(h, dtype=np.complex128)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        hist = np.zeros(max(steps, 1), dtype=np.float32)
+        self._check(library().mlfma_mlp_train(self._h, x.shape[0], _p(x), _p(G), _p(h), _p(c), steps, lr,
+                                              _p(hist)))
+        return hist[:steps]

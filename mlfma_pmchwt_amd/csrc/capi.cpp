@@ -227,6 +227,43 @@ int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int 
     return conv ? MLFMA_OK : MLFMA_ERR_NOT_CONVERGED;
 }
 
+int mlfma_precond_features(mlfma_handle* h, float* features) {
+    if (!h || !features) return MLFMA_ERR_ARG;
+    return guarded(h, [&] { h->eng->precond_features(features); });
+}
+
+int mlfma_precond_statistics(mlfma_handle* h, const double* rhs, int steps, int degree, double* G, double* hv,
+                             double* c) {
+    if (!h || !rhs || !G || !hv || !c) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        mlfma::Engine& e = *h->eng;
+        h->io.ensure((size_t)e.nunk);
+        MLFMA_HIP(hipMemcpy(h->io.x, rhs, (size_t)e.nunk * sizeof(double2), hipMemcpyHostToDevice));
+        e.precond_statistics(h->io.x, steps, degree, reinterpret_cast<double2*>(G), reinterpret_cast<double2*>(hv), c);
+    });
+}
+
+int mlfma_set_precond_poly(mlfma_handle* h, int degree, const double* alpha) {
+    if (!h || !alpha || degree < 0 || degree > 15) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        std::vector<mlfma::cplx> a(degree + 1);
+        for (int k = 0; k <= degree; ++k) a[k] = mlfma::cplx(alpha[2 * k], alpha[2 * k + 1]);
+        h->eng->set_poly(a);
+    });
+}
+
+int mlfma_precond_apply(mlfma_handle* h, int kind, int n, const double* x, double* y) {
+    if (!h || !x || !y) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        mlfma::Engine& e = *h->eng;
+        if (n != e.nunk || (kind != 1 && kind != 2)) throw mlfma::Error(MLFMA_ERR_ARG, "bad preconditioner apply");
+        h->io.ensure((size_t)n);
+        MLFMA_HIP(hipMemcpy(h->io.x, x, (size_t)n * sizeof(double2), hipMemcpyHostToDevice));
+        e.precond_apply_kind(kind, h->io.x, h->io.y);
+        MLFMA_HIP(hipMemcpy(y, h->io.y, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
 int mlfma_excitation(mlfma_handle* h, double pol, double th, double ph, double* v) {
     if (!h || !v) return MLFMA_ERR_ARG;
     return guarded(h, [&] {

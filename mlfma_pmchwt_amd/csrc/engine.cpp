@@ -11,6 +11,7 @@
 #include <type_traits>
 
 #include "kernels.hpp"
+#include "precond.hpp"
 #include "setup.hpp"
 
 namespace mlfma {
@@ -354,6 +355,9 @@ void Engine::build_near() {
         launch_narrow(st, (size_t)near_entries, d_Z_, d_Z32_);
         MLFMA_HIP(hipGetLastError());
     }
+    d_dinv_ = dalloc<double2>(nunk);
+    launch_near_diag_inv(st, nd_, d_Z_, d_dinv_);
+    MLFMA_HIP(hipGetLastError());
 }
 
 // Copy of one leaf-pair block, planes x (ns x nt column-major).
@@ -631,8 +635,12 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
         d_small_ = dalloc<double2>(V_cols_);
         d_rpart_ = dalloc<double>(kRedBlocks);
     }
-    const bool pc = sp.precond != 0;
-    if (pc) precond_setup();
+    if (sp.precond < 0 || sp.precond > 2) throw Error(MLFMA_ERR_ARG, "precond must be 0, 1 or 2");
+    stats_fallbacks_ = 0;
+    if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
+    bool pc = sp.precond != 0;
+    precond_kind_ = sp.precond;
+    if (sp.precond == 1) precond_setup();
     int matvecs = 0, fp64_matvecs = 0, total = 0;
     auto A = [&](const double2* in, double2* out) {
         if (pc) {
@@ -676,6 +684,7 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
     if (!converged && sp.solver == MLFMA_SOLVER_GMRES) {
         std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m), h1, h2;
         auto Hm = [&](int i, int j) -> cplx& { return H[i + (size_t)j * (m + 1)]; };
+        double err_cycle = err;
         for (int cyc = 1; cyc <= sp.itr_max && !converged; ++cyc) {
             launch_scale_copy(st, n, 1.0 / rnorm, d_r_, d_V_);
             std::fill(s.begin(), s.end(), cplx(0, 0));
@@ -738,12 +747,26 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
             } else {
                 launch_add_combination(st, n, mm, d_V_, (size_t)n, d_small_, d_x);
             }
+            const double err_prev = err_cycle;   // true residual at the start of this cycle
+            const double err_est = std::abs(s[std::min(kused, m)]) / bnorm;
             rnorm = true_residual();
             err = rnorm / bnorm;
             if (!std::isfinite(err)) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
+            if (P.verbose)
+                std::fprintf(stderr, "[mlfma] gmres cycle %d: %d iterations, residual estimate %.3e, true %.3e\n", cyc,
+                             total, err_est, err);
             if (err < sp.tol_rel || err < sp.tol_abs) converged = true;
             if (total >= sp.itr_max) break;
             if (err > 1000.0 * sp.tol_rel && cyc > 8) break;   // Code/GMRES.h:208-214
+            // fallback: a polynomial preconditioner that does not reduce the true
+            // residual by 10% over a whole cycle is dropped (plain GMRES from here)
+            if (!converged && precond_kind_ == 2 && err > 0.9 * err_prev) {
+                precond_kind_ = 0;
+                pc = false;
+                ++stats_fallbacks_;
+                if (P.verbose) std::fprintf(stderr, "[mlfma] polynomial preconditioner stalled: continuing without\n");
+            }
+            err_cycle = err;
         }
     } else if (!converged) {
         // Code/CGS.h:104-234 (bilinear inner products); columns of V as vectors
@@ -801,6 +824,8 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
     stats.iterations = total;
     stats.matvecs = matvecs;
     stats.fp64_matvecs = fp64_matvecs;
+    stats.precond_fallbacks = stats_fallbacks_;
+    stats_fallbacks_ = 0;
     stats.converged = converged ? 1 : 0;
     stats.rel_residual = err;
     stats.seconds = wall() - t0;
@@ -820,11 +845,16 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
     const int m = std::max(1, sp.restart);
     const double eps = 1e-16;
     if (sp.solver != MLFMA_SOLVER_GMRES) throw Error(MLFMA_ERR_ARG, "batched solve supports GMRES only");
-    if (sp.precond != 0) throw Error(MLFMA_ERR_ARG, "batched solve: preconditioner not supported");
+    if (sp.precond != 0 && sp.precond != 2)
+        throw Error(MLFMA_ERR_ARG, "batched solve: precond must be 0 or 2 (polynomial)");
+    if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
+    bool pc = sp.precond == 2;
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "batched solve: 1..256 right-hand sides");
     const size_t nR = (size_t)n * R;
+    stats_fallbacks_ = 0;
     if (bV_cols_ < m + 1 || bV_R_ != R) {
-        for (void* p : {(void*)d_bV_, (void*)d_bW_, (void*)d_bRr_, (void*)d_bpart_, (void*)d_bsmall_, (void*)d_bscale_})
+        for (void* p : {(void*)d_bV_, (void*)d_bW_, (void*)d_bRr_, (void*)d_bpart_, (void*)d_bsmall_, (void*)d_bscale_,
+                        (void*)d_bZ_})
             dfree(p);
         bV_cols_ = m + 1;
         bV_R_ = R;
@@ -834,6 +864,7 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         d_bpart_ = dalloc<double2>((size_t)(m + 1) * kRedBlocks * R);
         d_bsmall_ = dalloc<double2>((size_t)(m + 1) * R);
         d_bscale_ = dalloc<double>(R);
+        d_bZ_ = dalloc<double2>(nR);
     }
     int matvecs = 0, fp64_matvecs = 0;
     std::vector<int> iters(R, 0);
@@ -886,6 +917,7 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         done[r] = err[r] < sp.tol_rel || err[r] < sp.tol_abs;
     }
     auto all_done = [&] { return std::all_of(done.begin(), done.end(), [](char c) { return c != 0; }); };
+    std::vector<double> err_cycle = err;
     std::vector<cplx> H((size_t)R * (m + 1) * m), cs((size_t)R * m), sn((size_t)R * m), sv((size_t)R * (m + 1));
     auto Hm = [&](int r, int i, int j) -> cplx& { return H[((size_t)r * m + j) * (m + 1) + i]; };
     std::vector<cplx> h1, h2, yv;
@@ -907,7 +939,12 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         for (int k = 0; k < m && total < sp.itr_max; ++k) {
             if (std::none_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
             ++total;
-            apply_block(d_bV_ + (size_t)k * nR, d_bW_, R, apply_prec);
+            if (pc) {
+                poly_apply(d_bV_ + (size_t)k * nR, d_bZ_, R, apply_prec);
+                apply_block(d_bZ_, d_bW_, R, apply_prec);
+            } else {
+                apply_block(d_bV_ + (size_t)k * nR, d_bW_, R, apply_prec);
+            }
             matvecs += R;
             dots(k + 1, h1);
             put(h1);
@@ -975,14 +1012,31 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         }
         if (kmax > 0) {
             put(yv);
-            launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_X);
+            if (pc) {   // x += M^-1 (V y)
+                MLFMA_HIP(hipMemsetAsync(d_bW_, 0, nR * sizeof(double2), st));
+                launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_bW_);
+                poly_apply(d_bW_, d_bZ_, R, apply_prec);
+                launch_axpy(st, (int)nR, make_double2(1.0, 0.0), d_bZ_, d_X, d_X);
+            } else {
+                launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_X);
+            }
         }
+        const std::vector<double> err_prev = err_cycle;   // true residuals at the start of this cycle
         true_residual(rnorm);
+        bool progress = false;
         for (int r = 0; r < R; ++r) {
             err[r] = rnorm[r] / bnorm[r];
             if (!std::isfinite(err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
             if (err[r] < sp.tol_rel || err[r] < sp.tol_abs) done[r] = 1;
+            if (!done[r] && err[r] <= 0.9 * err_prev[r]) progress = true;
         }
+        // fallback as in solve(): drop a stalling polynomial preconditioner
+        if (pc && !all_done() && !progress) {
+            pc = false;
+            ++stats_fallbacks_;
+            if (P.verbose) std::fprintf(stderr, "[mlfma] polynomial preconditioner stalled: continuing without\n");
+        }
+        err_cycle = err;
         bool stalled = true;   // Code/GMRES.h:208-214, all right-hand sides
         for (int r = 0; r < R; ++r)
             if (!done[r] && !(err[r] > 1000.0 * sp.tol_rel)) stalled = false;
@@ -996,7 +1050,9 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         stats[r].converged = done[r] ? 1 : 0;
         stats[r].rel_residual = err[r];
         stats[r].seconds = secs;
+        stats[r].precond_fallbacks = stats_fallbacks_;
     }
+    stats_fallbacks_ = 0;
 }
 
 // Near-field block-diagonal preconditioner: exact inverse of every leaf's self
@@ -1065,8 +1121,177 @@ void Engine::precond_setup() {
     d_Doff_ = upload(off);
 }
 
+// Single application of preconditioner `kind` (1 block-diagonal, 2 polynomial)
+// with the apply precision, for inspection.
+void Engine::precond_apply_kind(int kind, const double2* x, double2* y) {
+    if (kind == 1) precond_setup();
+    const int keep = precond_kind_;
+    precond_kind_ = kind;
+    precond_apply(x, y);
+    precond_kind_ = keep;
+    MLFMA_HIP(hipStreamSynchronize(st));
+}
+
 void Engine::precond_apply(const double2* x, double2* y) {
+    if (precond_kind_ == 2) {
+        poly_apply(x, y, 1, apply_prec);
+        return;
+    }
     launch_block_diag(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_Doff_, d_Dinv_, max_leaf, x, y);
+}
+
+// ---------------------------------------------------------------- polynomial preconditioner
+// M^-1 r = sum_k alpha_k t_k, t_0 = D^-1 r, t_{k+1} = t_k - D^-1 Z_near t_k on an
+// RHS-fastest block of R vectors; the near products use the apply precision's
+// near-field storage.  No reference counterpart.
+void Engine::set_poly(const std::vector<cplx>& alpha) {
+    if (alpha.empty() || alpha.size() > 16) throw Error(MLFMA_ERR_ARG, "polynomial degree must be 0..15");
+    for (const cplx& a : alpha)
+        if (!std::isfinite(a.real()) || !std::isfinite(a.imag())) throw Error(MLFMA_ERR_NONFINITE, "non-finite coefficient");
+    poly_alpha_ = alpha;
+}
+
+void Engine::ensure_poly(int R) {
+    if (R <= poly_cap_) return;
+    dfree(d_pt_);
+    dfree(d_pu_);
+    d_pt_ = dalloc<double2>((size_t)nunk * R);
+    d_pu_ = dalloc<double2>((size_t)nunk * R);
+    poly_cap_ = R;
+}
+
+// Y = Z_near X for an RHS-fastest block (near storage of precision prec)
+void Engine::near_block(const double2* X, double2* Y, int R, int prec) {
+    const int cap = (ie == MLFMA_PMCHWT) ? 4 : 8;
+    for (int r0 = 0; r0 < R;) {
+        int rb = 1;
+        while (rb * 2 <= cap && r0 + rb * 2 <= R) rb *= 2;
+        if (prec == MLFMA_FP32 && d_Z32_) launch_near_b<float>(st, rb, R, r0, nd_, d_Z32_, X, Y);
+        else launch_near_b<double>(st, rb, R, r0, nd_, d_Z_, X, Y);
+        r0 += rb;
+    }
+}
+
+void Engine::poly_apply(const double2* X, double2* Y, int R, int prec) {
+    if (poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
+    ensure_poly(R);
+    launch_poly_init(st, nunk, R, d_dinv_, X, d2(poly_alpha_[0]), d_pt_, Y);
+    for (size_t k = 1; k < poly_alpha_.size(); ++k) {
+        near_block(d_pt_, d_pu_, R, prec);
+        launch_poly_step(st, nunk, R, d_dinv_, d_pu_, d2(poly_alpha_[k]), d_pt_, Y);
+    }
+}
+
+// Problem features for the coefficient network (kPrecondFeatures values):
+// spectral moments mu_k = z^H B^k z / n of B = D^-1 Z_near for k = 1..8 with a
+// seeded unit-modulus probe z (re, im), then eps_r (re, im), PMCHWT flag and
+// log10 of the unknown count.  FP64 near field.
+void Engine::precond_features(float* f) {
+    ensure_poly(1);
+    double2* z = d_pu_;
+    launch_probe(st, nunk, 12345u, d_pt_);
+    std::vector<double2> hz(nunk), hv(nunk);
+    MLFMA_HIP(hipMemcpyAsync(hz.data(), d_pt_, nunk * sizeof(double2), hipMemcpyDeviceToHost, st));
+    double2* v = dalloc<double2>(nunk);
+    MLFMA_HIP(hipMemcpyAsync(v, d_pt_, nunk * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    for (int k = 1; k <= 8; ++k) {
+        near_block(v, z, 1, MLFMA_FP64);
+        launch_diag_scale(st, nunk, 1, d_dinv_, z, v);
+        MLFMA_HIP(hipMemcpyAsync(hv.data(), v, nunk * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        cplx mu(0, 0);
+        for (int i = 0; i < nunk; ++i) mu += std::conj(cx(hz[i])) * cx(hv[i]);
+        mu /= (double)nunk;
+        f[k - 1] = (float)mu.real();
+        f[8 + k - 1] = (float)mu.imag();
+    }
+    dfree(v);
+    f[16] = (float)P.eps_r_re;
+    f[17] = (float)P.eps_r_im;
+    f[18] = ie == MLFMA_PMCHWT ? 1.0f : 0.0f;
+    f[19] = (float)std::log10((double)nunk);
+}
+
+// Least-squares statistics of the polynomial coefficients from the solver's
+// own Krylov pairs: `steps` Arnoldi steps of the unpreconditioned operator
+// (apply precision) from rhs give pairs (r_k = A v_k, e_k = v_k); with the basis
+// a_k = [t_0(r_k) .. t_d(r_k)], G = sum_k a_k^H a_k ((d+1) x (d+1), row-major),
+// h = sum_k a_k^H e_k and c = sum_k |e_k|^2, so that
+// sum_k |sum_j alpha_j t_j(r_k) - e_k|^2 = alpha^H G alpha - 2 Re alpha^H h + c.
+void Engine::precond_statistics(const double2* d_b, int steps, int degree, double2* G, double2* h, double* c) {
+    if (steps < 1 || steps > 256 || degree < 0 || degree > 15) throw Error(MLFMA_ERR_ARG, "bad steps / degree");
+    const int n = nunk, m = degree + 1;
+    const size_t nS = (size_t)n * steps;
+    double2* V = dalloc<double2>((size_t)n * (steps + 1));   // column-major Arnoldi basis
+    double2* W = dalloc<double2>(nS);                        // A v_k, column-major
+    double2* part = dalloc<double2>((size_t)(steps + 1) * kRedBlocks);
+    double2* small = dalloc<double2>(std::max(steps + 1, m * steps));
+    auto nrm = [&](const double2* x) {
+        launch_norm2_partial(st, n, x, d_rpart_ ? d_rpart_ : (double*)part, kRedBlocks);
+        std::vector<double> hp(kRedBlocks);
+        MLFMA_HIP(hipMemcpyAsync(hp.data(), d_rpart_ ? d_rpart_ : (double*)part, kRedBlocks * sizeof(double),
+                                 hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        double s2 = 0;
+        for (double x2 : hp) s2 += x2;
+        return std::sqrt(s2);
+    };
+    const double bn = nrm(d_b);
+    if (!(bn > 0)) throw Error(MLFMA_ERR_ARG, "zero right-hand side");
+    launch_scale_copy(st, n, 1.0 / bn, d_b, V);
+    int kept = 0;
+    for (int k = 0; k < steps; ++k) {
+        double2* w = W + (size_t)k * n;
+        apply_one(V + (size_t)k * n, w, apply_prec);
+        ++kept;
+        // CGS2 into the next basis vector
+        double2* nx = V + (size_t)(k + 1) * n;
+        MLFMA_HIP(hipMemcpyAsync(nx, w, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        for (int pass = 0; pass < 2; ++pass) {
+            launch_multidot(st, n, k + 1, V, (size_t)n, nx, part, kRedBlocks);
+            launch_reduce_partials(st, k + 1, kRedBlocks, part, small);
+            launch_sub_combination(st, n, k + 1, V, (size_t)n, small, nx);
+        }
+        const double hn = nrm(nx);
+        if (!(hn > 1e-14)) break;
+        launch_scale_copy(st, n, 1.0 / hn, nx, nx);
+    }
+    // RHS-fastest blocks of the pairs, then the basis t_j of every pair
+    const int R = kept;
+    ensure_poly(R);
+    double2* E = dalloc<double2>((size_t)n * R);
+    double2* T = dalloc<double2>((size_t)n * R * m);
+    launch_transpose(st, R, n, V, E);
+    launch_transpose(st, R, n, W, d_pt_);
+    launch_diag_scale(st, n, R, d_dinv_, d_pt_, T);
+    for (int j = 1; j < m; ++j) {
+        double2* Tj = T + (size_t)j * n * R;
+        MLFMA_HIP(hipMemcpyAsync(Tj, Tj - (size_t)n * R, (size_t)n * R * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        near_block(Tj, d_pu_, R, MLFMA_FP64);
+        launch_poly_step(st, n, R, d_dinv_, d_pu_, make_double2(0, 0), Tj, nullptr);
+    }
+    double2* bpart = dalloc<double2>((size_t)m * kRedBlocks * R);
+    double2* bsmall = dalloc<double2>((size_t)m * R);
+    std::vector<double2> hs((size_t)m * R);
+    auto block_dots = [&](const double2* w, std::vector<cplx>& out) {   // out[j] = sum_r <T_j, w>_r
+        launch_multidot_b(st, n, R, m, T, (size_t)n * R, w, bpart, kRedBlocks);
+        launch_reduce_partials_b(st, m, R, kRedBlocks, bpart, bsmall);
+        MLFMA_HIP(hipMemcpyAsync(hs.data(), bsmall, hs.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        out.assign(m, cplx(0, 0));
+        for (int j = 0; j < m; ++j)
+            for (int r = 0; r < R; ++r) out[j] += cx(hs[(size_t)j * R + r]);
+    };
+    std::vector<cplx> col;
+    for (int l = 0; l < m; ++l) {
+        block_dots(T + (size_t)l * n * R, col);
+        for (int j = 0; j < m; ++j) G[(size_t)j * m + l] = d2(col[j]);
+    }
+    block_dots(E, col);
+    for (int j = 0; j < m; ++j) h[j] = d2(col[j]);
+    *c = (double)R;   // unit basis vectors
+    for (void* q : {(void*)V, (void*)W, (void*)part, (void*)small, (void*)E, (void*)T, (void*)bpart, (void*)bsmall})
+        dfree(q);
 }
 
 void Engine::excitation(double pol, double th, double ph, double2* d_v) {

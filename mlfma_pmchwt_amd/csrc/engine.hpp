@@ -77,6 +77,12 @@ public:
     void solve_device(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, int R,
                       mlfma_solve_stats* stats);
     void ensure_block(int R);
+    // polynomial near-field preconditioner (precond = 2)
+    void set_poly(const std::vector<cplx>& alpha);
+    void poly_apply(const double2* X, double2* Y, int R, int prec);
+    void precond_features(float* f);
+    void precond_apply_kind(int kind, const double2* x, double2* y);
+    void precond_statistics(const double2* d_b, int steps, int degree, double2* G, double2* h, double* c);
     void excitation(double pol, double th, double ph, double2* d_v);
     void rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out);
 
@@ -138,6 +144,8 @@ private:
     std::vector<double2> near_block(int p);   // planes x (ns x nt column-major) of pair p
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
+    void ensure_poly(int R);
+    void near_block(const double2* X, double2* Y, int R, int prec);
     double norm(const double2* v, int n);
 
     std::vector<void*> allocs_;
@@ -173,11 +181,19 @@ private:
     double2* d_bpart_ = nullptr;
     double2* d_bsmall_ = nullptr;
     double* d_bscale_ = nullptr;
+    double2* d_bZ_ = nullptr;
     int bV_cols_ = 0, bV_R_ = 0;
+    int stats_fallbacks_ = 0;
     // profiling events: per wave 3 groups x 2, near 2, far 2
     bool prof_on_ = false;
     hipEvent_t ev_[16] = {};
     // block-diagonal preconditioner
+    double2* d_dinv_ = nullptr;   // 1 / diagonal of the operator, caller order
+    std::vector<cplx> poly_alpha_;
+    double2* d_pt_ = nullptr;
+    double2* d_pu_ = nullptr;
+    int poly_cap_ = 0;
+    int precond_kind_ = 0;
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };
