@@ -478,15 +478,17 @@ void Engine::collect_profile(int count) {
 
 // One pass of the matvec for right-hand sides r0 .. r0 + rb - 1 of RHS-fastest
 // vectors with stride ldr (single vector: ldr = 1, r0 = 0, rb = 1).
-void Engine::apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec) {
+void Engine::apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec, bool with_near) {
     const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
     for (int w = 0; w < nw; ++w) {
         if (f32) far_pass<float>(w, x, ldr, r0, rb);
         else far_pass<double>(w, x, ldr, r0, rb);
     }
     mark(12);
-    if (prec == MLFMA_FP32 && d_Z32_) launch_near_b<float>(st, rb, ldr, r0, nd_, d_Z32_, x, y);
-    else launch_near_b<double>(st, rb, ldr, r0, nd_, d_Z_, x, y);
+    if (with_near) {
+        if (prec == MLFMA_FP32 && d_Z32_) launch_near_b<float>(st, rb, ldr, r0, nd_, d_Z32_, x, y);
+        else launch_near_b<double>(st, rb, ldr, r0, nd_, d_Z_, x, y);
+    }
     mark(13);
     mark(14);
     if (f32) disaggregate<float>(y, ldr, r0, rb);
@@ -494,15 +496,29 @@ void Engine::apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, i
     mark(15);
 }
 
-void Engine::apply_one(const double2* x, double2* y, int prec) { apply_pass(x, y, 1, 0, 1, prec); }
+void Engine::apply_one(const double2* x, double2* y, int prec) { apply_pass(x, y, 1, 0, 1, prec, true); }
 
 // Y = A X for R right-hand sides, RHS-fastest (element (u, r) at u * R + r).
+// With the FP32 near field the near product of all R runs first on the
+// matrix cores; the far field follows in passes of rb right-hand sides.
 void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     const int cap = (prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32) ? rb_max<float>() : rb_max<double>();
+    const bool mfma_near = prec == MLFMA_FP32 && d_Z32_ && R >= 2 && near_mfma_;
+    if (mfma_near) {
+        if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
+        launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
+        if (prof_on_) {
+            MLFMA_HIP(hipEventRecord(ev_[17], st));
+            MLFMA_HIP(hipEventSynchronize(ev_[17]));
+            float ms = 0;
+            MLFMA_HIP(hipEventElapsedTime(&ms, ev_[16], ev_[17]));
+            prof_ms[3] += ms;
+        }
+    }
     for (int r0 = 0; r0 < R;) {
         int rb = 1;
         while (rb * 2 <= cap && r0 + rb * 2 <= R) rb *= 2;
-        apply_pass(X, Y, R, r0, rb, prec);
+        apply_pass(X, Y, R, r0, rb, prec, !mfma_near);
         if (prof_on_) collect_profile(rb);
         r0 += rb;
     }
@@ -1162,6 +1178,10 @@ void Engine::ensure_poly(int R) {
 
 // Y = Z_near X for an RHS-fastest block (near storage of precision prec)
 void Engine::near_block(const double2* X, double2* Y, int R, int prec) {
+    if (prec == MLFMA_FP32 && d_Z32_ && R >= 2 && near_mfma_) {
+        launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
+        return;
+    }
     const int cap = (ie == MLFMA_PMCHWT) ? 4 : 8;
     for (int r0 = 0; r0 < R;) {
         int rb = 1;

@@ -136,7 +136,7 @@ private:
     void far_pass(int w, const double2* x, int ldr, int r0, int rb);
     template <typename R>
     void disaggregate(double2* y, int ldr, int r0, int rb);
-    void apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec);
+    void apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec, bool with_near);
     void mark(int slot);
     void collect_profile(int count);
     void apply_one(const double2* x, double2* y, int prec);
@@ -186,7 +186,7 @@ private:
     int stats_fallbacks_ = 0;
     // profiling events: per wave 3 groups x 2, near 2, far 2
     bool prof_on_ = false;
-    hipEvent_t ev_[16] = {};
+    hipEvent_t ev_[18] = {};   // 16..17: the all-RHS near product of a batched matvec
     // block-diagonal preconditioner
     double2* d_dinv_ = nullptr;   // 1 / diagonal of the operator, caller order
     std::vector<cplx> poly_alpha_;
@@ -194,6 +194,7 @@ private:
     double2* d_pu_ = nullptr;
     int poly_cap_ = 0;
     int precond_kind_ = 0;
+    bool near_mfma_ = true;   // batched FP32 near product on the matrix cores
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };

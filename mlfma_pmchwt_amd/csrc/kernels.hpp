@@ -87,6 +87,11 @@ template <typename S>
 void launch_near_b(hipStream_t st, int rb, int ldr, int r0, const NearDev& nd, const cplx_t<S>* Z, const double2* x,
                    double2* y);
 
+// Near product of R right-hand sides at once on the matrix cores (FP32 near
+// entries, FP32 accumulation); vectors RHS-fastest with stride ldr from r0.
+void launch_near_mfma(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const float2* Z, const double2* x,
+                      double2* y);
+
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
 

@@ -91,3 +91,22 @@ def test_batched_limits(sphere_small):
     B = np.ones((op.n, 2), dtype=np.complex128)
     with pytest.raises(ValueError):
         op.solve_batch(B[:1])
+
+
+@pytest.mark.parametrize("ie", [api.CFIE, api.PMCHWT])
+@pytest.mark.parametrize("R", [2, 21])
+@pytest.mark.parametrize("freq", [12, 20])
+def test_batched_mfma_near_large_leaves(ie, R, freq):
+    """Dense meshes on a small sphere: leaves of 37-148 (freq 12) and about 400
+    (freq 20) RWGs exercise the matrix-core near product's multi-tile and
+    multi-pass (more than 256 rows) paths (FP32 near)."""
+    mesh = geometry.geodesic_sphere(freq, 0.5)
+    op = api.Operator(mesh, integral_equation=ie, eps_r_re=4.0, max_rhs=32, far_precision=api.FP32,
+                      near_precision=api.FP32)
+    X = np.stack([seeded_x(op.n, 300 + j) for j in range(R)], axis=1)
+    op.set_apply_precision(api.FP32)
+    Y = op.apply(X)
+    for j in range(R):
+        y1 = op.apply(np.ascontiguousarray(X[:, j]))
+        assert relerr(Y[:, j], y1) <= 2e-6
+    op.close()
