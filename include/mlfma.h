@@ -111,6 +111,22 @@ int mlfma_apply_device(mlfma_handle* h, int n, int nrhs, const void* x_dev, void
  * the reference's operator is FP64 only. */
 int mlfma_set_apply_precision(mlfma_handle* h, int precision);
 
+/* Persistent batched matvec: with on = 1, the kernel sequence of a batched
+ * matvec (mlfma_apply_device / mlfma_apply_batch with nrhs > 1, and every
+ * Arnoldi step of mlfma_solve_batch, preconditioner included) is recorded
+ * once per (right-hand-side count, precision) as a hipGraph and replayed.
+ * Changing coefficients, buffers or tuning drops the recordings.  No reference
+ * counterpart. */
+int mlfma_set_graphs(mlfma_handle* h, int on);
+
+/* Launch-configuration autotuner: times the batched matvec of nrhs right-hand
+ * sides in `precision` for each candidate (right-hand sides per far-field
+ * pass; near product on the matrix cores or on the vector units) and keeps
+ * the fastest for this handle.  out4 = {rhs per pass, matrix-core near (1/0),
+ * ms per batched matvec of the choice, ms of the slowest candidate}.  No
+ * reference counterpart. */
+int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
+
 /* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)
  * and MCGS::CGS::solve (Code/CGS.h:72-234) as driven by MLFMM::matrix_solver
  * (Code/MLFMM.cpp:370-397): x holds the initial guess on entry and the

@@ -69,6 +69,8 @@ def library() -> C.CDLL:
         lib.mlfma_apply_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_apply_device.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_set_apply_precision.argtypes = [vp, C.c_int]
+        lib.mlfma_set_graphs.argtypes = [vp, C.c_int]
+        lib.mlfma_autotune.argtypes = [vp, C.c_int, C.c_int, vp]
         lib.mlfma_precond_features.argtypes = [vp, vp]
         lib.mlfma_precond_statistics.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_set_precond_poly.argtypes = [vp, C.c_int, vp]
@@ -171,6 +173,16 @@ class Operator:
 
     def apply_device(self, x_ptr: int, y_ptr: int, nrhs: int = 1):
         self._check(library().mlfma_apply_device(self._h, self.n, nrhs, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
+
+    def set_graphs(self, on: bool = True):
+        """Record batched matvecs as hipGraphs and replay them."""
+        self._check(library().mlfma_set_graphs(self._h, int(on)))
+
+    def autotune(self, nrhs: int, precision: int = FP32) -> dict:
+        out = np.zeros(4)
+        self._check(library().mlfma_autotune(self._h, nrhs, precision, _p(out)))
+        return {"rhs_per_pass": int(out[0]), "matrix_core_near": bool(out[1]), "ms": float(out[2]),
+                "ms_slowest": float(out[3])}
 
     def set_apply_precision(self, precision: int):
         """FP64 or FP32 far field for apply* and the Krylov iterations of solve."""

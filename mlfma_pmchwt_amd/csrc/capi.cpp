@@ -157,6 +157,20 @@ int mlfma_apply_device(mlfma_handle* h, int n, int nrhs, const void* x_dev, void
     });
 }
 
+int mlfma_set_graphs(mlfma_handle* h, int on) {
+    if (!h) return MLFMA_ERR_ARG;
+    return guarded(h, [&] { h->eng->set_graphs(on != 0); });
+}
+
+int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4) {
+    if (!h || !out4) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        if (precision != MLFMA_FP64 && precision != MLFMA_FP32) throw mlfma::Error(MLFMA_ERR_ARG, "bad precision");
+        h->eng->autotune(nrhs, precision, out4);
+        MLFMA_HIP(hipStreamSynchronize(h->eng->st));
+    });
+}
+
 int mlfma_set_apply_precision(mlfma_handle* h, int precision) {
     if (!h) return MLFMA_ERR_ARG;
     return guarded(h, [&] { h->eng->set_apply_precision(precision); });

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -67,6 +68,7 @@ Engine::Engine(const mlfma_params& p) : P(p) {
 
 Engine::~Engine() {
     if (st) hipStreamSynchronize(st);
+    for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : ev_)
         if (e) hipEventDestroy(e);
     for (void* a : allocs_) hipFree(a);
@@ -502,7 +504,9 @@ void Engine::apply_one(const double2* x, double2* y, int prec) { apply_pass(x, y
 // With the FP32 near field the near product of all R runs first on the
 // matrix cores; the far field follows in passes of rb right-hand sides.
 void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
-    const int cap = (prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32) ? rb_max<float>() : rb_max<double>();
+    const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
+    int cap = f32 ? rb_max<float>() : rb_max<double>();
+    if (rb_tuned_[f32 ? 1 : 0] > 0) cap = std::min(cap, rb_tuned_[f32 ? 1 : 0]);
     const bool mfma_near = prec == MLFMA_FP32 && d_Z32_ && R >= 2 && near_mfma_;
     if (mfma_near) {
         if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
@@ -532,7 +536,11 @@ void Engine::apply_device(const double2* x, double2* y, int nrhs) {
     } else {
         ensure_block(nrhs);
         launch_transpose(st, nrhs, nunk, x, d_bx_);
-        apply_block(d_bx_, d_by_, nrhs, apply_prec);
+        if (use_graphs_ && !prof_on_) {
+            run_graph(GraphKey{0, nrhs, apply_prec}, [&] { apply_block(d_bx_, d_by_, nrhs, apply_prec); });
+        } else {
+            apply_block(d_bx_, d_by_, nrhs, apply_prec);
+        }
         launch_transpose(st, nunk, nrhs, d_by_, y);
     }
     MLFMA_HIP(hipGetLastError());
@@ -541,11 +549,99 @@ void Engine::apply_device(const double2* x, double2* y, int nrhs) {
 void Engine::ensure_block(int R) {
     if (R > 256) throw Error(MLFMA_ERR_ARG, "at most 256 right-hand sides per call");
     if (R <= block_cap_) return;
+    clear_graphs();
     dfree(d_bx_);
     dfree(d_by_);
     d_bx_ = dalloc<double2>((size_t)nunk * R);
     d_by_ = dalloc<double2>((size_t)nunk * R);
     block_cap_ = R;
+}
+
+// ---------------------------------------------------------------- hipGraph persistence
+// A graph per (kind, R, precision) records the kernel sequence of a batched
+// matvec (kind 0) or of a preconditioned batched matvec (kind 1) between fixed
+// buffers, and is replayed on the handle's stream.  Nothing in the recorded
+// sequence allocates, synchronises or clears memory with hipMemsetAsync.
+void Engine::run_graph(const GraphKey& key, const std::function<void()>& body) {
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        hipGraph_t g = nullptr;
+        MLFMA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipStreamEndCapture(st, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        MLFMA_HIP(hipStreamEndCapture(st, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        MLFMA_HIP(e);
+        it = graphs_.emplace(key, ex).first;
+    }
+    MLFMA_HIP(hipGraphLaunch(it->second, st));
+}
+
+void Engine::clear_graphs() {
+    if (!graphs_.empty()) hipStreamSynchronize(st);
+    for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+}
+
+void Engine::set_graphs(bool on) {
+    if (!on) clear_graphs();
+    use_graphs_ = on;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Times the batched matvec of R right-hand sides in precision prec for every
+// candidate launch configuration (right-hand sides per far-field pass; near
+// product on the matrix cores or on the vector units when the near field has
+// an FP32 copy) and keeps the fastest for this handle.  out[0] = chosen rb,
+// out[1] = matrix-core near (1/0), out[2] = milliseconds per matvec of the
+// choice, out[3] = milliseconds of the slowest candidate.
+void Engine::autotune(int R, int prec, double* out) {
+    if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "autotune: 1..256 right-hand sides");
+    const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
+    if (prec == MLFMA_FP32 && !f32 && !d_Z32_) throw Error(MLFMA_ERR_ARG, "no FP32 tables built");
+    ensure_block(R);
+    launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
+    const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
+    std::vector<int> mfma_opts = {0};
+    if (prec == MLFMA_FP32 && d_Z32_ && R >= 2) mfma_opts = {1, 0};
+    hipEvent_t a, b;
+    MLFMA_HIP(hipEventCreate(&a));
+    MLFMA_HIP(hipEventCreate(&b));
+    const bool keep_mfma = near_mfma_;
+    const int keep_rb = rb_tuned_[f32 ? 1 : 0];
+    double best = 1e30, worst = 0;
+    int best_rb = rbmax, best_m = keep_mfma ? 1 : 0;
+    for (int m : mfma_opts) {
+        for (int rb = 1; rb <= rbmax; rb *= 2) {
+            near_mfma_ = m != 0;
+            rb_tuned_[f32 ? 1 : 0] = rb;
+            apply_block(d_bx_, d_by_, R, prec);   // warm-up
+            MLFMA_HIP(hipEventRecord(a, st));
+            for (int k = 0; k < 3; ++k) apply_block(d_bx_, d_by_, R, prec);
+            MLFMA_HIP(hipEventRecord(b, st));
+            MLFMA_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            MLFMA_HIP(hipEventElapsedTime(&ms, a, b));
+            const double per = ms / 3.0;
+            if (P.verbose) std::fprintf(stderr, "[mlfma] autotune R=%d prec=%d rb=%d mfma=%d: %.3f ms\n", R, prec, rb, m, per);
+            if (per < best) best = per, best_rb = rb, best_m = m;
+            worst = std::max(worst, per);
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    (void)keep_rb;
+    rb_tuned_[f32 ? 1 : 0] = best_rb;
+    near_mfma_ = best_m != 0;
+    clear_graphs();   // recorded sequences follow the tuned configuration
+    if (out) out[0] = best_rb, out[1] = best_m, out[2] = best, out[3] = worst;
 }
 
 void Engine::dfree(void* p) {
@@ -870,7 +966,7 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
     stats_fallbacks_ = 0;
     if (bV_cols_ < m + 1 || bV_R_ != R) {
         for (void* p : {(void*)d_bV_, (void*)d_bW_, (void*)d_bRr_, (void*)d_bpart_, (void*)d_bsmall_, (void*)d_bscale_,
-                        (void*)d_bZ_})
+                        (void*)d_bZ_, (void*)d_bG_})
             dfree(p);
         bV_cols_ = m + 1;
         bV_R_ = R;
@@ -881,6 +977,8 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         d_bsmall_ = dalloc<double2>((size_t)(m + 1) * R);
         d_bscale_ = dalloc<double>(R);
         d_bZ_ = dalloc<double2>(nR);
+        d_bG_ = dalloc<double2>(nR);
+        clear_graphs();   // recorded sequences refer to the freed block buffers
     }
     int matvecs = 0, fp64_matvecs = 0;
     std::vector<int> iters(R, 0);
@@ -955,7 +1053,20 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         for (int k = 0; k < m && total < sp.itr_max; ++k) {
             if (std::none_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
             ++total;
-            if (pc) {
+            if (use_graphs_ && !prof_on_) {
+                // fixed input buffer, then one replay of the recorded (preconditioned) matvec
+                MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bV_ + (size_t)k * nR, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
+                const int prec = apply_prec;
+                if (pc) {
+                    ensure_poly(R);
+                    run_graph(GraphKey{1, R, prec}, [&] {
+                        poly_apply(d_bG_, d_bZ_, R, prec);
+                        apply_block(d_bZ_, d_bW_, R, prec);
+                    });
+                } else {
+                    run_graph(GraphKey{2, R, prec}, [&] { apply_block(d_bG_, d_bW_, R, prec); });
+                }
+            } else if (pc) {
                 poly_apply(d_bV_ + (size_t)k * nR, d_bZ_, R, apply_prec);
                 apply_block(d_bZ_, d_bW_, R, apply_prec);
             } else {
@@ -1165,10 +1276,12 @@ void Engine::set_poly(const std::vector<cplx>& alpha) {
     for (const cplx& a : alpha)
         if (!std::isfinite(a.real()) || !std::isfinite(a.imag())) throw Error(MLFMA_ERR_NONFINITE, "non-finite coefficient");
     poly_alpha_ = alpha;
+    clear_graphs();   // the coefficients are kernel arguments of recorded sequences
 }
 
 void Engine::ensure_poly(int R) {
     if (R <= poly_cap_) return;
+    clear_graphs();
     dfree(d_pt_);
     dfree(d_pu_);
     d_pt_ = dalloc<double2>((size_t)nunk * R);

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -77,6 +79,9 @@ public:
     void solve_device(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, int R,
                       mlfma_solve_stats* stats);
     void ensure_block(int R);
+    // hipGraph replay of batched matvecs; launch-configuration autotuner
+    void set_graphs(bool on);
+    void autotune(int R, int prec, double* out);
     // polynomial near-field preconditioner (precond = 2)
     void set_poly(const std::vector<cplx>& alpha);
     void poly_apply(const double2* X, double2* Y, int R, int prec);
@@ -123,6 +128,17 @@ private:
     template <typename T>
     T* upload(const std::vector<T>& v);
     void dfree(void* p);
+    struct GraphKey {
+        int kind, R, prec;
+        bool operator<(const GraphKey& o) const {
+            return kind != o.kind ? kind < o.kind : (R != o.R ? R < o.R : prec < o.prec);
+        }
+    };
+    void run_graph(const GraphKey& key, const std::function<void()>& body);
+    void clear_graphs();
+    std::map<GraphKey, hipGraphExec_t> graphs_;
+    bool use_graphs_ = false;
+    int rb_tuned_[2] = {0, 0};   // tuned right-hand sides per far pass (FP64, FP32); 0: widest
     void build_device_tree();
     void build_wave(int w);
     void build_near();
@@ -182,6 +198,7 @@ private:
     double2* d_bsmall_ = nullptr;
     double* d_bscale_ = nullptr;
     double2* d_bZ_ = nullptr;
+    double2* d_bG_ = nullptr;   // fixed input of recorded Arnoldi matvecs
     int bV_cols_ = 0, bV_R_ = 0;
     int stats_fallbacks_ = 0;
     // profiling events: per wave 3 groups x 2, near 2, far 2

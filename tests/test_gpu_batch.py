@@ -110,3 +110,58 @@ def test_batched_mfma_near_large_leaves(ie, R, freq):
         y1 = op.apply(np.ascontiguousarray(X[:, j]))
         assert relerr(Y[:, j], y1) <= 2e-6
     op.close()
+
+
+def test_graph_replays_match_eager():
+    """Recorded batched matvecs replayed several times, with eager work and
+    synchronisations in between, give the eager result every time."""
+    mesh = geometry.icosphere(2, 1.5)
+    op = api.Operator(mesh, integral_equation=api.PMCHWT, eps_r_re=2.5, eps_r_im=-0.1, max_rhs=16,
+                      far_precision=api.FP32, near_precision=api.FP32)
+    for prec in (api.FP64, api.FP32):
+        op.set_apply_precision(prec)
+        X = np.stack([seeded_x(op.n, 500 + j) for j in range(11)], axis=1)
+        op.set_graphs(False)
+        Y0 = op.apply(X)
+        op.set_graphs(True)
+        for rep in range(4):
+            Y = op.apply(X)
+            assert np.array_equal(Y, Y0), f"replay {rep} differs"
+            op.apply(np.ascontiguousarray(X[:, rep]))   # eager single matvec between replays
+            op.solve(X[:, 0], tol_rel=1e-1, restart=5, itr_max=5)
+        op.set_graphs(False)
+    op.close()
+
+
+def test_graph_batched_solve_matches_eager():
+    _, op, orc = pair("pmchwt_1lvl")
+    op.set_apply_precision(api.FP32)
+    B = np.stack([op.excitation(0.0, th, 0.0) for th in (0.0, 50.0, 110.0)], axis=1)
+    G, h, c = op.precond_statistics(B[:, 0], steps=30, degree=6)
+    from mlfma_pmchwt_amd import learned
+    op.set_precond_poly(learned.fit_coefficients(G, h, c))
+    for pc in (0, api.PRECOND_POLY):
+        op.set_graphs(False)
+        X0, s0 = op.solve_batch(B, tol_rel=1e-3, restart=30, precond=pc)
+        op.set_graphs(True)
+        X1, s1 = op.solve_batch(B, tol_rel=1e-3, restart=30, precond=pc)
+        X2, s2 = op.solve_batch(B, tol_rel=1e-3, restart=30, precond=pc)
+        op.set_graphs(False)
+        assert [s["iterations"] for s in s1] == [s["iterations"] for s in s0] == [s["iterations"] for s in s2]
+        assert np.array_equal(X1, X0) and np.array_equal(X2, X0)
+    op.set_apply_precision(api.FP64)
+
+
+def test_autotune_keeps_results():
+    mesh = geometry.icosphere(2, 1.5)
+    op = api.Operator(mesh, integral_equation=api.CFIE, max_rhs=16, far_precision=api.FP32,
+                      near_precision=api.FP32)
+    X = np.stack([seeded_x(op.n, 700 + j) for j in range(16)], axis=1)
+    op.set_apply_precision(api.FP32)
+    Y0 = op.apply(X)
+    t = op.autotune(16, api.FP32)
+    print(f"[autotune] {t}")
+    assert t["rhs_per_pass"] in (1, 2, 4, 8) and 0 < t["ms"] <= t["ms_slowest"]
+    Y1 = op.apply(X)
+    assert relerr(Y1, Y0) <= 2e-6   # another launch configuration: FP32 rounding only
+    op.close()
