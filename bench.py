@@ -138,6 +138,11 @@ def headline(args):
     alpha = lp.apply_to(op)
     predict_s = time.perf_counter() - t0
     op.set_apply_precision(api.FP32)
+    t0 = time.perf_counter()
+    tune = op.autotune(R, api.FP32)
+    tune_s = time.perf_counter() - t0
+    op.set_graphs(True)
+    log(f"autotune {tune_s:.2f} s: {tune}")
     X, st = op.solve_batch(B, tol_rel=1e-3, restart=30, itr_max=3000, precond=api.PRECOND_POLY)
     mixed_total = time.perf_counter() - t_all
     its = [s["iterations"] for s in st]
@@ -183,6 +188,7 @@ def headline(args):
                                       "5-level tree, 128 incidences (64 theta x V/H), GMRES(30) to 1e-3",
         "n_unknowns": n, "rhs": R,
         "mixed": {"total_s": round(mixed_total, 3), "setup_s": round(setup_mixed, 3), "predict_s": round(predict_s, 3),
+                  "autotune_s": round(tune_s, 3), "tuned": tune, "graphs": True,
                   "solve_s": round(st[0]["seconds"], 3), "iterations_min": min(its), "iterations_max": max(its),
                   "converged": conv, "precond_fallbacks": st[0]["precond_fallbacks"],
                   "max_true_rel_residual": max(s["rel_residual"] for s in st),

@@ -119,12 +119,13 @@ int mlfma_set_apply_precision(mlfma_handle* h, int precision);
  * counterpart. */
 int mlfma_set_graphs(mlfma_handle* h, int on);
 
-/* Launch-configuration autotuner: times the batched matvec of nrhs right-hand
- * sides in `precision` for each candidate (right-hand sides per far-field
- * pass; near product on the matrix cores or on the vector units) and keeps
- * the fastest for this handle.  out4 = {rhs per pass, matrix-core near (1/0),
- * ms per batched matvec of the choice, ms of the slowest candidate}.  No
- * reference counterpart. */
+/* Launch-configuration autotuner: times the batched matvec of min(nrhs, 32)
+ * right-hand sides in `precision` for each candidate (right-hand sides per
+ * far-field pass; near product and disaggregation on the matrix cores or on
+ * the vector units) and keeps the fastest for this handle.  out4 = {rhs per
+ * pass, matrix-core use (bit 0 near, bit 1 disaggregation), ms per batched
+ * matvec of the choice, ms of the slowest candidate}.  No reference
+ * counterpart. */
 int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
 
 /* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)

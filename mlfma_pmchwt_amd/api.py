@@ -181,8 +181,8 @@ class Operator:
     def autotune(self, nrhs: int, precision: int = FP32) -> dict:
         out = np.zeros(4)
         self._check(library().mlfma_autotune(self._h, nrhs, precision, _p(out)))
-        return {"rhs_per_pass": int(out[0]), "matrix_core_near": bool(out[1]), "ms": float(out[2]),
-                "ms_slowest": float(out[3])}
+        return {"rhs_per_pass": int(out[0]), "matrix_core_near": bool(int(out[1]) & 1),
+                "matrix_core_disaggregation": bool(int(out[1]) & 2), "ms": float(out[2]), "ms_slowest": float(out[3])}
 
     def set_apply_precision(self, precision: int):
         """FP64 or FP32 far field for apply* and the Krylov iterations of solve."""

@@ -211,7 +211,11 @@ void Engine::build_wave(int w) {
         D.t_kp[c] = upload(it.t_kp);
     }
     build_far<double>(w, T);
-    if (P.far_precision == MLFMA_FP32) build_far<float>(w, T);
+    if (P.far_precision == MLFMA_FP32) {
+        build_far<float>(w, T);
+        if (rb_max<float>() > 1)
+            D.E = dalloc<float2>((size_t)nleaf * rb_max<float>() * (ie == MLFMA_PMCHWT ? 2 : 1) * 3 * D.K[L]);
+    }
 }
 
 template <typename R>
@@ -417,6 +421,28 @@ void Engine::far_pass(int w, const double2* x, int ldr, int r0, int rb) {
 template <typename R>
 void Engine::disaggregate(double2* y, int ldr, int r0, int rb) {
     const int L = tree.max_level;
+    if constexpr (std::is_same<R, float>::value) {
+        if (rb >= 2 && disagg_mfma_ && wd_[0].E) {
+            const bool pm = ie == MLFMA_PMCHWT;
+            if (!pm) {
+                const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));   // Code/MLFMM.cpp:226
+                const double2 z = make_double2(0, 0);
+                launch_beff(st, nleaf, rb, wd_[0].K[L], false, wd_[0].f32.Bm[L], wd_[0].f32.kp0, d2(c), z, z, z,
+                            wd_[0].E);
+                launch_disagg_mfma(st, nleaf, N, ldr, r0, rb, d_leaf_order_, d_leaf_ptr_, d_perm_, false,
+                                   wd_[0].K[L], wd_[0].f32.Vf, wd_[0].E, 0, nullptr, nullptr, y);
+            } else {
+                const cplx c1 = med.k1 * med.k1 / (16.0 * kPi * kPi), c2 = med.k2 * med.k2 / (16.0 * kPi * kPi);
+                launch_beff(st, nleaf, rb, wd_[0].K[L], true, wd_[0].f32.Bm[L], wd_[0].f32.kp0, d2(c1), d2(c1),
+                            d2(c1), d2(c1), wd_[0].E);
+                launch_beff(st, nleaf, rb, wd_[1].K[L], true, wd_[1].f32.Bm[L], wd_[1].f32.kp0, d2(c2 * med.etai),
+                            d2(c2), d2(c2), d2(c2 / med.etai), wd_[1].E);
+                launch_disagg_mfma(st, nleaf, N, ldr, r0, rb, d_leaf_order_, d_leaf_ptr_, d_perm_, true,
+                                   wd_[0].K[L], wd_[0].f32.Vf, wd_[0].E, wd_[1].K[L], wd_[1].f32.Vf, wd_[1].E, y);
+            }
+            return;
+        }
+    }
     if (ie != MLFMA_PMCHWT) {
         // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
         const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));
@@ -596,40 +622,48 @@ void Engine::set_graphs(bool on) {
 }
 
 // ---------------------------------------------------------------- autotuner
-// Times the batched matvec of R right-hand sides in precision prec for every
+// Times the batched matvec of R (at most 32) right-hand sides in precision prec for every
 // candidate launch configuration (right-hand sides per far-field pass; near
 // product on the matrix cores or on the vector units when the near field has
-// an FP32 copy) and keeps the fastest for this handle.  out[0] = chosen rb,
-// out[1] = matrix-core near (1/0), out[2] = milliseconds per matvec of the
-// choice, out[3] = milliseconds of the slowest candidate.
+// an FP32 copy; disaggregation on the matrix cores or on the vector units in
+// FP32) and keeps the fastest for this handle.  out[0] = chosen rb, out[1] =
+// matrix-core use (bit 0 near, bit 1 disaggregation), out[2] = milliseconds
+// per matvec of the choice, out[3] = milliseconds of the slowest candidate.
 void Engine::autotune(int R, int prec, double* out) {
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "autotune: 1..256 right-hand sides");
+    R = std::min(R, 32);   // the per-pass choices do not depend on larger batches; keeps tuning short
     const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
     if (prec == MLFMA_FP32 && !f32 && !d_Z32_) throw Error(MLFMA_ERR_ARG, "no FP32 tables built");
     ensure_block(R);
     launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
     const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
-    std::vector<int> mfma_opts = {0};
-    if (prec == MLFMA_FP32 && d_Z32_ && R >= 2) mfma_opts = {1, 0};
+    std::vector<int> mfma_opts = {0};   // bit 0: near, bit 1: disaggregation on the matrix cores
+    if (R >= 2) {
+        const bool nm = prec == MLFMA_FP32 && d_Z32_, dm = f32 && wd_[0].E;
+        mfma_opts.clear();
+        for (int m = 3; m >= 0; --m)
+            if ((!(m & 1) || nm) && (!(m & 2) || dm)) mfma_opts.push_back(m);
+    }
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
     MLFMA_HIP(hipEventCreate(&b));
     const bool keep_mfma = near_mfma_;
     const int keep_rb = rb_tuned_[f32 ? 1 : 0];
     double best = 1e30, worst = 0;
-    int best_rb = rbmax, best_m = keep_mfma ? 1 : 0;
+    int best_rb = rbmax, best_m = (keep_mfma ? 1 : 0) | (disagg_mfma_ ? 2 : 0);
     for (int m : mfma_opts) {
         for (int rb = 1; rb <= rbmax; rb *= 2) {
-            near_mfma_ = m != 0;
+            near_mfma_ = (m & 1) != 0;
+            disagg_mfma_ = (m & 2) != 0;
             rb_tuned_[f32 ? 1 : 0] = rb;
             apply_block(d_bx_, d_by_, R, prec);   // warm-up
             MLFMA_HIP(hipEventRecord(a, st));
-            for (int k = 0; k < 3; ++k) apply_block(d_bx_, d_by_, R, prec);
+            for (int k = 0; k < 2; ++k) apply_block(d_bx_, d_by_, R, prec);
             MLFMA_HIP(hipEventRecord(b, st));
             MLFMA_HIP(hipEventSynchronize(b));
             float ms = 0;
             MLFMA_HIP(hipEventElapsedTime(&ms, a, b));
-            const double per = ms / 3.0;
+            const double per = ms / 2.0;
             if (P.verbose) std::fprintf(stderr, "[mlfma] autotune R=%d prec=%d rb=%d mfma=%d: %.3f ms\n", R, prec, rb, m, per);
             if (per < best) best = per, best_rb = rb, best_m = m;
             worst = std::max(worst, per);
@@ -639,7 +673,8 @@ void Engine::autotune(int R, int prec, double* out) {
     hipEventDestroy(b);
     (void)keep_rb;
     rb_tuned_[f32 ? 1 : 0] = best_rb;
-    near_mfma_ = best_m != 0;
+    near_mfma_ = (best_m & 1) != 0;
+    disagg_mfma_ = (best_m & 2) != 0;
     clear_graphs();   // recorded sequences follow the tuned configuration
     if (out) out[0] = best_rb, out[1] = best_m, out[2] = best, out[3] = worst;
 }

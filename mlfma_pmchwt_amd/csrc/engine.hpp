@@ -55,6 +55,7 @@ struct WaveDev {
     std::vector<int*> tap, t_ptr, t_kp;  // per child tree level
     FarDev<double> f64;                  // always built (true residuals, stage API)
     FarDev<float> f32;                   // built when far_precision = MLFMA_FP32
+    float2* E = nullptr;                 // FP32 batched disaggregation: effective receive-side matrices
 };
 
 class Engine {
@@ -211,7 +212,8 @@ private:
     double2* d_pu_ = nullptr;
     int poly_cap_ = 0;
     int precond_kind_ = 0;
-    bool near_mfma_ = true;   // batched FP32 near product on the matrix cores
+    bool near_mfma_ = true;     // batched FP32 near product on the matrix cores
+    bool disagg_mfma_ = true;   // batched FP32 disaggregation on the matrix cores
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };
