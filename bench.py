@@ -218,6 +218,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--batch", type=int, default=32, help="right-hand sides of the batched-matvec measurement")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed region, copy 1 GiB device to device once (counter calibration run)")
     ap.add_argument("--workload", default="pec_cfie", choices=["pec_cfie", "pmchwt_batch"],
                     help="pec_cfie: matvec/s on the 300k-RWG CFIE sphere (default); pmchwt_batch: end-to-end solve "
                          "of 128 incidences on the 150k-unknown PMCHWT sphere, mixed + batched + learned vs plain FP64")
@@ -291,6 +293,15 @@ def main():
     err32 = float(torch.linalg.vector_norm(ys[api.FP32] - ys[api.FP64]) / torch.linalg.vector_norm(ys[api.FP64]))
     log(f"FP32 mode (far field and near storage): {ms32:.3f} ms/matvec, rel. difference to FP64 {err32:.2e}")
 
+    if args.calibrate_copy:   # a plain copy kernel of known size for the HBM-counter calibration
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.5, out=dst)   # elementwise copy kernel: 1 GiB read, 1 GiB written
+        torch.cuda.synchronize()
+        log("calibration copy kernel: 1 GiB read, 1 GiB written")
+        del src, dst
+
     batched = None
     if args.batch > 1:
         Xb = torch.randn(args.batch, op.n, dtype=torch.complex128, device="cuda", generator=gen)
@@ -319,6 +330,12 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": nbytes, "ms_per_launch": round(per_launch[dom], 4)}
+    traffic_file = ROOT / "profiles" / f"{dom}_traffic_pec_cfie_300k_rwg.json"
+    if traffic_file.exists() and args.frequency == 100 and args.radius == 4.0:
+        measured = json.loads(traffic_file.read_text())
+        roofline["traffic"] = measured["traffic_bytes_per_launch"]
+        roofline["traffic_source"] = f"profiles/{traffic_file.name} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, " \
+                                     f"scaled by a 1 GiB copy)"
     stage_ms = {g: round(v["ms"] / max(prof["matvecs"], 1), 4) for g, v in groups.items()}
     stage_ms32 = {g: round(v["ms"] / max(prof32["matvecs"], 1), 4) for g, v in prof32["groups"].items()}
 
