@@ -122,11 +122,22 @@ int mlfma_set_graphs(mlfma_handle* h, int on);
 /* Launch-configuration autotuner: times the batched matvec of min(nrhs, 32)
  * right-hand sides in `precision` for each candidate (right-hand sides per
  * far-field pass; near product and disaggregation on the matrix cores or on
- * the vector units) and keeps the fastest for this handle.  out4 = {rhs per
- * pass, matrix-core use (bit 0 near, bit 1 disaggregation), ms per batched
- * matvec of the choice, ms of the slowest candidate}.  No reference
+ * the vector units; translation per box or by sibling clusters) and keeps the
+ * fastest for this handle.  out4 = {rhs per pass, choice bits (bit 0 near on
+ * the matrix cores, bit 1 disaggregation on the matrix cores, bit 2 cluster
+ * translation), ms per batched matvec of the choice, ms of the slowest
+ * candidate}.  No reference
  * counterpart. */
 int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
+
+/* Sets the launch configuration directly (e.g. one that mlfma_autotune chose
+ * in an earlier process): rhs_per_pass in {1, 2, 4, 8} (clamped to what the
+ * far-field precision allows; 0 keeps the default) for `precision`, and the
+ * choice bits of mlfma_autotune's out4[1]: bit 0 near field on the matrix
+ * cores, bit 1 disaggregation on the matrix cores, bit 2 translation by
+ * sibling clusters (up to 4 children of one parent share each source box,
+ * read once).  No reference counterpart. */
+int mlfma_set_tuning(mlfma_handle* h, int precision, int rhs_per_pass, int bits);
 
 /* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)
  * and MCGS::CGS::solve (Code/CGS.h:72-234) as driven by MLFMM::matrix_solver

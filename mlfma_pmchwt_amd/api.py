@@ -71,6 +71,7 @@ def library() -> C.CDLL:
         lib.mlfma_set_apply_precision.argtypes = [vp, C.c_int]
         lib.mlfma_set_graphs.argtypes = [vp, C.c_int]
         lib.mlfma_autotune.argtypes = [vp, C.c_int, C.c_int, vp]
+        lib.mlfma_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         lib.mlfma_precond_features.argtypes = [vp, vp]
         lib.mlfma_precond_statistics.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_set_precond_poly.argtypes = [vp, C.c_int, vp]
@@ -182,7 +183,13 @@ class Operator:
         out = np.zeros(4)
         self._check(library().mlfma_autotune(self._h, nrhs, precision, _p(out)))
         return {"rhs_per_pass": int(out[0]), "matrix_core_near": bool(int(out[1]) & 1),
-                "matrix_core_disaggregation": bool(int(out[1]) & 2), "ms": float(out[2]), "ms_slowest": float(out[3])}
+                "matrix_core_disaggregation": bool(int(out[1]) & 2), "cluster_translation": bool(int(out[1]) & 4),
+                "bits": int(out[1]), "ms": float(out[2]), "ms_slowest": float(out[3])}
+
+    def set_tuning(self, precision: int = FP32, rhs_per_pass: int = 0, bits: int = 7):
+        """Launch configuration without timing: bit 0 matrix-core near field,
+        bit 1 matrix-core disaggregation, bit 2 sibling-cluster translation."""
+        self._check(library().mlfma_set_tuning(self._h, int(precision), int(rhs_per_pass), int(bits)))
 
     def set_apply_precision(self, precision: int):
         """FP64 or FP32 far field for apply* and the Krylov iterations of solve."""

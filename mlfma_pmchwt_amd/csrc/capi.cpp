@@ -171,6 +171,17 @@ int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4) {
     });
 }
 
+int mlfma_set_tuning(mlfma_handle* h, int precision, int rhs_per_pass, int bits) {
+    if (!h) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        if (precision != MLFMA_FP64 && precision != MLFMA_FP32) throw mlfma::Error(MLFMA_ERR_ARG, "bad precision");
+        if (!(rhs_per_pass == 0 || rhs_per_pass == 1 || rhs_per_pass == 2 || rhs_per_pass == 4 || rhs_per_pass == 8) ||
+            bits < 0 || bits > 7)
+            throw mlfma::Error(MLFMA_ERR_ARG, "rhs_per_pass must be 0, 1, 2, 4 or 8 and bits in 0..7");
+        h->eng->set_tuning(precision, rhs_per_pass, bits);
+    });
+}
+
 int mlfma_set_apply_precision(mlfma_handle* h, int precision) {
     if (!h) return MLFMA_ERR_ARG;
     return guarded(h, [&] { h->eng->set_apply_precision(precision); });

@@ -192,6 +192,36 @@ void Engine::build_device_tree() {
         std::vector<int> oct(v.n);
         for (int i = 0; i < v.n; ++i) oct[i] = (int)(v.code[i] & 7);
         d.oct = upload(oct);
+        // sibling clusters: boxes are Morton-sorted, so the children of one
+        // parent are consecutive
+        std::vector<int> cl_child, cl_ptr(1, 0), cl_src, cl_vec;
+        for (int b0 = 0; b0 < v.n;) {
+            int b1 = b0;
+            while (b1 < v.n && v.parent[b1] == v.parent[b0] && b1 - b0 < 4) ++b1;
+            std::vector<int> srcs;
+            for (int b = b0; b < b1; ++b)
+                for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) srcs.push_back(v.dr_idx[j]);
+            std::sort(srcs.begin(), srcs.end());
+            srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
+            const size_t base = cl_src.size();
+            cl_src.insert(cl_src.end(), srcs.begin(), srcs.end());
+            cl_vec.resize(cl_src.size() * 4, -1);
+            for (int b = b0; b < b1; ++b)
+                for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) {
+                    const size_t u = base + (std::lower_bound(srcs.begin(), srcs.end(), v.dr_idx[j]) - srcs.begin());
+                    cl_vec[u * 4 + (b - b0)] = v.dr_vec[j];
+                }
+            for (int k = 0; k < 4; ++k) cl_child.push_back(b0 + k < b1 ? b0 + k : -1);
+            cl_ptr.push_back((int)cl_src.size());
+            d.cl_pairs += v.dr_ptr[b1] - v.dr_ptr[b0];
+            b0 = b1;
+        }
+        d.ncl = (int)cl_ptr.size() - 1;
+        d.cl_entries = (long long)cl_src.size();
+        d.cl_child = upload(cl_child);
+        d.cl_ptr = upload(cl_ptr);
+        d.cl_src = upload(cl_src);
+        d.cl_vec = upload(cl_vec);
     }
 }
 
@@ -408,13 +438,26 @@ void Engine::far_pass(int w, const double2* x, int ldr, int r0, int rb) {
                       D.tap[l + 1], F.tw[l + 1], F.sh_up[l + 1], F.Sm[l + 1], F.Sm[l]);
     mark(6 * w + 3);
     mark(6 * w + 4);
-    for (int l = 2; l <= L; ++l) {
-        const bool up = l > 2;
-        launch_m2l_l2l<R>(st, nc, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec, F.TFw[l],
-                          F.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct, D.t_ptr[l], D.t_kp[l], F.t_w[l],
-                          F.sh_dn[l], up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
-    }
+    for (int l = 2; l <= L; ++l) translate<R>(w, l, nc);
     mark(6 * w + 5);
+}
+
+// m2l + l2l of one level: sibling-cluster kernel (sources staged once per
+// cluster) or the per-box gather kernel.
+template <typename R>
+void Engine::translate(int w, int l, int nc) {
+    WaveDev& D = wd_[w];
+    FarDev<R>& F = far<R>(w);
+    const bool up = l > 2;
+    const LevelDev& v = lv_[l];
+    if (m2l_cluster_)
+        launch_m2l_cl<R>(st, nc, v.ncl, D.K[l], v.cl_child, v.cl_ptr, v.cl_src, v.cl_vec, F.TFw[l], F.Sm[l],
+                         up ? D.K[l - 1] : 0, v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l],
+                         up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
+    else
+        launch_m2l_l2l<R>(st, nc, v.n, D.K[l], v.dr_ptr, v.dr_idx, v.dr_vec, F.TFw[l], F.Sm[l], up ? D.K[l - 1] : 0,
+                          v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l], up ? F.Bm[l - 1] : nullptr,
+                          F.Bm[l]);
 }
 
 // y += far field (finest Bm of every wavenumber), same right-hand-side range
@@ -626,9 +669,10 @@ void Engine::set_graphs(bool on) {
 // candidate launch configuration (right-hand sides per far-field pass; near
 // product on the matrix cores or on the vector units when the near field has
 // an FP32 copy; disaggregation on the matrix cores or on the vector units in
-// FP32) and keeps the fastest for this handle.  out[0] = chosen rb, out[1] =
-// matrix-core use (bit 0 near, bit 1 disaggregation), out[2] = milliseconds
-// per matvec of the choice, out[3] = milliseconds of the slowest candidate.
+// FP32; translation by sibling clusters or per box) and keeps the fastest for
+// this handle.  out[0] = chosen rb, out[1] = choice bits (bit 0 matrix-core
+// near, bit 1 matrix-core disaggregation, bit 2 cluster translation), out[2] =
+// milliseconds per matvec of the choice, out[3] = of the slowest candidate.
 void Engine::autotune(int R, int prec, double* out) {
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "autotune: 1..256 right-hand sides");
     R = std::min(R, 32);   // the per-pass choices do not depend on larger batches; keeps tuning short
@@ -637,24 +681,24 @@ void Engine::autotune(int R, int prec, double* out) {
     ensure_block(R);
     launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
     const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
-    std::vector<int> mfma_opts = {0};   // bit 0: near, bit 1: disaggregation on the matrix cores
-    if (R >= 2) {
-        const bool nm = prec == MLFMA_FP32 && d_Z32_, dm = f32 && wd_[0].E;
-        mfma_opts.clear();
-        for (int m = 3; m >= 0; --m)
-            if ((!(m & 1) || nm) && (!(m & 2) || dm)) mfma_opts.push_back(m);
-    }
+    // bit 0: near on the matrix cores, bit 1: disaggregation on the matrix cores,
+    // bit 2: sibling-cluster translation
+    std::vector<int> mfma_opts;
+    const bool nm = R >= 2 && prec == MLFMA_FP32 && d_Z32_, dm = R >= 2 && f32 && wd_[0].E;
+    for (int m = 7; m >= 0; --m)
+        if ((!(m & 1) || nm) && (!(m & 2) || dm)) mfma_opts.push_back(m);
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
     MLFMA_HIP(hipEventCreate(&b));
     const bool keep_mfma = near_mfma_;
     const int keep_rb = rb_tuned_[f32 ? 1 : 0];
     double best = 1e30, worst = 0;
-    int best_rb = rbmax, best_m = (keep_mfma ? 1 : 0) | (disagg_mfma_ ? 2 : 0);
+    int best_rb = rbmax, best_m = (keep_mfma ? 1 : 0) | (disagg_mfma_ ? 2 : 0) | (m2l_cluster_ ? 4 : 0);
     for (int m : mfma_opts) {
         for (int rb = 1; rb <= rbmax; rb *= 2) {
             near_mfma_ = (m & 1) != 0;
             disagg_mfma_ = (m & 2) != 0;
+            m2l_cluster_ = (m & 4) != 0;
             rb_tuned_[f32 ? 1 : 0] = rb;
             apply_block(d_bx_, d_by_, R, prec);   // warm-up
             MLFMA_HIP(hipEventRecord(a, st));
@@ -675,8 +719,19 @@ void Engine::autotune(int R, int prec, double* out) {
     rb_tuned_[f32 ? 1 : 0] = best_rb;
     near_mfma_ = (best_m & 1) != 0;
     disagg_mfma_ = (best_m & 2) != 0;
+    m2l_cluster_ = (best_m & 4) != 0;
     clear_graphs();   // recorded sequences follow the tuned configuration
     if (out) out[0] = best_rb, out[1] = best_m, out[2] = best, out[3] = worst;
+}
+
+void Engine::set_tuning(int prec, int rb, int bits) {
+    const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
+    if (prec == MLFMA_FP32 && !f32 && !d_Z32_) throw Error(MLFMA_ERR_ARG, "no FP32 tables built");
+    rb_tuned_[f32 ? 1 : 0] = rb;
+    near_mfma_ = (bits & 1) != 0;
+    disagg_mfma_ = (bits & 2) != 0;
+    m2l_cluster_ = (bits & 4) != 0;
+    clear_graphs();
 }
 
 void Engine::dfree(void* p) {
@@ -725,17 +780,8 @@ void Engine::stage(int which, const double2* dx, double2* dy) {
             }
             break;
         case MLFMA_STAGE_M2L_L2L:
-            for (int w = 0; w < nw; ++w) {
-                WaveDev& D = wd_[w];
-                const FarDev<double>& F = D.f64;
-                for (int l = 2; l <= L; ++l) {
-                    const bool up = l > 2;
-                    launch_m2l_l2l<double>(st, ncur, lv_[l].n, D.K[l], lv_[l].dr_ptr, lv_[l].dr_idx, lv_[l].dr_vec,
-                                           F.TFw[l], F.Sm[l], up ? D.K[l - 1] : 0, lv_[l].parent, lv_[l].oct,
-                                           D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l], up ? F.Bm[l - 1] : nullptr,
-                                           F.Bm[l]);
-                }
-            }
+            for (int w = 0; w < nw; ++w)
+                for (int l = 2; l <= L; ++l) translate<double>(w, l, ncur);
             break;
         case MLFMA_STAGE_FAR:
             MLFMA_HIP(hipMemsetAsync(dy, 0, sizeof(double2) * nunk, st));

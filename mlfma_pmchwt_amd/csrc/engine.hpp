@@ -37,6 +37,15 @@ struct LevelDev {
     int* son_idx = nullptr;
     int* parent = nullptr;
     int* oct = nullptr;   // Morton octant of each node inside its parent
+    // sibling clusters for the translation: up to 4 children of one parent,
+    // the union of their distant relatives (ascending), and per union entry the
+    // translation vector of each member (-1: not its distant relative)
+    int ncl = 0;
+    int* cl_child = nullptr;   // [ncl][4], -1 for an empty slot
+    int* cl_ptr = nullptr;     // [ncl + 1] into cl_src
+    int* cl_src = nullptr;
+    int* cl_vec = nullptr;     // [entries][4]
+    long long cl_pairs = 0, cl_entries = 0;   // sum of DR list lengths, sum of union sizes
 };
 
 // Far-field tables and workspaces of one wavenumber in real type R.
@@ -83,6 +92,7 @@ public:
     // hipGraph replay of batched matvecs; launch-configuration autotuner
     void set_graphs(bool on);
     void autotune(int R, int prec, double* out);
+    void set_tuning(int prec, int rb, int bits);
     // polynomial near-field preconditioner (precond = 2)
     void set_poly(const std::vector<cplx>& alpha);
     void poly_apply(const double2* X, double2* Y, int R, int prec);
@@ -153,6 +163,8 @@ private:
     void far_pass(int w, const double2* x, int ldr, int r0, int rb);
     template <typename R>
     void disaggregate(double2* y, int ldr, int r0, int rb);
+    template <typename R>
+    void translate(int w, int l, int nc);
     void apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec, bool with_near);
     void mark(int slot);
     void collect_profile(int count);
@@ -214,6 +226,7 @@ private:
     int precond_kind_ = 0;
     bool near_mfma_ = true;     // batched FP32 near product on the matrix cores
     bool disagg_mfma_ = true;   // batched FP32 disaggregation on the matrix cores
+    bool m2l_cluster_ = true;   // translation by sibling clusters
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };

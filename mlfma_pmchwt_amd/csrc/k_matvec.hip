@@ -258,6 +258,108 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
 #undef M2L
 }
 
+// Sibling-cluster form of the same product: a thread owns one sample q, CPT of
+// the nc x 3 workspace columns and all (up to 4) children of one parent.  It
+// walks the union of their distant-relative lists in ascending box order,
+// reading each source box's CPT values once and applying them to every member
+// that has the box in its list, with that member's translation vector (a
+// member's sum thus runs over its own list in ascending box order).  The
+// interpolation from the parent reads each parent value once for all members.
+// G = nc x 3 / CPT wavefronts per workgroup, one per column group; no LDS and
+// no barriers.
+template <typename R, int NC, int G>
+__global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, const int* __restrict__ cl_child,
+                                                   const int* __restrict__ cl_ptr, const int* __restrict__ cl_src,
+                                                   const int* __restrict__ cl_vec, const cplx_t<R>* __restrict__ TFw,
+                                                   const cplx_t<R>* __restrict__ Sm, int Kp,
+                                                   const int* __restrict__ parent, const int* __restrict__ oct,
+                                                   const int* __restrict__ t_ptr, const int* __restrict__ t_kp,
+                                                   const R* __restrict__ t_w, const cplx_t<R>* __restrict__ shdn,
+                                                   const cplx_t<R>* __restrict__ Bmp, cplx_t<R>* __restrict__ Bm) {
+    using C = cplx_t<R>;
+    constexpr int CPT = NC * 3 / G;
+    const int lin = xcd_remap(blockIdx.x, ncl * ktiles);
+    const int cl = lin / ktiles, kt = lin % ktiles;
+    const int c0 = (threadIdx.x >> 6) * CPT;
+    const int q = kt * 64 + (threadIdx.x & 63);
+    if (q >= K) return;
+    int child[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) child[k] = cl_child[cl * 4 + k];
+    C acc[4][CPT];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) acc[k][j] = C{0, 0};
+    const int u1 = cl_ptr[cl + 1];
+    for (int u = cl_ptr[cl]; u < u1; ++u) {
+        const C* S = Sm + ((size_t)cl_src[u] * NC * 3 + c0) * K + q;
+        C sv[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) sv[j] = S[(size_t)j * K];
+        const int4 vv = *reinterpret_cast<const int4*>(cl_vec + (size_t)u * 4);
+        const int vk[4] = {vv.x, vv.y, vv.z, vv.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (vk[k] >= 0) {
+                const C T = TFw[(size_t)vk[k] * K + q];
+#pragma unroll
+                for (int j = 0; j < CPT; ++j) cfma(acc[k][j], T, sv[j]);
+            }
+    }
+    if (Bmp != nullptr) {
+        const C* P = Bmp + ((size_t)parent[child[0]] * NC * 3 + c0) * Kp;   // siblings share the parent
+        const C* sh[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sh[k] = shdn + (size_t)(child[k] >= 0 ? oct[child[k]] : 0) * Kp;
+        for (int j = t_ptr[q]; j < t_ptr[q + 1]; ++j) {
+            const int kp = t_kp[j];
+            const R w = t_w[j];
+            C pv[CPT];
+#pragma unroll
+            for (int jj = 0; jj < CPT; ++jj) pv[jj] = P[(size_t)jj * Kp + kp];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (child[k] >= 0) {
+                    C e = sh[k][kp];
+                    e.x *= w;
+                    e.y *= w;
+#pragma unroll
+                    for (int jj = 0; jj < CPT; ++jj) cfma(acc[k][jj], e, pv[jj]);
+                }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (child[k] >= 0) {
+            C* o = Bm + ((size_t)child[k] * NC * 3 + c0) * K + q;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) o[(size_t)j * K] = acc[k][j];
+        }
+}
+
+template <typename R>
+void launch_m2l_cl(hipStream_t st, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+                   const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
+                   const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
+                   const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
+    if (ncl == 0) return;
+    const int kt = (K + 63) / 64;
+    // columns per thread: 3 in FP64, 6 in FP32 (3 when there are only 3)
+    constexpr bool f64 = sizeof(R) == 8;
+#define M2LC(NC)                                                                                                      \
+    {                                                                                                                 \
+        constexpr int G = f64 ? NC : (NC == 1 ? 1 : NC / 2);                                                          \
+        k_m2l_cl<R, NC, G><<<ncl * kt, 64 * G, 0, st>>>(ncl, K, kt, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp, \
+                                                        parent, octant, t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm); \
+    }
+    if (ncur == 1) M2LC(1)
+    else if (ncur == 2) M2LC(2)
+    else if (ncur == 4) M2LC(4)
+    else M2LC(8)
+#undef M2LC
+}
+
 // ---------------------------------------------------------------- disaggregation
 // Code/MLFMM.cpp:211-227: w_m += (k eta/4pi)(k/4pi) sum_k Vfmj_m . Bm_leaf.
 // Each wavefront owns ROWS consecutive RWG rows (internal order) and streams
@@ -472,7 +574,10 @@ void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
     template void launch_disagg_pec_b<R>(hipStream_t, int, int, int, int, int, const int*, const int*,             \
                                          const cplx_t<R>*, const cplx_t<R>*, double2, double2*);                  \
     template void launch_disagg_pmchwt_b<R>(hipStream_t, int, int, int, const int*, const int*,                    \
-                                            const PmchwtFarArgs<R>&, double2*);
+                                            const PmchwtFarArgs<R>&, double2*);                                   \
+    template void launch_m2l_cl<R>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,      \
+                                   const cplx_t<R>*, const cplx_t<R>*, int, const int*, const int*, const int*,   \
+                                   const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);
 MLFMA_FAR_INSTANTIATE(double)
 MLFMA_FAR_INSTANTIATE(float)
 #undef MLFMA_FAR_INSTANTIATE

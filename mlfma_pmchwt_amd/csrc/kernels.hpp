@@ -44,6 +44,14 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
                     const int* octant, const int* t_ptr, const int* t_kp, const R* t_w, const cplx_t<R>* shift_down,
                     const cplx_t<R>* Bm_parent, cplx_t<R>* Bm);
 
+// Same product by sibling clusters: up to 4 children of one parent share every
+// source box of the union of their distant-relative lists, staged once in LDS.
+template <typename R>
+void launch_m2l_cl(hipStream_t st, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+                   const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
+                   const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
+                   const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm);
+
 // PEC: y[perm[i]] += c * sum_k Vf[i].B_leaf[k], leaf = leaf_of[i]
 template <typename R>
 void launch_disagg_pec(hipStream_t st, int N, int K, const int* leaf_of, const int* perm, const cplx_t<R>* Vf,

@@ -165,3 +165,27 @@ def test_autotune_keeps_results():
     Y1 = op.apply(X)
     assert relerr(Y1, Y0) <= 2e-6   # another launch configuration: FP32 rounding only
     op.close()
+
+
+@pytest.mark.parametrize("name", ["cfie_ml", "pmchwt_lossy_ml"])
+def test_cluster_translation_matches_per_box(name):
+    """Sibling-cluster translation reorders only the sum over distant relatives:
+    FP64 results agree with the per-box kernel to rounding, both precisions,
+    single and batched, and with the oracle at the FP64 bar."""
+    mesh, op, orc = pair(name)
+    X = np.stack([seeded_x(op.n, 900 + j) for j in range(5)], axis=1)
+    out = {}
+    for bits in (0, 4):
+        for prec in (api.FP64, api.FP32):
+            op.set_tuning(prec, 0, bits)
+            op.set_apply_precision(prec)
+            out[(bits, prec)] = (op.apply(X[:, 0]), op.apply(X))
+    op.set_apply_precision(api.FP64)
+    op.set_tuning(api.FP64, 0, 7)
+    op.set_tuning(api.FP32, 0, 7)
+    for k in range(2):
+        e64 = relerr(out[(4, api.FP64)][k], out[(0, api.FP64)][k])
+        e32 = relerr(out[(4, api.FP32)][k], out[(0, api.FP32)][k])
+        print(f"[cluster] {name} {'batched' if k else 'single'} fp64 {e64:.2e} fp32 {e32:.2e}")
+        assert e64 <= 1e-13 and e32 <= 2e-6
+    assert relerr(out[(4, api.FP64)][0], orc.apply(X[:, 0])) <= 1e-10
