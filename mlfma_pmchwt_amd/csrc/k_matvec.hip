@@ -128,16 +128,22 @@ void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const i
 
 // ---------------------------------------------------------------- interpolation
 // Code/ProcessMLFMM.h:61-107 (m2m): 3x3 Lagrange taps into the child grid, then
-// the octant phase shift, summed over the children of each parent.
+// the octant phase shift, summed over the children of each parent.  A thread
+// owns one parent sample and 3 of the nc x 3 workspace columns (G = nc
+// wavefronts per workgroup, one per column group); the q tiles of one parent
+// are consecutive after the XCD remap, so they share that XCD's L2 copy of the
+// children's workspaces.
 template <typename R, int NC>
-__global__ __launch_bounds__(256) void k_m2m(int Kp, int Kc, const int* __restrict__ son_ptr,
-                                             const int* __restrict__ son_idx, const int* __restrict__ oct_child,
-                                             const int* __restrict__ tap, const R* __restrict__ w,
-                                             const cplx_t<R>* __restrict__ shup, const cplx_t<R>* __restrict__ Smc,
-                                             cplx_t<R>* __restrict__ Smp) {
+__global__ __launch_bounds__(64 * NC) void k_m2m(int nparent, int qtiles, int Kp, int Kc,
+                                                 const int* __restrict__ son_ptr, const int* __restrict__ son_idx,
+                                                 const int* __restrict__ oct_child, const int* __restrict__ tap,
+                                                 const R* __restrict__ w, const cplx_t<R>* __restrict__ shup,
+                                                 const cplx_t<R>* __restrict__ Smc, cplx_t<R>* __restrict__ Smp) {
     using C = cplx_t<R>;
-    const int p = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lin = xcd_remap(blockIdx.x, nparent * qtiles);
+    const int p = lin / qtiles;
+    const int q = (lin % qtiles) * 64 + (threadIdx.x & 63);
+    const int c0 = (threadIdx.x >> 6) * 3;   // first workspace column of this thread
     if (q >= Kp) return;
     int t[9];
     R ww[9];
@@ -146,43 +152,40 @@ __global__ __launch_bounds__(256) void k_m2m(int Kp, int Kc, const int* __restri
         t[s] = tap[(size_t)s * Kp + q];
         ww[s] = w[(size_t)s * Kp + q];
     }
-    C acc[NC][3];
+    C acc[3];
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[c][d] = C{0, 0};
+    for (int d = 0; d < 3; ++d) acc[d] = C{0, 0};
     for (int j = son_ptr[p]; j < son_ptr[p + 1]; ++j) {
         const int c = son_idx[j];
         const C e = shup[(size_t)oct_child[c] * Kp + q];
-        const C* S = Smc + (size_t)c * NC * 3 * Kc;
+        const C* S = Smc + ((size_t)c * NC * 3 + c0) * Kc;
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
+        for (int d = 0; d < 3; ++d) {
+            const C* sd = S + (size_t)d * Kc;
+            C r = C{0, 0};
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const C* s = S + (size_t)(cc * 3 + d) * Kc;
-                C r = C{0, 0};
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    const C f = s[t[k]];
-                    r.x += f.x * ww[k];
-                    r.y += f.y * ww[k];
-                }
-                cfma(acc[cc][d], e, r);
+            for (int k = 0; k < 9; ++k) {
+                const C f = sd[t[k]];
+                r.x += f.x * ww[k];
+                r.y += f.y * ww[k];
             }
+            cfma(acc[d], e, r);
+        }
     }
-    C* o = Smp + (size_t)p * NC * 3 * Kp + q;
+    C* o = Smp + ((size_t)p * NC * 3 + c0) * Kp + q;
 #pragma unroll
-    for (int cc = 0; cc < NC; ++cc)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) o[(size_t)(cc * 3 + d) * Kp] = acc[cc][d];
+    for (int d = 0; d < 3; ++d) o[(size_t)d * Kp] = acc[d];
 }
 
 template <typename R>
 void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int* son_ptr, const int* son_idx,
                 const int* octant_child, const int* tap, const R* w, const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child,
                 cplx_t<R>* Sm_parent) {
-    dim3 grid((Kp + 255) / 256, nparent);
-#define M2M(NC) k_m2m<R, NC><<<grid, 256, 0, st>>>(Kp, Kc, son_ptr, son_idx, octant_child, tap, w, shift_up, Sm_child, Sm_parent)
+    if (nparent == 0) return;
+    const int qt = (Kp + 63) / 64;
+#define M2M(NC)                                                                                                       \
+    k_m2m<R, NC><<<nparent * qt, 64 * NC, 0, st>>>(nparent, qt, Kp, Kc, son_ptr, son_idx, octant_child, tap, w,      \
+                                                   shift_up, Sm_child, Sm_parent)
     if (ncur == 1) M2M(1);
     else if (ncur == 2) M2M(2);
     else if (ncur == 4) M2M(4);
