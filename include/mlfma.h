@@ -47,6 +47,7 @@ typedef struct mlfma_handle mlfma_handle;
  * operator). */
 #define MLFMA_FP64 0
 #define MLFMA_FP32 1
+#define MLFMA_BF16 2
 
 typedef struct mlfma_params {
     int integral_equation;  /* MLFMA_EFIE / MLFMA_CFIE / MLFMA_PMCHWT */
@@ -64,6 +65,10 @@ typedef struct mlfma_params {
     int max_rhs;            /* largest batch a single apply/solve will be given */
     int device;             /* HIP device ordinal */
     int verbose;            /* 1: setup timings on stderr */
+    int precond_precision;  /* near-field products of the polynomial preconditioner in FP32-mode
+                               solves: MLFMA_FP64 (same storage as the matvec's near field) or
+                               MLFMA_BF16 (a BF16 copy built at setup, matrix-core products with
+                               FP32 accumulation; residuals and the operator are unaffected) */
 } mlfma_params;
 
 /* Reference defaults: PMCHWT, 7 points, 299792458 Hz (lambda = 1 m), eps_r 1,
@@ -210,6 +215,12 @@ int mlfma_set_precond_poly(mlfma_handle* h, int degree, const double* alpha);
 
 /* y = M^-1 x for kind 1 (block-diagonal) or 2 (polynomial), host vectors. */
 int mlfma_precond_apply(mlfma_handle* h, int kind, int n, const double* x, double* y);
+
+/* The polynomial preconditioner (precond = 2) on nrhs <= 256 column-major
+ * vectors in one batched pass, in the apply precision (with
+ * precond_precision = MLFMA_BF16 and FP32 apply precision, its near products
+ * use the BF16 copy).  No reference counterpart. */
+int mlfma_precond_apply_batch(mlfma_handle* h, int n, int nrhs, const double* x, double* y);
 
 /* The coefficient network: n_in -> width -> width -> n_out, tanh hidden
  * layers, FP32, trained on the device.  Loss per sample s (features x_s,

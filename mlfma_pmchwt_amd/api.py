@@ -16,7 +16,7 @@ from .geometry import Mesh
 _LIB_PATH = Path(__file__).resolve().parent / "libmlfma.so"
 
 EFIE, CFIE, PMCHWT = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, BF16 = 0, 1, 2
 PRECOND_NONE, PRECOND_BLOCK_DIAG, PRECOND_POLY = 0, 1, 2
 PRECOND_FEATURES = 20
 GMRES, CGS = 0, 1
@@ -32,7 +32,7 @@ class Params(C.Structure):
                 ("mu_r_im", C.c_double), ("cfie_alpha", C.c_double), ("leaf_coeff", C.c_double),
                 ("far_precision", C.c_int), ("near_precision", C.c_int), ("tf_full_order", C.c_int),
                 ("max_rhs", C.c_int),
-                ("device", C.c_int), ("verbose", C.c_int)]
+                ("device", C.c_int), ("verbose", C.c_int), ("precond_precision", C.c_int)]
 
 
 class SolveParams(C.Structure):
@@ -76,6 +76,7 @@ def library() -> C.CDLL:
         lib.mlfma_precond_statistics.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_set_precond_poly.argtypes = [vp, C.c_int, vp]
         lib.mlfma_precond_apply.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_precond_apply_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]
         lib.mlfma_mlp_destroy.argtypes = [vp]
         lib.mlfma_mlp_last_error.argtypes = [vp]
@@ -247,6 +248,14 @@ class Operator:
         self._check(library().mlfma_set_precond_poly(self._h, a.size - 1, _p(a)))
 
     def precond_apply(self, x: np.ndarray, kind: int = 2) -> np.ndarray:
+        """M^-1 x; a 2-D x (n x nrhs) goes through the batched polynomial path (kind 2)."""
+        if np.ndim(x) == 2:
+            if kind != PRECOND_POLY:
+                raise ValueError("batched preconditioner apply is polynomial only")
+            xt = np.ascontiguousarray(np.asarray(x, dtype=np.complex128).T)
+            yt = np.empty_like(xt)
+            self._check(library().mlfma_precond_apply_batch(self._h, self.n, xt.shape[0], _p(xt), _p(yt)))
+            return yt.T
         x = np.ascontiguousarray(x, dtype=np.complex128)
         y = np.empty_like(x)
         self._check(library().mlfma_precond_apply(self._h, kind, x.size, _p(x), _p(y)))

@@ -66,6 +66,8 @@ int check_params(const mlfma_params* p, std::string& why) {
     if (!(p->leaf_coeff > 0)) return why = "leaf_coeff must be positive", MLFMA_ERR_ARG;
     if (p->far_precision != MLFMA_FP64 && p->far_precision != MLFMA_FP32) return why = "bad far_precision", MLFMA_ERR_ARG;
     if (p->near_precision != MLFMA_FP64 && p->near_precision != MLFMA_FP32) return why = "bad near_precision", MLFMA_ERR_ARG;
+    if (p->precond_precision != MLFMA_FP64 && p->precond_precision != MLFMA_BF16)
+        return why = "bad precond_precision", MLFMA_ERR_ARG;
     if (p->max_rhs < 1) return why = "max_rhs must be >= 1", MLFMA_ERR_ARG;
     const int g = p->gauss_points;
     if (g != 1 && g != 3 && g != 4 && g != 6 && g != 7 && g != 12 && g != 13 && g != 16)
@@ -88,6 +90,7 @@ void mlfma_params_default(mlfma_params* p) {
     p->leaf_coeff = 0.3;
     p->far_precision = MLFMA_FP64;
     p->near_precision = MLFMA_FP64;
+    p->precond_precision = MLFMA_FP64;
     p->tf_full_order = 0;
     p->max_rhs = 1;
     p->device = 0;
@@ -286,6 +289,19 @@ int mlfma_precond_apply(mlfma_handle* h, int kind, int n, const double* x, doubl
         MLFMA_HIP(hipMemcpy(h->io.x, x, (size_t)n * sizeof(double2), hipMemcpyHostToDevice));
         e.precond_apply_kind(kind, h->io.x, h->io.y);
         MLFMA_HIP(hipMemcpy(y, h->io.y, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int mlfma_precond_apply_batch(mlfma_handle* h, int n, int nrhs, const double* x, double* y) {
+    if (!h || !x || !y) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        mlfma::Engine& e = *h->eng;
+        if (n != e.nunk || nrhs < 1 || nrhs > 256) throw mlfma::Error(MLFMA_ERR_ARG, "bad preconditioner apply");
+        const size_t cnt = (size_t)n * nrhs;
+        h->io.ensure(cnt);
+        MLFMA_HIP(hipMemcpy(h->io.x, x, cnt * sizeof(double2), hipMemcpyHostToDevice));
+        e.poly_apply_device(h->io.x, h->io.y, nrhs);
+        MLFMA_HIP(hipMemcpy(y, h->io.y, cnt * sizeof(double2), hipMemcpyDeviceToHost));
     });
 }
 

@@ -362,6 +362,19 @@ void Engine::build_near() {
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return (long long)tgt_cols[a] * (lp[a + 1] - lp[a]) > (long long)tgt_cols[b] * (lp[b + 1] - lp[b]);
     });
+    // matrix-core items: row ranges of at most 64 rows, so that large leaves do
+    // not set the kernel's duration
+    std::vector<int2> items;
+    for (int t = 0; t < nleaf; ++t)
+        for (int rt = 0; rt * 16 < lp[t + 1] - lp[t]; rt += 4) items.push_back(make_int2(t, rt));
+    std::stable_sort(items.begin(), items.end(), [&](int2 a, int2 b) {
+        auto work = [&](int2 i) {
+            return (long long)tgt_cols[i.x] * std::min(64, lp[i.x + 1] - lp[i.x] - 16 * i.y);
+        };
+        return work(a) > work(b);
+    });
+    nd_.nitem = (int)items.size();
+    nd_.items = upload(items);
     nd_.planes = planes;
     nd_.nleaf = nleaf;
     nd_.npairs = (int)pair_src.size();
@@ -389,6 +402,14 @@ void Engine::build_near() {
     if (P.near_precision == MLFMA_FP32) {
         d_Z32_ = dalloc<float2>((size_t)near_entries);
         launch_narrow(st, (size_t)near_entries, d_Z_, d_Z32_);
+        MLFMA_HIP(hipGetLastError());
+    }
+    if (P.precond_precision == MLFMA_BF16) {
+        std::vector<long long> bf_off;
+        const long long recs = bf16_near_layout(tree.leaf_ptr, tgt_cols, planes, bf_off);
+        d_bfoff_ = upload(bf_off);
+        d_Zbf_ = dalloc<uint4>((size_t)recs);
+        launch_pack_bf16(st, nd_, d_Z_, d_bfoff_, d_Zbf_);
         MLFMA_HIP(hipGetLastError());
     }
     d_dinv_ = dalloc<double2>(nunk);
@@ -810,9 +831,12 @@ double Engine::norm(const double2* v, int n) {
 // (Givens QR of the Hessenberg matrix, exact residual at every restart,
 // divergence guard); the Arnoldi step uses classical Gram-Schmidt applied twice
 // (two block reductions per pass instead of one per basis vector), which has
-// the orthogonality of the reference's MGS + re-orthogonalisation.  With
-// precond = 1 the Krylov space is built for A M^{-1} (right preconditioning) and
-// the reported residual is the true ||b - A x|| / ||b||.
+// the orthogonality of the reference's MGS + re-orthogonalisation.  With a
+// preconditioner the Krylov space is built for A M^{-1} (right preconditioning)
+// in flexible form: z_k = M^-1 v_k is kept and the update is x += Z y, so a
+// preconditioner applied in reduced precision (not exactly linear) enters the
+// solution only through the z_k themselves.  The reported residual is the true
+// ||b - A x|| / ||b||.
 void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, mlfma_solve_stats& stats) {
     const double t0 = wall();
     const int n = nunk;
@@ -834,6 +858,11 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
     bool pc = sp.precond != 0;
     precond_kind_ = sp.precond;
     if (sp.precond == 1) precond_setup();
+    if (pc && sp.solver == MLFMA_SOLVER_GMRES && Zf_cols_ < m) {
+        dfree(d_Zf_);
+        d_Zf_ = dalloc<double2>((size_t)m * n);
+        Zf_cols_ = m;
+    }
     int matvecs = 0, fp64_matvecs = 0, total = 0;
     auto A = [&](const double2* in, double2* out) {
         if (pc) {
@@ -888,7 +917,14 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
                 if (total >= sp.itr_max) break;
                 ++total;
                 kused = k + 1;
-                A(d_V_ + (size_t)k * n, d_w_);
+                if (pc) {   // flexible: keep z_k = M^-1 v_k, so x += Z y needs no further application
+                    double2* z = d_Zf_ + (size_t)k * n;
+                    precond_apply(d_V_ + (size_t)k * n, z);
+                    apply_one(z, d_w_, apply_prec);
+                    ++matvecs;
+                } else {
+                    A(d_V_ + (size_t)k * n, d_w_);
+                }
                 dots(k + 1, d_w_, h1);
                 put_small(h1, k + 1);
                 launch_sub_combination(st, n, k + 1, d_V_, (size_t)n, d_small_, d_w_);
@@ -933,10 +969,7 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
             }
             put_small(y, mm);
             if (pc) {
-                MLFMA_HIP(hipMemsetAsync(d_w_, 0, sizeof(double2) * n, st));
-                launch_add_combination(st, n, mm, d_V_, (size_t)n, d_small_, d_w_);
-                precond_apply(d_w_, d_t_);
-                launch_axpy(st, n, make_double2(1.0, 0.0), d_t_, d_x, d_x);
+                launch_add_combination(st, n, mm, d_Zf_, (size_t)n, d_small_, d_x);
             } else {
                 launch_add_combination(st, n, mm, d_V_, (size_t)n, d_small_, d_x);
             }
@@ -1030,7 +1063,8 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
 // side (the same recurrences as Engine::solve).  Block vectors are
 // RHS-fastest.  A right-hand side that has converged inside a cycle keeps its
 // Krylov column count fixed; its later columns are not used.  Krylov steps run
-// with the apply precision, every restart starts from the true FP64 residual.
+// with the apply precision, every restart starts from the true FP64 residual;
+// the polynomial preconditioner is applied in flexible form as in solve().
 void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R,
                          mlfma_solve_stats* stats) {
     const double t0 = wall();
@@ -1060,6 +1094,12 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         d_bZ_ = dalloc<double2>(nR);
         d_bG_ = dalloc<double2>(nR);
         clear_graphs();   // recorded sequences refer to the freed block buffers
+    }
+    if (pc && (bZf_cols_ < m || bZf_nR_ != nR)) {
+        dfree(d_bZf_);
+        d_bZf_ = dalloc<double2>(nR * m);
+        bZf_cols_ = m;
+        bZf_nR_ = nR;
     }
     int matvecs = 0, fp64_matvecs = 0;
     std::vector<int> iters(R, 0);
@@ -1144,12 +1184,14 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
                         poly_apply(d_bG_, d_bZ_, R, prec);
                         apply_block(d_bZ_, d_bW_, R, prec);
                     });
+                    MLFMA_HIP(hipMemcpyAsync(d_bZf_ + (size_t)k * nR, d_bZ_, nR * sizeof(double2),
+                                             hipMemcpyDeviceToDevice, st));
                 } else {
                     run_graph(GraphKey{2, R, prec}, [&] { apply_block(d_bG_, d_bW_, R, prec); });
                 }
-            } else if (pc) {
-                poly_apply(d_bV_ + (size_t)k * nR, d_bZ_, R, apply_prec);
-                apply_block(d_bZ_, d_bW_, R, apply_prec);
+            } else if (pc) {   // flexible: keep z_k = M^-1 v_k
+                poly_apply(d_bV_ + (size_t)k * nR, d_bZf_ + (size_t)k * nR, R, apply_prec);
+                apply_block(d_bZf_ + (size_t)k * nR, d_bW_, R, apply_prec);
             } else {
                 apply_block(d_bV_ + (size_t)k * nR, d_bW_, R, apply_prec);
             }
@@ -1220,11 +1262,8 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         }
         if (kmax > 0) {
             put(yv);
-            if (pc) {   // x += M^-1 (V y)
-                MLFMA_HIP(hipMemsetAsync(d_bW_, 0, nR * sizeof(double2), st));
-                launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_bW_);
-                poly_apply(d_bW_, d_bZ_, R, apply_prec);
-                launch_axpy(st, (int)nR, make_double2(1.0, 0.0), d_bZ_, d_X, d_X);
+            if (pc) {   // x += Z y with the kept z_k = M^-1 v_k
+                launch_add_combination_b(st, n, R, kmax, d_bZf_, nR, d_bsmall_, d_X);
             } else {
                 launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_X);
             }
@@ -1340,6 +1379,16 @@ void Engine::precond_apply_kind(int kind, const double2* x, double2* y) {
     MLFMA_HIP(hipStreamSynchronize(st));
 }
 
+// Polynomial preconditioner on nrhs column-major device vectors (apply precision).
+void Engine::poly_apply_device(const double2* x, double2* y, int nrhs) {
+    ensure_block(nrhs);
+    ensure_poly(nrhs);
+    launch_transpose(st, nrhs, nunk, x, d_bx_);
+    poly_apply(d_bx_, d_by_, nrhs, apply_prec);
+    launch_transpose(st, nunk, nrhs, d_by_, y);
+    MLFMA_HIP(hipStreamSynchronize(st));
+}
+
 void Engine::precond_apply(const double2* x, double2* y) {
     if (precond_kind_ == 2) {
         poly_apply(x, y, 1, apply_prec);
@@ -1367,11 +1416,21 @@ void Engine::ensure_poly(int R) {
     dfree(d_pu_);
     d_pt_ = dalloc<double2>((size_t)nunk * R);
     d_pu_ = dalloc<double2>((size_t)nunk * R);
+    if (d_Zbf_) {
+        dfree(d_xb_);
+        d_xb_ = dalloc<unsigned>((size_t)nunk * R);
+    }
     poly_cap_ = R;
 }
 
-// Y = Z_near X for an RHS-fastest block (near storage of precision prec)
-void Engine::near_block(const double2* X, double2* Y, int R, int prec) {
+// Y = Z_near X for an RHS-fastest block (near storage of precision prec; the
+// preconditioner's products in FP32 mode use the BF16 copy when there is one)
+void Engine::near_block(const double2* X, double2* Y, int R, int prec, bool precond) {
+    if (precond && prec == MLFMA_FP32 && d_Zbf_) {
+        if (R > poly_cap_) throw Error(MLFMA_ERR_STATE, "preconditioner buffers smaller than the batch");
+        launch_near_bf16(st, R, R, 0, nd_, d_Zbf_, d_bfoff_, X, d_xb_, Y);
+        return;
+    }
     if (prec == MLFMA_FP32 && d_Z32_ && R >= 2 && near_mfma_) {
         launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
         return;
@@ -1391,7 +1450,7 @@ void Engine::poly_apply(const double2* X, double2* Y, int R, int prec) {
     ensure_poly(R);
     launch_poly_init(st, nunk, R, d_dinv_, X, d2(poly_alpha_[0]), d_pt_, Y);
     for (size_t k = 1; k < poly_alpha_.size(); ++k) {
-        near_block(d_pt_, d_pu_, R, prec);
+        near_block(d_pt_, d_pu_, R, prec, true);
         launch_poly_step(st, nunk, R, d_dinv_, d_pu_, d2(poly_alpha_[k]), d_pt_, Y);
     }
 }

@@ -97,6 +97,7 @@ public:
     void set_poly(const std::vector<cplx>& alpha);
     void poly_apply(const double2* X, double2* Y, int R, int prec);
     void precond_features(float* f);
+    void poly_apply_device(const double2* x, double2* y, int nrhs);
     void precond_apply_kind(int kind, const double2* x, double2* y);
     void precond_statistics(const double2* d_b, int steps, int degree, double2* G, double2* h, double* c);
     void excitation(double pol, double th, double ph, double2* d_v);
@@ -174,7 +175,7 @@ private:
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
     void ensure_poly(int R);
-    void near_block(const double2* X, double2* Y, int R, int prec);
+    void near_block(const double2* X, double2* Y, int R, int prec, bool precond = false);
     double norm(const double2* v, int n);
 
     std::vector<void*> allocs_;
@@ -190,6 +191,9 @@ private:
     NearDev nd_{};
     double2* d_Z_ = nullptr;
     float2* d_Z32_ = nullptr;   // FP32 copy (near_precision = MLFMA_FP32)
+    uint4* d_Zbf_ = nullptr;    // BF16 record copy for the preconditioner (precond_precision = MLFMA_BF16)
+    long long* d_bfoff_ = nullptr;
+    unsigned* d_xb_ = nullptr;  // BF16 copy of the preconditioner's near-product input, nunk x poly_cap_
     // RHS-fastest block vectors of the batched matvec
     double2* d_bx_ = nullptr;
     double2* d_by_ = nullptr;
@@ -220,6 +224,11 @@ private:
     // block-diagonal preconditioner
     double2* d_dinv_ = nullptr;   // 1 / diagonal of the operator, caller order
     std::vector<cplx> poly_alpha_;
+    double2* d_Zf_ = nullptr;    // flexible GMRES: preconditioned basis z_k = M^-1 v_k (n x Zf_cols_)
+    int Zf_cols_ = 0;
+    double2* d_bZf_ = nullptr;   // same for the batched solver (n R x bZf_cols_)
+    int bZf_cols_ = 0;
+    size_t bZf_nR_ = 0;
     double2* d_pt_ = nullptr;
     double2* d_pu_ = nullptr;
     int poly_cap_ = 0;

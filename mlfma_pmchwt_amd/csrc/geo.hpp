@@ -37,6 +37,8 @@ struct NearDev {
     const int* pair_tgt;      // set-up only
     const int* pair_src;
     const int* pair_col;      // first column of the pair inside its target
+    int nitem;                // matrix-core work items: (target, first 16-row tile), <= 4 tiles each,
+    const int2* items;        // by decreasing rows * C_t
 };
 
 }  // namespace mlfma

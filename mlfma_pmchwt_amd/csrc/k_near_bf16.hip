@@ -1,0 +1,239 @@
+// Near-field product of the polynomial preconditioner in BF16 on the matrix
+// cores (precond_precision = MLFMA_BF16): for every target leaf t,
+// Y_t (nt x R) = Z_t (nt x C_t) X_t (C_t x R), entries and X rounded to BF16,
+// FP32 accumulation.  Same products as Code/MLFMM.cpp:228-231 (PEC) and
+// :354-366 (PMCHWT: Y_J = Z11 X_J - ZK X_M, Y_M = ZK X_J + Z22 X_M); the
+// reference has no preconditioner, and only the preconditioner uses this copy
+// (the solver's operator and residuals keep their own precision).
+//
+// v_mfma_f32_16x16x32_bf16: D(16x16) += A(16x32) B(32x16); lane l holds
+// A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16], i < 8, and
+// D[4 (l / 16) + v][l % 16], v < 4.  The complex product uses the same
+// [Re Y (8) | Im Y (8)] column split as the FP32 kernel:
+//   D += Re Z . [Re X | Im X] + Im Z . [-Im X | Re X].
+//
+// Storage: per target, per 32-column block kb, 16-row tile rt and plane p, a
+// 2 KB record holding the A fragments of the 64 lanes for Re Z, then for Im Z
+// (8 BF16 = 16 B per lane), so a wavefront reads each record with one
+// coalesced 16-byte load per lane.  Rows past nt and columns past C_t are zero.
+#include <vector>
+
+#include "cplx.hpp"
+#include "kernels.hpp"
+
+namespace mlfma {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+constexpr int kRTC = 4;        // row tiles per work item (the items of NearDev)
+constexpr int kKC = 64;        // staged columns per chunk (two 32-column blocks)
+constexpr int kKP = kKC + 8;   // padded LDS row (144 B, keeps 16-byte alignment)
+
+// round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+__device__ __forceinline__ uint4 neg8(uint4 v) {   // flips the sign of 8 packed BF16 values
+    v.x ^= 0x80008000u;
+    v.y ^= 0x80008000u;
+    v.z ^= 0x80008000u;
+    v.w ^= 0x80008000u;
+    return v;
+}
+
+__device__ __forceinline__ floatx4 mfma(uint4 a, uint4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+}  // namespace
+
+// One workgroup per target leaf: writes its records from the FP64 entries.
+__global__ void k_pack_bf16(NearDev nd, const double2* __restrict__ Z, const long long* __restrict__ bf_off,
+                            uint4* __restrict__ out) {
+    const int t = blockIdx.x;
+    const int nt = nd.leaf_ptr[t + 1] - nd.leaf_ptr[t];
+    if (nt == 0) return;
+    const int C = nd.tgt_cols[t];
+    const int nrt = (nt + 15) >> 4, nkb = (C + 31) >> 5;
+    const size_t plane = (size_t)C * nt;
+    const double2* Zt = Z + nd.tgt_off[t];
+    uint4* o = out + bf_off[t];
+    const int units = nkb * nrt * nd.planes * 64;
+    for (int u = threadIdx.x; u < units; u += blockDim.x) {
+        const int lane = u & 63, rec = u >> 6;   // rec = (kb * nrt + rt) * planes + p
+        const int p = rec % nd.planes, rt = (rec / nd.planes) % nrt, kb = rec / (nd.planes * nrt);
+        const int row = rt * 16 + (lane & 15), cb = kb * 32 + 8 * (lane >> 4);
+        unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 8; ++i) {
+            const int col = cb + i;
+            double2 z = make_double2(0.0, 0.0);
+            if (row < nt && col < C) z = Zt[p * plane + (size_t)col * nt + row];
+            re[i >> 1] |= bf16_bits((float)z.x) << (16 * (i & 1));
+            im[i >> 1] |= bf16_bits((float)z.y) << (16 * (i & 1));
+        }
+        o[(size_t)rec * 128 + lane] = make_uint4(re[0], re[1], re[2], re[3]);
+        o[(size_t)rec * 128 + 64 + lane] = make_uint4(im[0], im[1], im[2], im[3]);
+    }
+}
+
+// xb[u * R + r] = (BF16 Re, BF16 Im) of x[u * ldr + r0 + r]; u < nu, r < R
+__global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __restrict__ x, unsigned* __restrict__ xb) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const size_t u = e / R;
+    const int r = (int)(e - u * R);
+    const double2 v = x[u * ldr + r0 + r];
+    xb[e] = bf16_bits((float)v.x) | (bf16_bits((float)v.y) << 16);
+}
+
+template <int P, int CT>
+__global__ __launch_bounds__(64 * CT) void k_near_bf16(NearDev nd, int R, int ldr, int r0,
+                                                       const uint4* __restrict__ Zb,
+                                                       const long long* __restrict__ bf_off,
+                                                       const unsigned* __restrict__ xb, double2* __restrict__ y) {
+    constexpr int NX = (P == 3) ? 2 : 1;   // staged currents
+    constexpr int NO = (P == 3) ? 2 : 1;   // outputs
+    constexpr int RT = 8 * CT;             // right-hand sides per workgroup
+    __shared__ __align__(16) unsigned short xs[NX][2][RT][kKP];   // [current][Re, Im][rhs][column]
+    const int2 item = nd.items[blockIdx.x];
+    const int t = item.x, rp = item.y;
+    const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
+    const int nrt = (nt + 15) >> 4;
+    const int nrp = min(kRTC, nrt - rp);
+    const int rb0 = blockIdx.y * RT;
+    const int rv = min(RT, R - rb0);
+    const int C = nd.tgt_cols[t];
+    const int nkb = (C + 31) >> 5;
+    const int* cref = nd.colref + nd.col_ptr[t];
+    const uint4* Zt = Zb + bf_off[t];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int part = li >> 3;   // 0: real-part column, 1: imaginary-part column
+    const int rcol = wave * 8 + (li & 7);
+    floatx4 acc[kRTC][NO];
+#pragma unroll
+    for (int s = 0; s < kRTC; ++s)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) acc[s][o] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < C; c0 += kKC) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < NX * kKC * RT; e += 64 * CT) {
+            const int xj = e / (kKC * RT), rem = e - xj * (kKC * RT);
+            const int c = rem / RT, r = rem - c * RT;
+            unsigned v = 0;
+            if (c0 + c < C && r < rv) {
+                const size_t u = (size_t)cref[c0 + c] + (xj ? (size_t)nd.N : 0);
+                v = xb[u * R + rb0 + r];
+            }
+            xs[xj][0][r][c] = (unsigned short)(v & 0xffffu);
+            xs[xj][1][r][c] = (unsigned short)(v >> 16);
+        }
+        __syncthreads();
+        for (int kb2 = 0; kb2 < 2; ++kb2) {
+            const int kb = (c0 >> 5) + kb2;
+            if (kb >= nkb) break;
+            uint4 zr[kRTC][P], zi[kRTC][P];
+#pragma unroll
+            for (int s = 0; s < kRTC; ++s)
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    zr[s][p] = zi[s][p] = make_uint4(0, 0, 0, 0);
+                    if (s < nrp) {
+                        const uint4* rec = Zt + ((size_t)(kb * nrt + rp + s) * P + p) * 128;
+                        zr[s][p] = rec[lane];
+                        zi[s][p] = rec[64 + lane];
+                    }
+                }
+            const int kof = kb2 * 32 + 8 * lk;
+            const uint4 xa = *reinterpret_cast<const uint4*>(&xs[0][part][rcol][kof]);       // Re X | Im X
+            const uint4 xo = *reinterpret_cast<const uint4*>(&xs[0][1 - part][rcol][kof]);   // Im X | Re X
+            const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
+            uint4 bM1 = make_uint4(0, 0, 0, 0), bM2 = bM1;
+            if (P == 3) {
+                const uint4 ma = *reinterpret_cast<const uint4*>(&xs[NX - 1][part][rcol][kof]);
+                const uint4 mo = *reinterpret_cast<const uint4*>(&xs[NX - 1][1 - part][rcol][kof]);
+                bM1 = ma;
+                bM2 = part ? mo : neg8(mo);
+            }
+#pragma unroll
+            for (int s = 0; s < kRTC; ++s) {
+                if (s >= nrp) continue;
+                floatx4 a = acc[s][0];
+                a = mfma(zr[s][0], bJ1, a);
+                a = mfma(zi[s][0], bJ2, a);
+                if (P == 3) {
+                    a = mfma(zr[s][1], neg8(bM1), a);   // - ZK X_M
+                    a = mfma(zi[s][1], neg8(bM2), a);
+                    floatx4 b = acc[s][NO - 1];
+                    b = mfma(zr[s][1], bJ1, b);
+                    b = mfma(zi[s][1], bJ2, b);
+                    b = mfma(zr[s][2], bM1, b);
+                    b = mfma(zi[s][2], bM2, b);
+                    acc[s][NO - 1] = b;
+                }
+                acc[s][0] = a;
+            }
+        }
+    }
+    // D[4 lk + v][li]: lanes li < 8 hold Re of right-hand side wave * 8 + li, lane li + 8 its Im
+#pragma unroll
+    for (int s = 0; s < kRTC; ++s) {
+        if (s >= nrp) continue;
+#pragma unroll
+        for (int o = 0; o < NO; ++o)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float re = acc[s][o][v];
+                const float im = __shfl_xor(re, 8, 64);
+                const int row = (rp + s) * 16 + 4 * lk + v;
+                if (part == 0 && rcol < rv && row < nt) {
+                    const size_t u = (size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0);
+                    y[u * ldr + r0 + rb0 + rcol] = make_double2(re, im);
+                }
+            }
+    }
+}
+
+long long bf16_near_layout(const std::vector<int>& leaf_ptr, const std::vector<int>& tgt_cols, int planes,
+                           std::vector<long long>& bf_off) {
+    const int nleaf = (int)tgt_cols.size();
+    bf_off.assign(nleaf + 1, 0);
+    for (int t = 0; t < nleaf; ++t) {
+        const int nt = leaf_ptr[t + 1] - leaf_ptr[t];
+        const long long recs = nt == 0 ? 0 : (long long)((tgt_cols[t] + 31) / 32) * ((nt + 15) / 16) * planes;
+        bf_off[t + 1] = bf_off[t] + recs * 128;
+    }
+    return bf_off[nleaf];
+}
+
+void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const long long* bf_off, uint4* out) {
+    if (nd.nleaf == 0) return;
+    k_pack_bf16<<<nd.nleaf, 256, 0, st>>>(nd, Z, bf_off, out);
+}
+
+void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                      const long long* bf_off, const double2* x, unsigned* xb, double2* y) {
+    if (nd.nitem == 0 || R < 1) return;
+    const size_t total = (size_t)nd.N * (nd.planes == 3 ? 2 : 1) * R;   // unknowns x right-hand sides
+    k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
+    const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
+    dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
+#define NB(P, CT) k_near_bf16<P, CT><<<grid, 64 * CT, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
+    if (nd.planes == 1) {
+        if (ct == 4) NB(1, 4);
+        else if (ct == 2) NB(1, 2);
+        else NB(1, 1);
+    } else {
+        if (ct == 4) NB(3, 4);
+        else if (ct == 2) NB(3, 2);
+        else NB(3, 1);
+    }
+#undef NB
+}
+
+}  // namespace mlfma

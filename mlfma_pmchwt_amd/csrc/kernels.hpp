@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "geo.hpp"
 
 namespace mlfma {
@@ -111,5 +113,14 @@ void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int r
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
+
+// BF16 near-field copy for the polynomial preconditioner: per-target record
+// offsets (in 16-byte units; returns the total), packing from the FP64
+// entries, and the batched product (x rounded to BF16 into xb, nu x R).
+long long bf16_near_layout(const std::vector<int>& leaf_ptr, const std::vector<int>& tgt_cols, int planes,
+                           std::vector<long long>& bf_off);
+void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const long long* bf_off, uint4* out);
+void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                      const long long* bf_off, const double2* x, unsigned* xb, double2* y);
 
 }  // namespace mlfma
