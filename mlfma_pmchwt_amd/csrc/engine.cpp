@@ -431,11 +431,12 @@ std::vector<double2> Engine::near_block(int p) {
 }
 
 // Right-hand sides per pass of the batched matvec: nc = rb * ncur workspace
-// currents, at most 4 in FP64 and 8 in FP32 (register budget of the
-// translation and disaggregation kernels), and at most max_rhs.
+// currents, at most 4 in FP64 and 16 in FP32 (register budget of the
+// per-box translation and vector disaggregation kernels), rb at most 8 (one
+// matrix-core column tile of the disaggregation) and at most max_rhs.
 template <typename R>
 int Engine::rb_max() const {
-    const int cap = (std::is_same<R, double>::value ? 4 : 8) / ncur;
+    const int cap = std::min(8, (std::is_same<R, double>::value ? 4 : 16) / ncur);
     int rb = 1;
     while (rb * 2 <= cap && rb * 2 <= P.max_rhs) rb *= 2;
     return rb;

@@ -9,6 +9,8 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include <stdexcept>
+
 #include "cplx.hpp"
 
 namespace mlfma {
@@ -115,7 +117,9 @@ void launch_aggregate_b(hipStream_t st, int ncur, int rb, int nleaf, int K, int 
     } else {
         if (rb == 1) AGG(2, 1);
         else if (rb == 2) AGG(2, 2);
-        else AGG(2, 4);
+        else if (rb == 4) AGG(2, 4);
+        else if constexpr (sizeof(R) == 4) AGG(2, 8);
+        else throw std::invalid_argument("FP64 aggregation: at most 4 right-hand sides per pass");
     }
 #undef AGG
 }
@@ -129,12 +133,12 @@ void launch_aggregate(hipStream_t st, int ncur, int nleaf, int K, int N, const i
 // ---------------------------------------------------------------- interpolation
 // Code/ProcessMLFMM.h:61-107 (m2m): 3x3 Lagrange taps into the child grid, then
 // the octant phase shift, summed over the children of each parent.  A thread
-// owns one parent sample and 3 of the nc x 3 workspace columns (G = nc
-// wavefronts per workgroup, one per column group); the q tiles of one parent
+// owns one parent sample and CPT of the nc x 3 workspace columns (one
+// wavefront per column group); the q tiles of one parent
 // are consecutive after the XCD remap, so they share that XCD's L2 copy of the
 // children's workspaces.
-template <typename R, int NC>
-__global__ __launch_bounds__(64 * NC) void k_m2m(int nparent, int qtiles, int Kp, int Kc,
+template <typename R, int NC, int CPT>
+__global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m(int nparent, int qtiles, int Kp, int Kc,
                                                  const int* __restrict__ son_ptr, const int* __restrict__ son_idx,
                                                  const int* __restrict__ oct_child, const int* __restrict__ tap,
                                                  const R* __restrict__ w, const cplx_t<R>* __restrict__ shup,
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(64 * NC) void k_m2m(int nparent, int qtiles, int Kp
     const int lin = xcd_remap(blockIdx.x, nparent * qtiles);
     const int p = lin / qtiles;
     const int q = (lin % qtiles) * 64 + (threadIdx.x & 63);
-    const int c0 = (threadIdx.x >> 6) * 3;   // first workspace column of this thread
+    const int c0 = (threadIdx.x >> 6) * CPT;   // first workspace column of this thread
     if (q >= Kp) return;
     int t[9];
     R ww[9];
@@ -152,15 +156,15 @@ __global__ __launch_bounds__(64 * NC) void k_m2m(int nparent, int qtiles, int Kp
         t[s] = tap[(size_t)s * Kp + q];
         ww[s] = w[(size_t)s * Kp + q];
     }
-    C acc[3];
+    C acc[CPT];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d] = C{0, 0};
+    for (int d = 0; d < CPT; ++d) acc[d] = C{0, 0};
     for (int j = son_ptr[p]; j < son_ptr[p + 1]; ++j) {
         const int c = son_idx[j];
         const C e = shup[(size_t)oct_child[c] * Kp + q];
         const C* S = Smc + ((size_t)c * NC * 3 + c0) * Kc;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
+        for (int d = 0; d < CPT; ++d) {
             const C* sd = S + (size_t)d * Kc;
             C r = C{0, 0};
 #pragma unroll
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(64 * NC) void k_m2m(int nparent, int qtiles, int Kp
     }
     C* o = Smp + ((size_t)p * NC * 3 + c0) * Kp + q;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) o[(size_t)d * Kp] = acc[d];
+    for (int d = 0; d < CPT; ++d) o[(size_t)d * Kp] = acc[d];
 }
 
 template <typename R>
@@ -183,13 +187,15 @@ void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int
                 cplx_t<R>* Sm_parent) {
     if (nparent == 0) return;
     const int qt = (Kp + 63) / 64;
-#define M2M(NC)                                                                                                       \
-    k_m2m<R, NC><<<nparent * qt, 64 * NC, 0, st>>>(nparent, qt, Kp, Kc, son_ptr, son_idx, octant_child, tap, w,      \
-                                                   shift_up, Sm_child, Sm_parent)
-    if (ncur == 1) M2M(1);
-    else if (ncur == 2) M2M(2);
-    else if (ncur == 4) M2M(4);
-    else M2M(8);
+#define M2M(NC, CPT)                                                                                                  \
+    k_m2m<R, NC, CPT><<<nparent * qt, 64 * NC * 3 / CPT, 0, st>>>(nparent, qt, Kp, Kc, son_ptr, son_idx,             \
+                                                                  octant_child, tap, w, shift_up, Sm_child, Sm_parent)
+    if (ncur == 1) M2M(1, 3);
+    else if (ncur == 2) M2M(2, 3);
+    else if (ncur == 4) M2M(4, 3);
+    else if (ncur == 8) M2M(8, 3);
+    else if constexpr (sizeof(R) == 4) M2M(16, 6);
+    else throw std::invalid_argument("FP64 far field: at most 8 workspace currents per pass");
 #undef M2M
 }
 
@@ -257,7 +263,9 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
     if (ncur == 1) M2L(1);
     else if (ncur == 2) M2L(2);
     else if (ncur == 4) M2L(4);
-    else M2L(8);
+    else if (ncur == 8) M2L(8);
+    else if constexpr (sizeof(R) == 4) M2L(16);
+    else throw std::invalid_argument("FP64 far field: at most 8 workspace currents per pass");
 #undef M2L
 }
 
@@ -359,7 +367,9 @@ void launch_m2l_cl(hipStream_t st, int ncur, int ncl, int K, const int* cl_child
     if (ncur == 1) M2LC(1)
     else if (ncur == 2) M2LC(2)
     else if (ncur == 4) M2LC(4)
-    else M2LC(8)
+    else if (ncur == 8) M2LC(8)
+    else if constexpr (sizeof(R) == 4) M2LC(16)
+    else throw std::invalid_argument("FP64 far field: at most 8 workspace currents per pass");
 #undef M2LC
 }
 
@@ -541,7 +551,9 @@ void launch_disagg_pmchwt_b(hipStream_t st, int rb, int ldr, int r0, const int* 
     const int blocks = (a.N + 3) / 4;
     if (rb == 1) k_disagg_pmchwt<R, 1><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
     else if (rb == 2) k_disagg_pmchwt<R, 2><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
-    else k_disagg_pmchwt<R, 4><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
+    else if (rb == 4) k_disagg_pmchwt<R, 4><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
+    else if constexpr (sizeof(R) == 4) k_disagg_pmchwt<R, 8><<<blocks, 256, 0, st>>>(ldr, r0, leaf_of, perm, a, y);
+    else throw std::invalid_argument("FP64 disaggregation: at most 4 right-hand sides per pass");
 }
 
 template <typename R>

@@ -173,7 +173,7 @@ def test_cluster_translation_matches_per_box(name):
     FP64 results agree with the per-box kernel to rounding, both precisions,
     single and batched, and with the oracle at the FP64 bar."""
     mesh, op, orc = pair(name)
-    X = np.stack([seeded_x(op.n, 900 + j) for j in range(5)], axis=1)
+    X = np.stack([seeded_x(op.n, 900 + j) for j in range(9)], axis=1)   # passes of 8 and 1
     out = {}
     for bits in (0, 4):
         for prec in (api.FP64, api.FP32):
