@@ -125,14 +125,14 @@ int mlfma_set_apply_precision(mlfma_handle* h, int precision);
 int mlfma_set_graphs(mlfma_handle* h, int on);
 
 /* Launch-configuration autotuner: times the batched matvec of min(nrhs, 32)
- * right-hand sides in `precision` for each candidate (right-hand sides per
- * far-field pass; near product and disaggregation on the matrix cores or on
- * the vector units; translation per box or by sibling clusters) and keeps the
- * fastest for this handle.  out4 = {rhs per pass, choice bits (bit 0 near on
+ * right-hand sides in `precision` and picks, by coordinate descent from the
+ * widest configuration, the right-hand sides per far-field pass and whether
+ * the near product and the disaggregation run on the matrix cores or on the
+ * vector units and the translation by sibling clusters or per box; keeps the
+ * choice for this handle.  out4 = {rhs per pass, choice bits (bit 0 near on
  * the matrix cores, bit 1 disaggregation on the matrix cores, bit 2 cluster
  * translation), ms per batched matvec of the choice, ms of the slowest
- * candidate}.  No reference
- * counterpart. */
+ * configuration timed}.  No reference counterpart. */
 int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
 
 /* Sets the launch configuration directly (e.g. one that mlfma_autotune chose

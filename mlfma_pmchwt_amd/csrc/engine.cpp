@@ -687,14 +687,17 @@ void Engine::set_graphs(bool on) {
 }
 
 // ---------------------------------------------------------------- autotuner
-// Times the batched matvec of R (at most 32) right-hand sides in precision prec for every
-// candidate launch configuration (right-hand sides per far-field pass; near
-// product on the matrix cores or on the vector units when the near field has
-// an FP32 copy; disaggregation on the matrix cores or on the vector units in
-// FP32; translation by sibling clusters or per box) and keeps the fastest for
-// this handle.  out[0] = chosen rb, out[1] = choice bits (bit 0 matrix-core
-// near, bit 1 matrix-core disaggregation, bit 2 cluster translation), out[2] =
-// milliseconds per matvec of the choice, out[3] = of the slowest candidate.
+// Launch-configuration search for the batched matvec of R (at most 32)
+// right-hand sides in precision prec, by coordinate descent from the widest
+// configuration (the most right-hand sides per far-field pass, every
+// available matrix-core and cluster path): each choice (bit 0 near product on
+// the matrix cores, needs the FP32 near copy; bit 1 disaggregation on the
+// matrix cores, FP32 far field; bit 2 translation by sibling clusters) is
+// flipped in turn and kept when faster, then narrower passes are tried.  The
+// choices act on separate kernels, so they barely interact, and the search
+// times about six configurations instead of all of them.  out[0] = chosen rb,
+// out[1] = choice bits, out[2] = milliseconds per matvec of the choice, out[3]
+// = of the slowest configuration timed.
 void Engine::autotune(int R, int prec, double* out) {
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "autotune: 1..256 right-hand sides");
     R = std::min(R, 32);   // the per-pass choices do not depend on larger batches; keeps tuning short
@@ -703,41 +706,43 @@ void Engine::autotune(int R, int prec, double* out) {
     ensure_block(R);
     launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
     const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
-    // bit 0: near on the matrix cores, bit 1: disaggregation on the matrix cores,
-    // bit 2: sibling-cluster translation
-    std::vector<int> mfma_opts;
     const bool nm = R >= 2 && prec == MLFMA_FP32 && d_Z32_, dm = R >= 2 && f32 && wd_[0].E;
-    for (int m = 7; m >= 0; --m)
-        if ((!(m & 1) || nm) && (!(m & 2) || dm)) mfma_opts.push_back(m);
+    const int valid = (nm ? 1 : 0) | (dm ? 2 : 0) | 4;
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
     MLFMA_HIP(hipEventCreate(&b));
-    const bool keep_mfma = near_mfma_;
-    const int keep_rb = rb_tuned_[f32 ? 1 : 0];
-    double best = 1e30, worst = 0;
-    int best_rb = rbmax, best_m = (keep_mfma ? 1 : 0) | (disagg_mfma_ ? 2 : 0) | (m2l_cluster_ ? 4 : 0);
-    for (int m : mfma_opts) {
-        for (int rb = 1; rb <= rbmax; rb *= 2) {
-            near_mfma_ = (m & 1) != 0;
-            disagg_mfma_ = (m & 2) != 0;
-            m2l_cluster_ = (m & 4) != 0;
-            rb_tuned_[f32 ? 1 : 0] = rb;
-            apply_block(d_bx_, d_by_, R, prec);   // warm-up
-            MLFMA_HIP(hipEventRecord(a, st));
-            for (int k = 0; k < 2; ++k) apply_block(d_bx_, d_by_, R, prec);
-            MLFMA_HIP(hipEventRecord(b, st));
-            MLFMA_HIP(hipEventSynchronize(b));
-            float ms = 0;
-            MLFMA_HIP(hipEventElapsedTime(&ms, a, b));
-            const double per = ms / 2.0;
-            if (P.verbose) std::fprintf(stderr, "[mlfma] autotune R=%d prec=%d rb=%d mfma=%d: %.3f ms\n", R, prec, rb, m, per);
-            if (per < best) best = per, best_rb = rb, best_m = m;
-            worst = std::max(worst, per);
-        }
+    double worst = 0;
+    auto time = [&](int rb, int m) {
+        near_mfma_ = (m & 1) != 0;
+        disagg_mfma_ = (m & 2) != 0;
+        m2l_cluster_ = (m & 4) != 0;
+        rb_tuned_[f32 ? 1 : 0] = rb;
+        apply_block(d_bx_, d_by_, R, prec);   // warm-up
+        MLFMA_HIP(hipEventRecord(a, st));
+        for (int k = 0; k < 2; ++k) apply_block(d_bx_, d_by_, R, prec);
+        MLFMA_HIP(hipEventRecord(b, st));
+        MLFMA_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        MLFMA_HIP(hipEventElapsedTime(&ms, a, b));
+        const double per = ms / 2.0;
+        if (P.verbose) std::fprintf(stderr, "[mlfma] autotune R=%d prec=%d rb=%d bits=%d: %.3f ms\n", R, prec, rb, m, per);
+        worst = std::max(worst, per);
+        return per;
+    };
+    int best_rb = rbmax, best_m = valid;
+    double best = time(best_rb, best_m);
+    for (int bit = 1; bit <= 4; bit <<= 1) {
+        if (!(valid & bit)) continue;
+        const double t = time(best_rb, best_m ^ bit);
+        if (t < best) best = t, best_m ^= bit;
+    }
+    for (int rb = rbmax / 2; rb >= std::max(1, rbmax / 4); rb /= 2) {
+        const double t = time(rb, best_m);
+        if (t < best) best = t, best_rb = rb;
+        else break;
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
-    (void)keep_rb;
     rb_tuned_[f32 ? 1 : 0] = best_rb;
     near_mfma_ = (best_m & 1) != 0;
     disagg_mfma_ = (best_m & 2) != 0;
@@ -1277,6 +1282,13 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
             if (!std::isfinite(err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
             if (err[r] < sp.tol_rel || err[r] < sp.tol_abs) done[r] = 1;
             if (!done[r] && err[r] <= 0.9 * err_prev[r]) progress = true;
+        }
+        if (P.verbose) {
+            int nd = 0;
+            double emax = 0;
+            for (int r = 0; r < R; ++r) nd += done[r] ? 1 : 0, emax = std::max(emax, err[r]);
+            std::fprintf(stderr, "[mlfma] batched gmres cycle: %d of %d converged, max true residual %.3e\n", nd, R,
+                         emax);
         }
         // fallback as in solve(): drop a stalling polynomial preconditioner
         if (pc && !all_done() && !progress) {
