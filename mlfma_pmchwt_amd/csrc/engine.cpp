@@ -241,11 +241,21 @@ void Engine::build_wave(int w) {
         D.t_kp[c] = upload(it.t_kp);
     }
     build_far<double>(w, T);
+    const int no = ie == MLFMA_PMCHWT ? 2 : 1;
+    if (rb_max<double>() > 1) D.E64 = dalloc<double2>((size_t)nleaf * rb_max<double>() * no * 3 * D.K[L]);
     if (P.far_precision == MLFMA_FP32) {
         build_far<float>(w, T);
-        if (rb_max<float>() > 1)
-            D.E = dalloc<float2>((size_t)nleaf * rb_max<float>() * (ie == MLFMA_PMCHWT ? 2 : 1) * 3 * D.K[L]);
+        if (rb_max<float>() > 1) D.E = dalloc<float2>((size_t)nleaf * rb_max<float>() * no * 3 * D.K[L]);
     }
+}
+
+template <>
+float2* Engine::beff_buf<float>(int w) {
+    return wd_[w].E;
+}
+template <>
+double2* Engine::beff_buf<double>(int w) {
+    return wd_[w].E64;
 }
 
 template <typename R>
@@ -431,12 +441,12 @@ std::vector<double2> Engine::near_block(int p) {
 }
 
 // Right-hand sides per pass of the batched matvec: nc = rb * ncur workspace
-// currents, at most 4 in FP64 and 16 in FP32 (register budget of the
+// currents, at most 8 in FP64 and 16 in FP32 (register budget of the
 // per-box translation and vector disaggregation kernels), rb at most 8 (one
 // matrix-core column tile of the disaggregation) and at most max_rhs.
 template <typename R>
 int Engine::rb_max() const {
-    const int cap = std::min(8, (std::is_same<R, double>::value ? 4 : 16) / ncur);
+    const int cap = std::min(8, (std::is_same<R, double>::value ? 8 : 16) / ncur);
     int rb = 1;
     while (rb * 2 <= cap && rb * 2 <= P.max_rhs) rb *= 2;
     return rb;
@@ -486,27 +496,26 @@ void Engine::translate(int w, int l, int nc) {
 template <typename R>
 void Engine::disaggregate(double2* y, int ldr, int r0, int rb) {
     const int L = tree.max_level;
-    if constexpr (std::is_same<R, float>::value) {
-        if (rb >= 2 && disagg_mfma_ && wd_[0].E) {
-            const bool pm = ie == MLFMA_PMCHWT;
-            if (!pm) {
-                const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));   // Code/MLFMM.cpp:226
-                const double2 z = make_double2(0, 0);
-                launch_beff(st, nleaf, rb, wd_[0].K[L], false, wd_[0].f32.Bm[L], wd_[0].f32.kp0, d2(c), z, z, z,
-                            wd_[0].E);
-                launch_disagg_mfma(st, nleaf, N, ldr, r0, rb, d_leaf_order_, d_leaf_ptr_, d_perm_, false,
-                                   wd_[0].K[L], wd_[0].f32.Vf, wd_[0].E, 0, nullptr, nullptr, y);
-            } else {
-                const cplx c1 = med.k1 * med.k1 / (16.0 * kPi * kPi), c2 = med.k2 * med.k2 / (16.0 * kPi * kPi);
-                launch_beff(st, nleaf, rb, wd_[0].K[L], true, wd_[0].f32.Bm[L], wd_[0].f32.kp0, d2(c1), d2(c1),
-                            d2(c1), d2(c1), wd_[0].E);
-                launch_beff(st, nleaf, rb, wd_[1].K[L], true, wd_[1].f32.Bm[L], wd_[1].f32.kp0, d2(c2 * med.etai),
-                            d2(c2), d2(c2), d2(c2 / med.etai), wd_[1].E);
-                launch_disagg_mfma(st, nleaf, N, ldr, r0, rb, d_leaf_order_, d_leaf_ptr_, d_perm_, true,
-                                   wd_[0].K[L], wd_[0].f32.Vf, wd_[0].E, wd_[1].K[L], wd_[1].f32.Vf, wd_[1].E, y);
-            }
-            return;
+    if (rb >= 2 && disagg_mfma_ && beff_buf<R>(0)) {
+        const bool pm = ie == MLFMA_PMCHWT;
+        const FarDev<R>& F1 = far<R>(0);
+        if (!pm) {
+            const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));   // Code/MLFMM.cpp:226
+            const double2 z = make_double2(0, 0);
+            launch_beff<R>(st, nleaf, rb, wd_[0].K[L], false, F1.Bm[L], F1.kp0, d2(c), z, z, z, beff_buf<R>(0));
+            launch_disagg_mfma<R>(st, nleaf, N, ldr, r0, rb, d_leaf_order_, d_leaf_ptr_, d_perm_, false, wd_[0].K[L],
+                                  F1.Vf, beff_buf<R>(0), 0, nullptr, nullptr, y);
+        } else {
+            const FarDev<R>& F2 = far<R>(1);
+            const cplx c1 = med.k1 * med.k1 / (16.0 * kPi * kPi), c2 = med.k2 * med.k2 / (16.0 * kPi * kPi);
+            launch_beff<R>(st, nleaf, rb, wd_[0].K[L], true, F1.Bm[L], F1.kp0, d2(c1), d2(c1), d2(c1), d2(c1),
+                           beff_buf<R>(0));
+            launch_beff<R>(st, nleaf, rb, wd_[1].K[L], true, F2.Bm[L], F2.kp0, d2(c2 * med.etai), d2(c2), d2(c2),
+                           d2(c2 / med.etai), beff_buf<R>(1));
+            launch_disagg_mfma<R>(st, nleaf, N, ldr, r0, rb, d_leaf_order_, d_leaf_ptr_, d_perm_, true, wd_[0].K[L],
+                                  F1.Vf, beff_buf<R>(0), wd_[1].K[L], F2.Vf, beff_buf<R>(1), y);
         }
+        return;
     }
     if (ie != MLFMA_PMCHWT) {
         // (k eta / 4 pi)(k / 4 pi), Code/MLFMM.cpp:226
@@ -598,10 +607,11 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
     int cap = f32 ? rb_max<float>() : rb_max<double>();
     if (rb_tuned_[f32 ? 1 : 0] > 0) cap = std::min(cap, rb_tuned_[f32 ? 1 : 0]);
-    const bool mfma_near = prec == MLFMA_FP32 && d_Z32_ && R >= 2 && near_mfma_;
+    const bool mfma_near = R >= 2 && near_mfma_;
     if (mfma_near) {
         if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
-        launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
+        if (prec == MLFMA_FP32 && d_Z32_) launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
+        else launch_near_mfma64(st, R, R, 0, nd_, d_Z_, X, Y);
         if (prof_on_) {
             MLFMA_HIP(hipEventRecord(ev_[17], st));
             MLFMA_HIP(hipEventSynchronize(ev_[17]));
@@ -691,8 +701,8 @@ void Engine::set_graphs(bool on) {
 // right-hand sides in precision prec, by coordinate descent from the widest
 // configuration (the most right-hand sides per far-field pass, every
 // available matrix-core and cluster path): each choice (bit 0 near product on
-// the matrix cores, needs the FP32 near copy; bit 1 disaggregation on the
-// matrix cores, FP32 far field; bit 2 translation by sibling clusters) is
+// the matrix cores, FP32 or FP64 entries; bit 1 disaggregation on the matrix
+// cores; bit 2 translation by sibling clusters) is
 // flipped in turn and kept when faster, then narrower passes are tried.  The
 // choices act on separate kernels, so they barely interact, and the search
 // times about six configurations instead of all of them.  out[0] = chosen rb,
@@ -706,7 +716,7 @@ void Engine::autotune(int R, int prec, double* out) {
     ensure_block(R);
     launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
     const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
-    const bool nm = R >= 2 && prec == MLFMA_FP32 && d_Z32_, dm = R >= 2 && f32 && wd_[0].E;
+    const bool nm = R >= 2, dm = R >= 2 && (f32 ? wd_[0].E != nullptr : wd_[0].E64 != nullptr);
     const int valid = (nm ? 1 : 0) | (dm ? 2 : 0) | 4;
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
@@ -1444,8 +1454,9 @@ void Engine::near_block(const double2* X, double2* Y, int R, int prec, bool prec
         launch_near_bf16(st, R, R, 0, nd_, d_Zbf_, d_bfoff_, X, d_xb_, Y);
         return;
     }
-    if (prec == MLFMA_FP32 && d_Z32_ && R >= 2 && near_mfma_) {
-        launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
+    if (R >= 2 && near_mfma_) {
+        if (prec == MLFMA_FP32 && d_Z32_) launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
+        else launch_near_mfma64(st, R, R, 0, nd_, d_Z_, X, Y);
         return;
     }
     const int cap = (ie == MLFMA_PMCHWT) ? 4 : 8;

@@ -64,7 +64,8 @@ struct WaveDev {
     std::vector<int*> tap, t_ptr, t_kp;  // per child tree level
     FarDev<double> f64;                  // always built (true residuals, stage API)
     FarDev<float> f32;                   // built when far_precision = MLFMA_FP32
-    float2* E = nullptr;                 // FP32 batched disaggregation: effective receive-side matrices
+    float2* E = nullptr;                 // batched matrix-core disaggregation: effective receive-side matrices,
+    double2* E64 = nullptr;              // FP32 and FP64 far field
 };
 
 class Engine {
@@ -166,6 +167,8 @@ private:
     void disaggregate(double2* y, int ldr, int r0, int rb);
     template <typename R>
     void translate(int w, int l, int nc);
+    template <typename R>
+    cplx_t<R>* beff_buf(int w);
     void apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec, bool with_near);
     void mark(int slot);
     void collect_profile(int count);
@@ -233,8 +236,8 @@ private:
     double2* d_pu_ = nullptr;
     int poly_cap_ = 0;
     int precond_kind_ = 0;
-    bool near_mfma_ = true;     // batched FP32 near product on the matrix cores
-    bool disagg_mfma_ = true;   // batched FP32 disaggregation on the matrix cores
+    bool near_mfma_ = true;     // batched near product on the matrix cores (FP32 copy or FP64 entries)
+    bool disagg_mfma_ = true;   // batched disaggregation on the matrix cores
     bool m2l_cluster_ = true;   // translation by sibling clusters
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;

@@ -91,11 +91,19 @@ __global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __
     xb[e] = bf16_bits((float)v.x) | (bf16_bits((float)v.y) << 16);
 }
 
+// A workgroup owns one work item (a target leaf's range of up to 4 16-row
+// tiles) and CT <= 8 column tiles of 8 right-hand sides.  Wavefront w owns row
+// tile w of the item and every column tile, so each record is read from
+// memory once, by one wavefront, for up to 64 right-hand sides; the next
+// 32-column block's records are loaded while the current block's products
+// run.  X is staged in LDS 64 columns at a time as BF16, [Re, Im][rhs][column]
+// rows of 144 B: a thread gathers 8 consecutive columns of one right-hand side
+// (each load coalesced over the wavefront's right-hand sides) and writes them
+// with two 16-byte stores, which 8 consecutive lanes spread over all 32 banks.
 template <int P, int CT>
-__global__ __launch_bounds__(64 * CT) void k_near_bf16(NearDev nd, int R, int ldr, int r0,
-                                                       const uint4* __restrict__ Zb,
-                                                       const long long* __restrict__ bf_off,
-                                                       const unsigned* __restrict__ xb, double2* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, int r0, const uint4* __restrict__ Zb,
+                                                   const long long* __restrict__ bf_off,
+                                                   const unsigned* __restrict__ xb, double2* __restrict__ y) {
     constexpr int NX = (P == 3) ? 2 : 1;   // staged currents
     constexpr int NO = (P == 3) ? 2 : 1;   // outputs
     constexpr int RT = 8 * CT;             // right-hand sides per workgroup
@@ -110,93 +118,100 @@ __global__ __launch_bounds__(64 * CT) void k_near_bf16(NearDev nd, int R, int ld
     const int C = nd.tgt_cols[t];
     const int nkb = (C + 31) >> 5;
     const int* cref = nd.colref + nd.col_ptr[t];
-    const uint4* Zt = Zb + bf_off[t];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const int part = li >> 3;   // 0: real-part column, 1: imaginary-part column
-    const int rcol = wave * 8 + (li & 7);
-    floatx4 acc[kRTC][NO];
+    const int part = li >> 3;          // 0: real-part column, 1: imaginary-part column
+    const bool wact = wave < nrp;      // this wavefront's row tile exists
+    const uint4* zrec = Zb + bf_off[t] + (size_t)(rp + (wact ? wave : 0)) * P * 128 + lane;
+    const size_t kstride = (size_t)nrt * P * 128;   // records of one 32-column block
+    auto load = [&](int kb, uint4* zr, uint4* zi) {
 #pragma unroll
-    for (int s = 0; s < kRTC; ++s)
+        for (int p = 0; p < P; ++p) {
+            zr[p] = zi[p] = make_uint4(0, 0, 0, 0);
+            if (wact && kb < nkb) {
+                const uint4* rec = zrec + (size_t)kb * kstride + p * 128;
+                zr[p] = rec[0];
+                zi[p] = rec[64];
+            }
+        }
+    };
+    floatx4 acc[CT][NO];
 #pragma unroll
-        for (int o = 0; o < NO; ++o) acc[s][o] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) acc[c][o] = floatx4{0.f, 0.f, 0.f, 0.f};
+    uint4 nr[P], ni[P];
+    load(0, nr, ni);
     for (int c0 = 0; c0 < C; c0 += kKC) {
         __syncthreads();
-        for (int e = threadIdx.x; e < NX * kKC * RT; e += 64 * CT) {
-            const int xj = e / (kKC * RT), rem = e - xj * (kKC * RT);
-            const int c = rem / RT, r = rem - c * RT;
-            unsigned v = 0;
-            if (c0 + c < C && r < rv) {
-                const size_t u = (size_t)cref[c0 + c] + (xj ? (size_t)nd.N : 0);
-                v = xb[u * R + rb0 + r];
+        for (int e = threadIdx.x; e < NX * 8 * RT; e += 256) {
+            const int xj = e / (8 * RT), rem = e - xj * (8 * RT);
+            const int cg = rem / RT, r = rem - cg * RT;
+            unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = c0 + cg * 8 + i;
+                unsigned v = 0;
+                if (c < C && r < rv) v = xb[((size_t)cref[c] + (xj ? (size_t)nd.N : 0)) * R + rb0 + r];
+                re[i >> 1] |= (v & 0xffffu) << (16 * (i & 1));
+                im[i >> 1] |= (v >> 16) << (16 * (i & 1));
             }
-            xs[xj][0][r][c] = (unsigned short)(v & 0xffffu);
-            xs[xj][1][r][c] = (unsigned short)(v >> 16);
+            *reinterpret_cast<uint4*>(&xs[xj][0][r][cg * 8]) = make_uint4(re[0], re[1], re[2], re[3]);
+            *reinterpret_cast<uint4*>(&xs[xj][1][r][cg * 8]) = make_uint4(im[0], im[1], im[2], im[3]);
         }
         __syncthreads();
         for (int kb2 = 0; kb2 < 2; ++kb2) {
             const int kb = (c0 >> 5) + kb2;
             if (kb >= nkb) break;
-            uint4 zr[kRTC][P], zi[kRTC][P];
+            uint4 zr[P], zi[P];
 #pragma unroll
-            for (int s = 0; s < kRTC; ++s)
-#pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    zr[s][p] = zi[s][p] = make_uint4(0, 0, 0, 0);
-                    if (s < nrp) {
-                        const uint4* rec = Zt + ((size_t)(kb * nrt + rp + s) * P + p) * 128;
-                        zr[s][p] = rec[lane];
-                        zi[s][p] = rec[64 + lane];
-                    }
-                }
+            for (int p = 0; p < P; ++p) zr[p] = nr[p], zi[p] = ni[p];
+            load(kb + 1, nr, ni);   // next block
+            if (!wact) continue;
             const int kof = kb2 * 32 + 8 * lk;
-            const uint4 xa = *reinterpret_cast<const uint4*>(&xs[0][part][rcol][kof]);       // Re X | Im X
-            const uint4 xo = *reinterpret_cast<const uint4*>(&xs[0][1 - part][rcol][kof]);   // Im X | Re X
-            const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
-            uint4 bM1 = make_uint4(0, 0, 0, 0), bM2 = bM1;
-            if (P == 3) {
-                const uint4 ma = *reinterpret_cast<const uint4*>(&xs[NX - 1][part][rcol][kof]);
-                const uint4 mo = *reinterpret_cast<const uint4*>(&xs[NX - 1][1 - part][rcol][kof]);
-                bM1 = ma;
-                bM2 = part ? mo : neg8(mo);
-            }
 #pragma unroll
-            for (int s = 0; s < kRTC; ++s) {
-                if (s >= nrp) continue;
-                floatx4 a = acc[s][0];
-                a = mfma(zr[s][0], bJ1, a);
-                a = mfma(zi[s][0], bJ2, a);
+            for (int ct = 0; ct < CT; ++ct) {
+                const int rc = ct * 8 + (li & 7);
+                const uint4 xa = *reinterpret_cast<const uint4*>(&xs[0][part][rc][kof]);       // Re X | Im X
+                const uint4 xo = *reinterpret_cast<const uint4*>(&xs[0][1 - part][rc][kof]);   // Im X | Re X
+                const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
+                floatx4 a = acc[ct][0];
+                a = mfma(zr[0], bJ1, a);
+                a = mfma(zi[0], bJ2, a);
                 if (P == 3) {
-                    a = mfma(zr[s][1], neg8(bM1), a);   // - ZK X_M
-                    a = mfma(zi[s][1], neg8(bM2), a);
-                    floatx4 b = acc[s][NO - 1];
-                    b = mfma(zr[s][1], bJ1, b);
-                    b = mfma(zi[s][1], bJ2, b);
-                    b = mfma(zr[s][2], bM1, b);
-                    b = mfma(zi[s][2], bM2, b);
-                    acc[s][NO - 1] = b;
+                    const uint4 ma = *reinterpret_cast<const uint4*>(&xs[NX - 1][part][rc][kof]);
+                    const uint4 mo = *reinterpret_cast<const uint4*>(&xs[NX - 1][1 - part][rc][kof]);
+                    const uint4 bM1 = ma, bM2 = part ? mo : neg8(mo);
+                    a = mfma(zr[1], neg8(bM1), a);   // - ZK X_M
+                    a = mfma(zi[1], neg8(bM2), a);
+                    floatx4 b = acc[ct][NO - 1];
+                    b = mfma(zr[1], bJ1, b);
+                    b = mfma(zi[1], bJ2, b);
+                    b = mfma(zr[2], bM1, b);
+                    b = mfma(zi[2], bM2, b);
+                    acc[ct][NO - 1] = b;
                 }
-                acc[s][0] = a;
+                acc[ct][0] = a;
             }
         }
     }
-    // D[4 lk + v][li]: lanes li < 8 hold Re of right-hand side wave * 8 + li, lane li + 8 its Im
+    if (!wact) return;
+    // D[4 lk + v][li]: lanes li < 8 hold Re of right-hand side ct * 8 + li, lane li + 8 its Im
 #pragma unroll
-    for (int s = 0; s < kRTC; ++s) {
-        if (s >= nrp) continue;
+    for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int o = 0; o < NO; ++o)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float re = acc[s][o][v];
+                const float re = acc[ct][o][v];
                 const float im = __shfl_xor(re, 8, 64);
-                const int row = (rp + s) * 16 + 4 * lk + v;
-                if (part == 0 && rcol < rv && row < nt) {
+                const int row = (rp + wave) * 16 + 4 * lk + v;
+                const int rc = ct * 8 + li;
+                if (part == 0 && rc < rv && row < nt) {
                     const size_t u = (size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0);
-                    y[u * ldr + r0 + rb0 + rcol] = make_double2(re, im);
+                    y[u * ldr + r0 + rb0 + rc] = make_double2(re, im);
                 }
             }
-    }
 }
 
 long long bf16_near_layout(const std::vector<int>& leaf_ptr, const std::vector<int>& tgt_cols, int planes,
@@ -221,18 +236,20 @@ void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd,
     if (nd.nitem == 0 || R < 1) return;
     const size_t total = (size_t)nd.N * (nd.planes == 3 ? 2 : 1) * R;   // unknowns x right-hand sides
     k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
-    const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
+    const int ct = R > 32 ? 8 : (R > 16 ? 4 : (R > 8 ? 2 : 1));
     dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
-#define NB(P, CT) k_near_bf16<P, CT><<<grid, 64 * CT, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
+#define NB(P, CT) k_near_bf16<P, CT><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
+#define NBP(P)                                                                                                        \
+    if (ct == 8) NB(P, 8);                                                                                            \
+    else if (ct == 4) NB(P, 4);                                                                                       \
+    else if (ct == 2) NB(P, 2);                                                                                       \
+    else NB(P, 1);
     if (nd.planes == 1) {
-        if (ct == 4) NB(1, 4);
-        else if (ct == 2) NB(1, 2);
-        else NB(1, 1);
+        NBP(1)
     } else {
-        if (ct == 4) NB(3, 4);
-        else if (ct == 2) NB(3, 2);
-        else NB(3, 1);
+        NBP(3)
     }
+#undef NBP
 #undef NB
 }
 

@@ -1,11 +1,15 @@
 // Batched near-field product on the matrix cores: for every target leaf t,
-// Y_t (nt x R) = Z_t (nt x C_t) X_t (C_t x R) with the FP32 copy of the near
-// field (near_precision = MLFMA_FP32), FP32 accumulation.
+// Y_t (nt x R) = Z_t (nt x C_t) X_t (C_t x R), either with the FP32 copy of
+// the near field (near_precision = MLFMA_FP32; v_mfma_f32_16x16x4_f32, FP32
+// accumulation) or with the FP64 entries (v_mfma_f64_16x16x4_f64, FP64
+// accumulation; the batched FP64 products, e.g. the true residuals of a
+// batched solve).
 // Code/MLFMM.cpp:228-231 (PEC) and :354-366 (PMCHWT: Y_J = Z11 X_J - ZK X_M,
 // Y_M = ZK X_J + Z22 X_M, Code/Zmartix.h:611-614) for many right-hand sides.
 //
-// v_mfma_f32_16x16x4_f32 computes D(16x16) += A(16x4) B(4x16); lane l holds
-// A[l % 16][l / 16], B[l / 16][l % 16] and D[4 (l / 16) + v][l % 16], v < 4.
+// v_mfma_{f32,f64}_16x16x4 compute D(16x16) += A(16x4) B(4x16); lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16], and D[4 (l / 16) + v][l % 16]
+// (FP32) or D[l / 16 + 4 v][l % 16] (FP64), v < 4.
 // A complex product Z X over 8 right-hand sides is two real products with
 // the 16 output columns read as [Re Y (8) | Im Y (8)]:
 //   D += Re Z . [Re X | Im X] + Im Z . [-Im X | Re X].
@@ -15,27 +19,50 @@
 // column tile, so each near-field entry is read from HBM once for up to 32
 // right-hand sides and the wavefronts share it through L1.  The near-field
 // values of the next k-step are loaded while the current one's matrix-core
-// products run.  X is staged in LDS 64 columns at a time, in FP32.
+// products run.  X is staged in LDS 64 (FP32) or 32 (FP64) columns at a time.
 #include "cplx.hpp"
 #include "kernels.hpp"
 
 namespace mlfma {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 namespace {
-constexpr int kCC = 64;   // staged columns per chunk
 constexpr int kRTC = 4;   // row tiles per work item
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    using acc = floatx4;
+    static constexpr int kCC = 64;   // staged columns per chunk
+    __device__ static acc op(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static void stage(const double2& x, float2& v) { narrow(x, v); }
+    __device__ static int row(int lk, int v) { return 4 * lk + v; }   // D row held in element v of lane group lk
+};
+template <>
+struct Mfma<double> {
+    using acc = doublex4;
+    static constexpr int kCC = 32;
+    __device__ static acc op(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static void stage(const double2& x, double2& v) { v = x; }
+    __device__ static int row(int lk, int v) { return lk + 4 * v; }   // the FP64 product interleaves its rows
+};
 }  // namespace
 
-template <int P, int CT>
+template <typename T, int P, int CT>
 __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int r0, int R,
-                                                       const float2* __restrict__ Z, const double2* __restrict__ x,
+                                                       const cplx_t<T>* __restrict__ Z, const double2* __restrict__ x,
                                                        double2* __restrict__ y) {
+    using C2 = cplx_t<T>;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    constexpr int kCC = M::kCC;
     constexpr int NX = (P == 3) ? 2 : 1;   // staged currents
     constexpr int NO = (P == 3) ? 2 : 1;   // outputs
     constexpr int RT = 8 * CT;             // right-hand sides per workgroup
-    __shared__ float2 xs[NX][kCC][RT];
+    __shared__ C2 xs[NX][kCC][RT];
     const int2 item = nd.items[blockIdx.x];
     const int t = item.x, rp = item.y;
     const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
@@ -45,7 +72,7 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
     const int C = nd.tgt_cols[t];
     const int* cref = nd.colref + nd.col_ptr[t];
     const int plane = C * nt;
-    const float2* Zt = Z + nd.tgt_off[t];
+    const C2* Zt = Z + nd.tgt_off[t];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;   // 0: real-part column, 1: imaginary-part column
@@ -54,10 +81,10 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
 #pragma unroll
     for (int s = 0; s < kRTC; ++s) rok[s] = s < nrp && (rp + s) * 16 + li < nt;
     // near-field values of global column gc for this lane's rows
-    auto load = [&](int gc, float2* z1, float2* zk, float2* z2) {
+    auto load = [&](int gc, C2* z1, C2* zk, C2* z2) {
 #pragma unroll
         for (int s = 0; s < kRTC; ++s) {
-            z1[s] = zk[s] = z2[s] = make_float2(0.f, 0.f);
+            z1[s] = zk[s] = z2[s] = C2{0, 0};
             if (rok[s] && gc < C) {
                 const int off = gc * nt + (rp + s) * 16 + li;
                 z1[s] = Zt[off];
@@ -68,62 +95,63 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
             }
         }
     };
-    floatx4 acc[kRTC][NO];
+    acc_t acc[kRTC][NO];
 #pragma unroll
     for (int s = 0; s < kRTC; ++s)
 #pragma unroll
-        for (int o = 0; o < NO; ++o) acc[s][o] = floatx4{0.f, 0.f, 0.f, 0.f};
-    float2 n1[kRTC], nk[kRTC], n2[kRTC];
+        for (int o = 0; o < NO; ++o) acc[s][o] = acc_t{0, 0, 0, 0};
+    C2 n1[kRTC], nk[kRTC], n2[kRTC];
     load(lk, n1, nk, n2);
     for (int c0 = 0; c0 < C; c0 += kCC) {
         __syncthreads();
         for (int e = threadIdx.x; e < NX * kCC * RT; e += 64 * CT) {
             const int xj = e / (kCC * RT), rem = e - xj * (kCC * RT);
             const int c = rem / RT, r = rem - c * RT;
-            float2 v = make_float2(0.f, 0.f);
+            C2 v = C2{0, 0};
             if (c0 + c < C && r < rv) {
                 const size_t u = (size_t)cref[c0 + c] + (xj ? (size_t)nd.N : 0);
-                narrow(x[u * ldr + r0 + rb0 + r], v);
+                M::stage(x[u * ldr + r0 + rb0 + r], v);
             }
-            xs[xj][c][r] = v;
+            xs[xj][c][r].x = v.x;
+            xs[xj][c][r].y = v.y;
         }
         __syncthreads();
         const int kend = min(kCC, C - c0);
         for (int kk = 0; kk < kend; kk += 4) {
-            float2 z1[kRTC], zk[kRTC], z2[kRTC];
+            C2 z1[kRTC], zk[kRTC], z2[kRTC];
 #pragma unroll
             for (int s = 0; s < kRTC; ++s) z1[s] = n1[s], zk[s] = nk[s], z2[s] = n2[s];
             load(c0 + kk + 4 + lk, n1, nk, n2);   // next k-step
             const int cl = kk + lk;                 // this lane's column in the chunk (< kCC); zero-staged past kend
-            const float2 xj = xs[0][cl][rcol];
-            const float bJ1 = part ? xj.y : xj.x, bJ2 = part ? xj.x : -xj.y;
-            float bM1 = 0.f, bM2 = 0.f;
+            const C2 xj = xs[0][cl][rcol];
+            const T bJ1 = part ? xj.y : xj.x, bJ2 = part ? xj.x : -xj.y;
+            T bM1 = 0, bM2 = 0;
             if (P == 3) {
-                const float2 xm = xs[NX - 1][cl][rcol];
+                const C2 xm = xs[NX - 1][cl][rcol];
                 bM1 = part ? xm.y : xm.x;
                 bM2 = part ? xm.x : -xm.y;
             }
 #pragma unroll
             for (int s = 0; s < kRTC; ++s) {
                 if (s >= nrp) continue;
-                floatx4 a = acc[s][0];
-                a = __builtin_amdgcn_mfma_f32_16x16x4f32(z1[s].x, bJ1, a, 0, 0, 0);
-                a = __builtin_amdgcn_mfma_f32_16x16x4f32(z1[s].y, bJ2, a, 0, 0, 0);
+                acc_t a = acc[s][0];
+                a = M::op(z1[s].x, bJ1, a);
+                a = M::op(z1[s].y, bJ2, a);
                 if (P == 3) {
-                    a = __builtin_amdgcn_mfma_f32_16x16x4f32(zk[s].x, -bM1, a, 0, 0, 0);   // - ZK X_M
-                    a = __builtin_amdgcn_mfma_f32_16x16x4f32(zk[s].y, -bM2, a, 0, 0, 0);
-                    floatx4 b = acc[s][NO - 1];
-                    b = __builtin_amdgcn_mfma_f32_16x16x4f32(zk[s].x, bJ1, b, 0, 0, 0);
-                    b = __builtin_amdgcn_mfma_f32_16x16x4f32(zk[s].y, bJ2, b, 0, 0, 0);
-                    b = __builtin_amdgcn_mfma_f32_16x16x4f32(z2[s].x, bM1, b, 0, 0, 0);
-                    b = __builtin_amdgcn_mfma_f32_16x16x4f32(z2[s].y, bM2, b, 0, 0, 0);
+                    a = M::op(zk[s].x, -bM1, a);   // - ZK X_M
+                    a = M::op(zk[s].y, -bM2, a);
+                    acc_t b = acc[s][NO - 1];
+                    b = M::op(zk[s].x, bJ1, b);
+                    b = M::op(zk[s].y, bJ2, b);
+                    b = M::op(z2[s].x, bM1, b);
+                    b = M::op(z2[s].y, bM2, b);
                     acc[s][NO - 1] = b;
                 }
                 acc[s][0] = a;
             }
         }
     }
-    // D[4 lk + v][li]: lanes li < 8 hold Re of right-hand side wave * 8 + li, lane li + 8 its Im
+    // D[row(lk, v)][li]: lanes li < 8 hold Re of right-hand side wave * 8 + li, lane li + 8 its Im
 #pragma unroll
     for (int s = 0; s < kRTC; ++s) {
         if (s >= nrp) continue;
@@ -131,23 +159,24 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
         for (int o = 0; o < NO; ++o)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float re = acc[s][o][v];
-                const float im = __shfl_xor(re, 8, 64);
-                const int row = (rp + s) * 16 + 4 * lk + v;
+                const T re = acc[s][o][v];
+                const T im = __shfl_xor(re, 8, 64);
+                const int row = (rp + s) * 16 + M::row(lk, v);
                 if (part == 0 && rcol < rv && row < nt) {
                     const size_t u = (size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0);
-                    y[u * ldr + r0 + rb0 + rcol] = make_double2(re, im);
+                    y[u * ldr + r0 + rb0 + rcol] = make_double2((double)re, (double)im);
                 }
             }
     }
 }
 
-void launch_near_mfma(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const float2* Z, const double2* x,
-                      double2* y) {
+template <typename T>
+static void near_mfma_t(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const cplx_t<T>* Z,
+                        const double2* x, double2* y) {
     if (nd.nitem == 0 || R < 1) return;
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
-#define NM(P, CT) k_near_mfma<P, CT><<<grid, 64 * CT, 0, st>>>(nd, ldr, r0, R, Z, x, y)
+#define NM(P, CT) k_near_mfma<T, P, CT><<<grid, 64 * CT, 0, st>>>(nd, ldr, r0, R, Z, x, y)
     if (nd.planes == 1) {
         if (ct == 4) NM(1, 4);
         else if (ct == 2) NM(1, 2);
@@ -158,6 +187,16 @@ void launch_near_mfma(hipStream_t st, int R, int ldr, int r0, const NearDev& nd,
         else NM(3, 1);
     }
 #undef NM
+}
+
+void launch_near_mfma(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const float2* Z, const double2* x,
+                      double2* y) {
+    near_mfma_t<float>(st, R, ldr, r0, nd, Z, x, y);
+}
+
+void launch_near_mfma64(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const double2* Z,
+                        const double2* x, double2* y) {
+    near_mfma_t<double>(st, R, ldr, r0, nd, Z, x, y);
 }
 
 }  // namespace mlfma

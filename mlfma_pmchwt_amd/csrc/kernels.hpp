@@ -101,15 +101,19 @@ void launch_near_b(hipStream_t st, int rb, int ldr, int r0, const NearDev& nd, c
 // entries, FP32 accumulation); vectors RHS-fastest with stride ldr from r0.
 void launch_near_mfma(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const float2* Z, const double2* x,
                       double2* y);
+void launch_near_mfma64(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const double2* Z,
+                        const double2* x, double2* y);
 
 // Batched FP32 disaggregation on the matrix cores: effective receive-side
 // matrices E[leaf][rr][o][3][K] from the finest Bm of a pass (constants and
 // kp x B folded in), then y += V E per leaf (o = 0: rows of J / PEC, 1: M).
-void launch_beff(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, const float2* Bm, const float* kp,
-                 double2 ca, double2 cb, double2 cc, double2 cd, float2* E);
+template <typename T>
+void launch_beff(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, const cplx_t<T>* Bm, const T* kp,
+                 double2 ca, double2 cb, double2 cc, double2 cd, cplx_t<T>* E);
+template <typename T>
 void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int rb, const int* order,
-                        const int* leaf_ptr, const int* perm, bool pmchwt, int K1, const float2* V1,
-                        const float2* E1, int K2, const float2* V2, const float2* E2, double2* y);
+                        const int* leaf_ptr, const int* perm, bool pmchwt, int K1, const cplx_t<T>* V1,
+                        const cplx_t<T>* E1, int K2, const cplx_t<T>* V2, const cplx_t<T>* E2, double2* y);
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);

@@ -54,7 +54,7 @@ def case(name):
         mesh = geometry.icosphere(s, r)
         print(f"[case] {name}: icosphere s={s} r={r} ie={ie} eps={eps} bf16 preconditioner", flush=True)
         op = api.Operator(mesh, integral_equation=ie, eps_r_re=eps[0], eps_r_im=eps[1], far_precision=api.FP32,
-                          near_precision=api.FP32, precond_precision=api.BF16, max_rhs=8)
+                          near_precision=api.FP32, precond_precision=api.BF16, max_rhs=40)
         orc = pyoracle.Oracle(mesh, ie=ie, eps=eps)
         _cache[name] = (op, orc, dense_near(orc))
     return _cache[name]
@@ -80,9 +80,9 @@ def test_bf16_step_matches_rounded_restatement(name):
     assert 1e-5 < e_x <= 2e-2          # the BF16 rounding is there, and small
 
 
-@pytest.mark.parametrize("R", [3, 12, 20])
+@pytest.mark.parametrize("R", [3, 12, 20, 40])
 def test_bf16_batched_step(R):
-    """The batched pass (8, 16 and 32 right-hand sides per workgroup) gives the
+    """The batched pass (8, 16, 32 and 64 right-hand sides per workgroup) gives the
     single-vector results column by column."""
     op, orc, An = case("pmchwt_1lvl")
     d = np.diag(An)
