@@ -1081,6 +1081,8 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
 // Krylov column count fixed; its later columns are not used.  Krylov steps run
 // with the apply precision, every restart starts from the true FP64 residual;
 // the polynomial preconditioner is applied in flexible form as in solve().
+// Once a quarter of the block has converged, the rest continues as a narrower
+// block (a recursive call), so converged columns cost no further matvecs.
 void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R,
                          mlfma_solve_stats* stats) {
     const double t0 = wall();
@@ -1311,6 +1313,43 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         for (int r = 0; r < R; ++r)
             if (!done[r] && !(err[r] > 1000.0 * sp.tol_rel)) stalled = false;
         if (stalled && cyc > 8) break;
+        // compaction: once a quarter of the block has converged, the remaining
+        // right-hand sides continue as a narrower block (same recurrences; each
+        // restarts from its own current iterate), so no matvec work is spent on
+        // converged columns
+        std::vector<int> idx;
+        for (int r = 0; r < R; ++r)
+            if (!done[r]) idx.push_back(r);
+        const int Rc = (int)idx.size();
+        if (Rc > 0 && 4 * Rc <= 3 * R && total < sp.itr_max) {
+            int* d_idx = upload(idx);
+            double2* Bc = dalloc<double2>((size_t)n * Rc);
+            double2* Xc = dalloc<double2>((size_t)n * Rc);
+            launch_gather_cols(st, n, R, Rc, d_idx, d_B, Bc);
+            launch_gather_cols(st, n, R, Rc, d_idx, d_X, Xc);
+            mlfma_solve_params sp2 = sp;
+            sp2.itr_max = sp.itr_max - total;
+            if (!pc) sp2.precond = 0;   // a preconditioner dropped above stays dropped
+            const int fb = stats_fallbacks_;
+            std::vector<mlfma_solve_stats> st2(Rc);
+            if (P.verbose) std::fprintf(stderr, "[mlfma] batched gmres: continuing with %d of %d right-hand sides\n", Rc, R);
+            solve_block(sp2, Bc, Xc, Rc, st2.data());
+            launch_scatter_cols(st, n, R, Rc, d_idx, Xc, d_X);
+            MLFMA_HIP(hipStreamSynchronize(st));
+            dfree(Bc);
+            dfree(Xc);
+            dfree(d_idx);
+            stats_fallbacks_ = fb + st2[0].precond_fallbacks;
+            for (int j = 0; j < Rc; ++j) {
+                const int r = idx[j];
+                iters[r] += st2[j].iterations;
+                done[r] = st2[j].converged ? 1 : 0;
+                err[r] = st2[j].rel_residual;
+            }
+            matvecs += st2[0].matvecs * Rc;
+            fp64_matvecs += st2[0].fp64_matvecs * Rc;
+            break;
+        }
     }
     const double secs = wall() - t0;
     for (int r = 0; r < R; ++r) {

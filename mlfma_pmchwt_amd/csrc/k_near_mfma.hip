@@ -16,10 +16,11 @@
 // A workgroup owns one work item (a target leaf's range of up to 4 16-row
 // tiles; items are sorted by decreasing work so large leaves do not set the
 // duration) and CT column tiles of 8 right-hand sides, one wavefront per
-// column tile, so each near-field entry is read from HBM once for up to 32
+// column tile, so each near-field entry is read from HBM once for up to 64
 // right-hand sides and the wavefronts share it through L1.  The near-field
 // values of the next k-step are loaded while the current one's matrix-core
-// products run.  X is staged in LDS 64 (FP32) or 32 (FP64) columns at a time.
+// products run.  X is staged in LDS up to 64 (FP32) or 32 (FP64) columns at a
+// time, fewer for wide batches (the stage stays within 32 KB).
 #include "cplx.hpp"
 #include "kernels.hpp"
 
@@ -58,10 +59,13 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
     using C2 = cplx_t<T>;
     using M = Mfma<T>;
     using acc_t = typename M::acc;
-    constexpr int kCC = M::kCC;
     constexpr int NX = (P == 3) ? 2 : 1;   // staged currents
     constexpr int NO = (P == 3) ? 2 : 1;   // outputs
     constexpr int RT = 8 * CT;             // right-hand sides per workgroup
+    // staged columns per chunk, halved until the stage fits 32 KB of LDS
+    constexpr int kCC = NX * RT * (int)sizeof(C2) * M::kCC <= 32768       ? M::kCC
+                        : NX * RT * (int)sizeof(C2) * M::kCC / 2 <= 32768 ? M::kCC / 2
+                                                                          : M::kCC / 4;
     __shared__ C2 xs[NX][kCC][RT];
     const int2 item = nd.items[blockIdx.x];
     const int t = item.x, rp = item.y;
@@ -174,18 +178,20 @@ template <typename T>
 static void near_mfma_t(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const cplx_t<T>* Z,
                         const double2* x, double2* y) {
     if (nd.nitem == 0 || R < 1) return;
-    const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
+    const int ct = R > 32 ? 8 : (R > 16 ? 4 : (R > 8 ? 2 : 1));
     dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
 #define NM(P, CT) k_near_mfma<T, P, CT><<<grid, 64 * CT, 0, st>>>(nd, ldr, r0, R, Z, x, y)
+#define NMP(P)                                                                                                        \
+    if (ct == 8) NM(P, 8);                                                                                            \
+    else if (ct == 4) NM(P, 4);                                                                                       \
+    else if (ct == 2) NM(P, 2);                                                                                       \
+    else NM(P, 1);
     if (nd.planes == 1) {
-        if (ct == 4) NM(1, 4);
-        else if (ct == 2) NM(1, 2);
-        else NM(1, 1);
+        NMP(1)
     } else {
-        if (ct == 4) NM(3, 4);
-        else if (ct == 2) NM(3, 2);
-        else NM(3, 1);
+        NMP(3)
     }
+#undef NMP
 #undef NM
 }
 

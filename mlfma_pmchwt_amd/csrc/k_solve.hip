@@ -272,6 +272,28 @@ void launch_transpose(hipStream_t st, int rows, int cols, const double2* in, dou
     k_transpose<<<grid, dim3(32, 8), 0, st>>>(rows, cols, in, out);
 }
 
+// Column selection of RHS-fastest blocks (n rows): gather out[u][j] = in[u][idx[j]]
+// (out has Rc columns, in has R), scatter out[u][idx[j]] = in[u][j].
+__global__ void k_cols(size_t total, int R, int Rc, int scatter, const int* __restrict__ idx,
+                       const double2* __restrict__ in, double2* __restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const size_t u = e / Rc;
+    const int j = (int)(e - u * Rc);
+    if (scatter) out[u * R + idx[j]] = in[e];
+    else out[e] = in[u * R + idx[j]];
+}
+
+void launch_gather_cols(hipStream_t st, int n, int R, int Rc, const int* idx, const double2* in, double2* out) {
+    const size_t total = (size_t)n * Rc;
+    if (total) k_cols<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, Rc, 0, idx, in, out);
+}
+
+void launch_scatter_cols(hipStream_t st, int n, int R, int Rc, const int* idx, const double2* in, double2* out) {
+    const size_t total = (size_t)n * Rc;
+    if (total) k_cols<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, Rc, 1, idx, in, out);
+}
+
 // y_t = D_t^{-1} x_t on every leaf; D_t^{-1} dense column-major of order
 // nt (PEC) or 2 nt (PMCHWT, unknowns ordered J then M).
 __global__ __launch_bounds__(256) void k_block_diag(int planes, int N, int max_leaf, const int* __restrict__ leaf_ptr,

@@ -63,6 +63,8 @@ void launch_add_combination_b(hipStream_t st, int n, int R, int m, const double2
 void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, double2* y);
 // out (cols x rows) = transpose of in (rows x cols), both row-major
 void launch_transpose(hipStream_t st, int rows, int cols, const double2* in, double2* out);
+void launch_gather_cols(hipStream_t st, int n, int R, int Rc, const int* idx, const double2* in, double2* out);
+void launch_scatter_cols(hipStream_t st, int n, int R, int Rc, const int* idx, const double2* in, double2* out);
 
 // block-diagonal preconditioner: y_t = D_t^{-1} x_t per leaf (dense inverse blocks)
 void launch_block_diag(hipStream_t st, int planes, int nleaf, int N, const int* leaf_ptr, const int* perm,
