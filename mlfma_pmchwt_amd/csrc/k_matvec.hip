@@ -302,14 +302,24 @@ __global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, c
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int j = 0; j < CPT; ++j) acc[k][j] = C{0, 0};
-    const int u1 = cl_ptr[cl + 1];
-    for (int u = cl_ptr[cl]; u < u1; ++u) {
+    const int u0 = cl_ptr[cl], u1 = cl_ptr[cl + 1];
+    // software pipeline: the next source's values and vector ids are loaded
+    // while the current source is applied
+    C sn[CPT];
+    int4 vn = make_int4(-1, -1, -1, -1);
+    auto fetch = [&](int u) {
         const C* S = Sm + ((size_t)cl_src[u] * NC * 3 + c0) * K + q;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) sn[j] = S[(size_t)j * K];
+        vn = *reinterpret_cast<const int4*>(cl_vec + (size_t)u * 4);
+    };
+    if (u0 < u1) fetch(u0);
+    for (int u = u0; u < u1; ++u) {
         C sv[CPT];
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) sv[j] = S[(size_t)j * K];
-        const int4 vv = *reinterpret_cast<const int4*>(cl_vec + (size_t)u * 4);
-        const int vk[4] = {vv.x, vv.y, vv.z, vv.w};
+        for (int j = 0; j < CPT; ++j) sv[j] = sn[j];
+        const int vk[4] = {vn.x, vn.y, vn.z, vn.w};
+        if (u + 1 < u1) fetch(u + 1);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (vk[k] >= 0) {

@@ -92,9 +92,11 @@ __global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __
 }
 
 // A workgroup owns one work item (a target leaf's range of up to 4 16-row
-// tiles) and CT <= 8 column tiles of 8 right-hand sides.  Wavefront w owns row
-// tile w of the item and every column tile, so each record is read from
-// memory once, by one wavefront, for up to 64 right-hand sides; the next
+// tiles) and CT <= 4 column tiles of 8 right-hand sides (at 64 right-hand
+// sides, two workgroups of 32 with 4 waves per SIMD measured faster than one
+// of 64 with 3).  Wavefront w owns row tile w of the item and every column
+// tile, so each record is read from memory once per workgroup, by one
+// wavefront, for up to 32 right-hand sides; the next
 // 32-column block's records are loaded while the current block's products
 // run.  X is staged in LDS 64 columns at a time as BF16, [Re, Im][rhs][column]
 // rows of 144 B: a thread gathers 8 consecutive columns of one right-hand side
@@ -236,12 +238,11 @@ void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd,
     if (nd.nitem == 0 || R < 1) return;
     const size_t total = (size_t)nd.N * (nd.planes == 3 ? 2 : 1) * R;   // unknowns x right-hand sides
     k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
-    const int ct = R > 32 ? 8 : (R > 16 ? 4 : (R > 8 ? 2 : 1));
+    const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
 #define NB(P, CT) k_near_bf16<P, CT><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
 #define NBP(P)                                                                                                        \
-    if (ct == 8) NB(P, 8);                                                                                            \
-    else if (ct == 4) NB(P, 4);                                                                                       \
+    if (ct == 4) NB(P, 4);                                                                                            \
     else if (ct == 2) NB(P, 2);                                                                                       \
     else NB(P, 1);
     if (nd.planes == 1) {
