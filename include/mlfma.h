@@ -257,6 +257,58 @@ int mlfma_excitation(mlfma_handle* h, double pol_deg, double theta_deg, double p
 int mlfma_rcs(mlfma_handle* h, const double* I, int n_angles, const double* theta_deg, const double* phi_deg,
               double pol_deg, double* out);
 
+/* n_inc plane-wave right-hand sides at once, column-major into v (column j at
+ * v + 2*n*j, interleaved complex); pol/theta/phi per column, as
+ * mlfma_excitation.  The batched form of RHS::computeV_* (Code/Vm.h:150-203)
+ * for a monostatic or multi-incidence run. */
+int mlfma_excitation_batch(mlfma_handle* h, int n_inc, const double* pol_deg, const double* theta_deg,
+                           const double* phi_deg, double* v);
+
+/* Monostatic sweep: for each of n_angles directions the plane wave incident
+ * from it is solved (all directions together in lockstep batched GMRES,
+ * chunks of at most 256; solver must be GMRES) and the backscatter RCS of its
+ * solution taken at the same direction: out3[3*a + {0,1,2}] = total, V, H in
+ * dBsm, stats[a] per direction (may be NULL).  Replaces
+ * RCSUtils::computeMonoStatic / computeMonoStatic_PMCHWT (Code/RCS.h:272-325,
+ * :428-497), which solve the directions one at a time.  Returns
+ * MLFMA_ERR_NOT_CONVERGED if any direction did not converge (out3 written). */
+int mlfma_monostatic(mlfma_handle* h, const mlfma_solve_params* sp, double pol_deg, int n_angles,
+                     const double* theta_deg, const double* phi_deg, double* out3, mlfma_solve_stats* stats);
+
+/* The reference's scattering-angle grid (Code/RCS.h:46-51, :239-246): a theta
+ * sweep th_s + i*step at phi = ph_s, or, when the theta range is empty, a phi
+ * sweep ph_s + i*step at theta = th_s; floor(range / step) + 1 points.  With
+ * theta or phi NULL only *n is set; otherwise *n is the capacity on entry and
+ * the count on return.  Host only. */
+int mlfma_scatter_grid(double th_s, double th_f, double ph_s, double ph_f, double step, int* n, double* theta,
+                       double* phi);
+
+/* ---- Batch runner (replaces BatchConfigParser::parse, BatchConfig::generateTasks
+ * and BatchRunner::run, Code/BatchRunner.h:55-484, on the configuration format
+ * of Code/batch_config.txt).  Tasks expand as mesh x frequency x incidence x
+ * algorithm x material, each with polarisations "h" (pol 0 here) and "v"
+ * (pol 1).  One operator is built per (mesh, frequency, material) and all its
+ * (incidence, polarisation) runs are solved together (dual: one batched GMRES
+ * and the RCS of every run on the scattering grid; mono: one monostatic sweep
+ * per polarisation).  MoM / FMM tasks (algorithm 0 / 1) are not run and report
+ * MLFMA_ERR_ARG.  Results are kept in memory (no files). */
+typedef struct mlfma_batch mlfma_batch;
+int mlfma_batch_load(const char* config_path, mlfma_batch** out);
+int mlfma_batch_load_text(const char* config_text, mlfma_batch** out);
+void mlfma_batch_destroy(mlfma_batch* b);
+int mlfma_batch_count(const mlfma_batch* b, int* n_tasks, int* n_points);
+/* d7 = {frequency, incidence theta, phi, eps_r re, im, mu_r re, im};
+ * i5 = {algorithm, integral equation, matrix solver, gauss points, mono (1) / dual (0)} */
+int mlfma_batch_task(const mlfma_batch* b, int task, double* d7, int* i5, char* nas_path, int nas_capacity);
+int mlfma_batch_grid(const mlfma_batch* b, double* theta_deg, double* phi_deg);
+/* base: operator parameters other than equation, Gauss rule, frequency and
+ * medium (NULL: defaults); sp: NULL for the reference drivers' GMRES(50),
+ * tol_rel 1e-4, tol_abs 1e-8, n iterations (Code/RCS.h:215-219). */
+int mlfma_batch_run(mlfma_batch* b, const mlfma_params* base, const mlfma_solve_params* sp);
+/* rcs3[3*i + {0,1,2}] = total, V, H (dBsm) on the scattering grid; returns the
+ * run's status (MLFMA_ERR_STATE before mlfma_batch_run). */
+int mlfma_batch_result(const mlfma_batch* b, int task, int pol, double* rcs3, mlfma_solve_stats* stats);
+
 /* ------------------------------------------------------------------ stages
  * One MLFMA stage on the handle's device workspaces, for per-stage parity and
  * profiling; mirrors ProcessMLFMM (Code/ProcessMLFMM.h:14-249).

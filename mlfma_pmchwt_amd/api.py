@@ -92,6 +92,18 @@ def library() -> C.CDLL:
         lib.mlfma_solve_batch.argtypes = [vp, C.POINTER(SolveParams), C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_excitation.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         lib.mlfma_rcs.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
+        lib.mlfma_excitation_batch.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        lib.mlfma_monostatic.argtypes = [vp, vp, C.c_double, C.c_int, vp, vp, vp, vp]
+        lib.mlfma_scatter_grid.argtypes = [C.c_double] * 5 + [vp, vp, vp]
+        lib.mlfma_batch_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        lib.mlfma_batch_load_text.argtypes = [C.c_char_p, C.POINTER(vp)]
+        lib.mlfma_batch_destroy.argtypes = [vp]
+        lib.mlfma_batch_destroy.restype = None
+        lib.mlfma_batch_count.argtypes = [vp, vp, vp]
+        lib.mlfma_batch_task.argtypes = [vp, C.c_int, vp, vp, C.c_char_p, C.c_int]
+        lib.mlfma_batch_grid.argtypes = [vp, vp, vp]
+        lib.mlfma_batch_run.argtypes = [vp, vp, vp]
+        lib.mlfma_batch_result.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_stage.argtypes = [vp, C.c_int, vp, vp]
         lib.mlfma_get_workspace.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
         lib.mlfma_set_profiling.argtypes = [vp, C.c_int]
@@ -274,6 +286,28 @@ class Operator:
         self._check(library().mlfma_rcs(self._h, _p(I), th.size, _p(th), _p(ph), pol, _p(out)))
         return out
 
+    def excitation_batch(self, pol, theta, phi) -> np.ndarray:
+        """Plane-wave right-hand sides, one column per (pol, theta, phi)."""
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        pl = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(pol), th.shape), dtype=np.float64)
+        ph = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(phi), th.shape), dtype=np.float64)
+        v = np.empty((th.size, self.n), dtype=np.complex128)   # column-major n x m
+        self._check(library().mlfma_excitation_batch(self._h, th.size, _p(pl), _p(th), _p(ph), _p(v)))
+        return v.T
+
+    def monostatic(self, theta, phi, pol=0.0, restart=50, tol_rel=1e-4, tol_abs=1e-8, itr_max=None, precond=0):
+        """Monostatic RCS (total, V, H dBsm per direction) and per-direction solve statistics."""
+        th = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        ph = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(phi), th.shape), dtype=np.float64)
+        sp = SolveParams(GMRES, itr_max if itr_max is not None else self.n, restart, tol_rel, tol_abs, precond)
+        out = np.empty((th.size, 3))
+        st = (SolveStats * th.size)()
+        rc = library().mlfma_monostatic(self._h, C.byref(sp), pol, th.size, _p(th), _p(ph), _p(out), st)
+        if rc not in (0, 5):
+            self._check(rc)
+        return out, [{"iterations": s.iterations, "converged": bool(s.converged), "rel_residual": s.rel_residual}
+                     for s in st]
+
     def stage(self, which: int, x: np.ndarray | None = None) -> np.ndarray:
         y = np.zeros(self.n, dtype=np.complex128)
         xp = None if x is None else np.ascontiguousarray(x, dtype=np.complex128)
@@ -445,3 +479,71 @@ class Mlp:
         self._check(library().mlfma_mlp_train(self._h, x.shape[0], _p(x), _p(G), _p(h), _p(c), steps, lr,
                                               _p(hist)))
         return hist[:steps]
+
+
+def scatter_grid(theta_start, theta_end, phi_start, phi_end, step):
+    """The reference's scattering-angle grid: (theta, phi) arrays."""
+    n = C.c_int(0)
+    rc = library().mlfma_scatter_grid(theta_start, theta_end, phi_start, phi_end, step, C.byref(n), None, None)
+    if rc != 0:
+        raise MlfmaError(rc, "bad scattering grid")
+    th, ph = np.empty(n.value), np.empty(n.value)
+    library().mlfma_scatter_grid(theta_start, theta_end, phi_start, phi_end, step, C.byref(n), _p(th), _p(ph))
+    return th, ph
+
+
+class Batch:
+    """The reference's batch configuration (Code/batch_config.txt format), run on the GPU."""
+
+    def __init__(self, path=None, text=None):
+        h = C.c_void_p()
+        if text is not None:
+            rc = library().mlfma_batch_load_text(text.encode(), C.byref(h))
+        else:
+            rc = library().mlfma_batch_load(str(path).encode(), C.byref(h))
+        if rc != 0:
+            raise MlfmaError(rc, "cannot read batch configuration")
+        self._h = h
+        nt, npt = C.c_int(0), C.c_int(0)
+        library().mlfma_batch_count(self._h, C.byref(nt), C.byref(npt))
+        self.n_tasks, self.n_points = nt.value, npt.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mlfma_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def task(self, i: int) -> dict:
+        d = np.zeros(7)
+        iv = np.zeros(5, dtype=np.int32)
+        buf = C.create_string_buffer(4096)
+        rc = library().mlfma_batch_task(self._h, i, _p(d), _p(iv), buf, 4096)
+        if rc != 0:
+            raise MlfmaError(rc, "bad task index")
+        return {"nas": buf.value.decode(), "frequency": d[0], "theta": d[1], "phi": d[2], "eps_r": complex(d[3], d[4]),
+                "mu_r": complex(d[5], d[6]), "algorithm": int(iv[0]), "integral_equation": int(iv[1]),
+                "solver": int(iv[2]), "gauss_points": int(iv[3]), "mono": bool(iv[4])}
+
+    def grid(self):
+        th, ph = np.empty(self.n_points), np.empty(self.n_points)
+        library().mlfma_batch_grid(self._h, _p(th), _p(ph))
+        return th, ph
+
+    def run(self, solve: SolveParams | None = None, **params) -> int:
+        """Runs every task on the GPU; params override operator defaults (not equation/frequency/medium)."""
+        base = default_params(**params) if params else None
+        return library().mlfma_batch_run(self._h, None if base is None else C.byref(base),
+                                         None if solve is None else C.byref(solve))
+
+    def result(self, task: int, pol: int):
+        """(status, rcs[n_points, 3], stats) of run (task, pol); pol 0 = "h", 1 = "v"."""
+        out = np.zeros((self.n_points, 3))
+        st = SolveStats()
+        rc = library().mlfma_batch_result(self._h, task, pol, _p(out), C.byref(st))
+        return rc, out, {"iterations": st.iterations, "converged": bool(st.converged), "rel_residual": st.rel_residual}

@@ -1,5 +1,6 @@
 // C ABI of libmlfma.so (declared in include/mlfma.h).
 #include <cstring>
+#include <cmath>
 #include <memory>
 #include <string>
 #include <vector>
@@ -324,6 +325,53 @@ int mlfma_rcs(mlfma_handle* h, const double* I, int nang, const double* th, cons
         MLFMA_HIP(hipMemcpy(h->io.x, I, (size_t)e.nunk * sizeof(double2), hipMemcpyHostToDevice));
         e.rcs(h->io.x, nang, th, ph, pol, out);
     });
+}
+
+int mlfma_excitation_batch(mlfma_handle* h, int n_inc, const double* pol, const double* th, const double* ph,
+                           double* v) {
+    if (!h || n_inc < 1 || !pol || !th || !ph || !v) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        mlfma::Engine& e = *h->eng;
+        const size_t cnt = (size_t)e.nunk * n_inc;
+        h->io.ensure(cnt);
+        e.excitation_batch(n_inc, pol, th, ph, h->io.x);
+        MLFMA_HIP(hipMemcpy(v, h->io.x, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int mlfma_monostatic(mlfma_handle* h, const mlfma_solve_params* sp, double pol, int n_angles, const double* th,
+                     const double* ph, double* out3, mlfma_solve_stats* stats) {
+    if (!h || !sp || n_angles < 1 || !th || !ph || !out3) return MLFMA_ERR_ARG;
+    int conv = 1;
+    const int rc = guarded(h, [&] {
+        if (sp->solver != MLFMA_SOLVER_GMRES) throw mlfma::Error(MLFMA_ERR_ARG, "monostatic sweep: GMRES only");
+        if (sp->itr_max < 0 || sp->restart < 1 || sp->tol_rel < 0) throw mlfma::Error(MLFMA_ERR_ARG, "bad solver parameters");
+        std::vector<mlfma_solve_stats> s(n_angles);
+        h->eng->monostatic(*sp, pol, n_angles, th, ph, out3, s.data());
+        for (int a = 0; a < n_angles; ++a) {
+            if (stats) stats[a] = s[a];
+            conv = conv && s[a].converged;
+        }
+    });
+    if (rc != MLFMA_OK) return rc;
+    return conv ? MLFMA_OK : MLFMA_ERR_NOT_CONVERGED;
+}
+
+int mlfma_scatter_grid(double th_s, double th_f, double ph_s, double ph_f, double step, int* n, double* theta,
+                       double* phi) {
+    if (!n || !(step > 0)) return MLFMA_ERR_ARG;
+    // Code/RCS.h:46-51 (point count) and :239-246 (theta sweep unless the theta range is empty)
+    const bool along_phi = std::abs(th_f - th_s) < 1e-6;
+    const int cnt = (int)(std::abs(along_phi ? ph_f - ph_s : th_f - th_s) / step) + 1;
+    if (theta && phi) {
+        if (*n < cnt) return MLFMA_ERR_ARG;
+        for (int i = 0; i < cnt; ++i) {
+            theta[i] = along_phi ? th_s : th_s + i * step;
+            phi[i] = along_phi ? ph_s + i * step : ph_s;
+        }
+    }
+    *n = cnt;
+    return MLFMA_OK;
 }
 
 int mlfma_stage(mlfma_handle* h, int stage, const double* x, double* y) {

@@ -1641,7 +1641,55 @@ void Engine::excitation(double pol, double th, double ph, double2* d_v) {
     MLFMA_HIP(hipStreamSynchronize(st));
 }
 
+// Right-hand sides of n_inc plane waves into column-major d_V (column j at d_V + j * nunk).
+void Engine::excitation_batch(int n_inc, const double* pol, const double* th, const double* ph, double2* d_V) {
+    for (int j = 0; j < n_inc; ++j) {
+        ExcitationArgs a;
+        double vth[3], vph[3];
+        plane_wave(pol[j], th[j], ph[j], a.kinc, a.einc, a.hinc, vth, vph);
+        a.k = d2(med.k1);
+        a.alpha = P.cfie_alpha;
+        launch_excitation(st, N, ie, d_geo_, d_perm_, gauss_, a, d_V + (size_t)j * nunk);
+    }
+    MLFMA_HIP(hipGetLastError());
+    MLFMA_HIP(hipStreamSynchronize(st));
+}
+
+// Monostatic sweep (Code/RCS.h:272-325 PEC, :428-497 PMCHWT): for every angle the
+// plane wave incident from it, solved for all angles together in batched GMRES
+// (chunks of at most 256), then the backscatter RCS of each solution at its own
+// angle.  out[3 a + {0,1,2}] = total, V, H in dBsm.
+void Engine::monostatic(const mlfma_solve_params& sp, double pol, int nang, const double* th, const double* ph,
+                        double* out, mlfma_solve_stats* stats) {
+    if (nang < 1) throw Error(MLFMA_ERR_ARG, "monostatic: at least one angle");
+    const int chunk = std::min(nang, 256);
+    double2* B = dalloc<double2>((size_t)nunk * chunk);
+    double2* X = dalloc<double2>((size_t)nunk * chunk);
+    double2* Bt = dalloc<double2>((size_t)nunk * chunk);
+    double2* Xt = dalloc<double2>((size_t)nunk * chunk);
+    std::vector<double> pols(chunk, pol);
+    for (int a0 = 0; a0 < nang; a0 += chunk) {
+        const int R = std::min(chunk, nang - a0);
+        excitation_batch(R, pols.data(), th + a0, ph + a0, B);
+        launch_transpose(st, R, nunk, B, Bt);   // column-major -> RHS-fastest
+        MLFMA_HIP(hipMemsetAsync(Xt, 0, (size_t)nunk * R * sizeof(double2), st));
+        solve_block(sp, Bt, Xt, R, stats + a0);
+        launch_transpose(st, nunk, R, Xt, X);
+        rcs_multi(X, (size_t)nunk, R, th + a0, ph + a0, pol, out + 3 * (size_t)a0);
+    }
+    dfree(B);
+    dfree(X);
+    dfree(Bt);
+    dfree(Xt);
+}
+
 void Engine::rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out) {
+    rcs_multi(d_I, 0, nang, th, ph, pol, out);
+}
+
+// RCS at nang angles; angle a uses the current vector d_I + a * istride.
+void Engine::rcs_multi(const double2* d_I, size_t istride, int nang, const double* th, const double* ph, double pol,
+                       double* out) {
     std::vector<double> dirs(3 * (size_t)nang);
     for (int a = 0; a < nang; ++a) {
         const double t = th[a] / 180.0 * kPi, p = ph[a] / 180.0 * kPi;
@@ -1655,7 +1703,8 @@ void Engine::rcs(const double2* d_I, int nang, const double* th, const double* p
     MLFMA_HIP(hipMalloc(&d_dirs, dirs.size() * sizeof(double)));
     MLFMA_HIP(hipMalloc(&d_part, (size_t)nang * nblk * 3 * sizeof(double2)));
     MLFMA_HIP(hipMemcpy(d_dirs, dirs.data(), dirs.size() * sizeof(double), hipMemcpyHostToDevice));
-    launch_farfield(st, N, ie == MLFMA_PMCHWT, d_geo_, d_perm_, gauss_, d2(med.k1), nang, d_dirs, d_I, d_part, nblk);
+    launch_farfield(st, N, ie == MLFMA_PMCHWT, d_geo_, d_perm_, gauss_, d2(med.k1), nang, d_dirs, d_I, istride, d_part,
+                    nblk);
     std::vector<double2> part((size_t)nang * nblk * 3);
     MLFMA_HIP(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
     MLFMA_HIP(hipStreamSynchronize(st));

@@ -103,6 +103,11 @@ public:
     void precond_statistics(const double2* d_b, int steps, int degree, double2* G, double2* h, double* c);
     void excitation(double pol, double th, double ph, double2* d_v);
     void rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out);
+    void rcs_multi(const double2* d_I, size_t istride, int nang, const double* th, const double* ph, double pol,
+                   double* out);
+    void excitation_batch(int n_inc, const double* pol, const double* th, const double* ph, double2* d_V);
+    void monostatic(const mlfma_solve_params& sp, double pol, int nang, const double* th, const double* ph,
+                    double* out, mlfma_solve_stats* stats);
 
     // per-group kernel timing with hipEvents on st
     void set_profiling(bool on);

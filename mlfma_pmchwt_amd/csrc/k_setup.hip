@@ -410,9 +410,10 @@ void launch_excitation(hipStream_t st, int N, int mode, const RwgGeo* geo, const
 __global__ __launch_bounds__(256) void k_farfield(int N, int pmchwt, const RwgGeo* __restrict__ geo,
                                                   const int* __restrict__ perm, GaussRule g, double2 k,
                                                   const double* __restrict__ dirs, const double2* __restrict__ I,
-                                                  double2* __restrict__ partial) {
+                                                  size_t istride, double2* __restrict__ partial) {
     __shared__ double2 red[3][4];
     const int ang = blockIdx.y;
+    I += (size_t)ang * istride;   // monostatic: every angle has its own current vector
     const R3 rd = {dirs[3 * ang], dirs[3 * ang + 1], dirs[3 * ang + 2]};
     double2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,9 +461,10 @@ __global__ __launch_bounds__(256) void k_farfield(int N, int pmchwt, const RwgGe
 }
 
 void launch_farfield(hipStream_t st, int N, int pmchwt, const RwgGeo* geo, const int* perm, const GaussRule& g,
-                     double2 k, int nang, const double* dirs, const double2* I, double2* partial, int nblocks) {
+                     double2 k, int nang, const double* dirs, const double2* I, size_t istride, double2* partial,
+                     int nblocks) {
     dim3 grid(nblocks, nang);
-    k_farfield<<<grid, 256, 0, st>>>(N, pmchwt, geo, perm, g, k, dirs, I, partial);
+    k_farfield<<<grid, 256, 0, st>>>(N, pmchwt, geo, perm, g, k, dirs, I, istride, partial);
 }
 
 }  // namespace mlfma

@@ -27,8 +27,10 @@ void launch_patterns(hipStream_t st, bool cfie, int N, int K, const RwgGeo* geo,
                      double2* Vf);
 void launch_excitation(hipStream_t st, int N, int mode, const RwgGeo* geo, const int* perm, const GaussRule& g,
                        const ExcitationArgs& a, double2* v);
+// istride: offset between the current vectors of consecutive angles (0: one vector for all angles)
 void launch_farfield(hipStream_t st, int N, int pmchwt, const RwgGeo* geo, const int* perm, const GaussRule& g,
-                     double2 k, int nang, const double* dirs, const double2* I, double2* partial, int nblocks);
+                     double2 k, int nang, const double* dirs, const double2* I, size_t istride, double2* partial,
+                     int nblocks);
 
 // Krylov vector kernels (k_solve.hip)
 // partial[j*nblk + b] = sum over block b of conj(V_j) . w, j < m
