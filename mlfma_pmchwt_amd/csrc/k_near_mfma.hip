@@ -15,11 +15,11 @@
 //   D += Re Z . [Re X | Im X] + Im Z . [-Im X | Re X].
 // A workgroup owns one work item (a target leaf's range of up to 4 16-row
 // tiles; items are sorted by decreasing work so large leaves do not set the
-// duration) and CT column tiles of 8 right-hand sides, one wavefront per
-// column tile, so each near-field entry is read from HBM once for up to 64
-// right-hand sides and the wavefronts share it through L1.  The near-field
-// values of the next k-step are loaded while the current one's matrix-core
-// products run.  X is staged in LDS up to 64 (FP32) or 32 (FP64) columns at a
+// duration) and CT column tiles of 8 right-hand sides (up to 64 in FP32, 32 in
+// FP64).  Wavefront w owns row tile w and every column tile, so each
+// near-field entry is loaded once, by one wavefront, and the right-hand sides
+// come from LDS.  The near-field values of the next k-step are loaded while
+// the current one's matrix-core products run.  X is staged in LDS up to 64 (FP32) or 32 (FP64) columns at a
 // time, fewer for wide batches (the stage stays within 32 KB).
 #include "cplx.hpp"
 #include "kernels.hpp"
@@ -53,9 +53,9 @@ struct Mfma<double> {
 }  // namespace
 
 template <typename T, int P, int CT>
-__global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int r0, int R,
-                                                       const cplx_t<T>* __restrict__ Z, const double2* __restrict__ x,
-                                                       double2* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_near_mfma(NearDev nd, int ldr, int r0, int R,
+                                                   const cplx_t<T>* __restrict__ Z, const double2* __restrict__ x,
+                                                   double2* __restrict__ y) {
     using C2 = cplx_t<T>;
     using M = Mfma<T>;
     using acc_t = typename M::acc;
@@ -80,35 +80,31 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;   // 0: real-part column, 1: imaginary-part column
-    const int rcol = wave * 8 + (li & 7);
-    bool rok[kRTC];
-#pragma unroll
-    for (int s = 0; s < kRTC; ++s) rok[s] = s < nrp && (rp + s) * 16 + li < nt;
-    // near-field values of global column gc for this lane's rows
-    auto load = [&](int gc, C2* z1, C2* zk, C2* z2) {
-#pragma unroll
-        for (int s = 0; s < kRTC; ++s) {
-            z1[s] = zk[s] = z2[s] = C2{0, 0};
-            if (rok[s] && gc < C) {
-                const int off = gc * nt + (rp + s) * 16 + li;
-                z1[s] = Zt[off];
-                if (P == 3) {
-                    zk[s] = Zt[plane + off];
-                    z2[s] = Zt[2 * plane + off];
-                }
+    const bool wact = wave < nrp;              // this wavefront's row tile exists
+    const int arow = (rp + wave) * 16 + li;    // this lane's row of the A operand
+    const bool rok = wact && arow < nt;
+    // near-field values of global column gc for this lane's row
+    auto load = [&](int gc, C2& z1, C2& zk, C2& z2) {
+        z1 = zk = z2 = C2{0, 0};
+        if (rok && gc < C) {
+            const int off = gc * nt + arow;
+            z1 = Zt[off];
+            if (P == 3) {
+                zk = Zt[plane + off];
+                z2 = Zt[2 * plane + off];
             }
         }
     };
-    acc_t acc[kRTC][NO];
+    acc_t acc[CT][NO];
 #pragma unroll
-    for (int s = 0; s < kRTC; ++s)
+    for (int c = 0; c < CT; ++c)
 #pragma unroll
-        for (int o = 0; o < NO; ++o) acc[s][o] = acc_t{0, 0, 0, 0};
-    C2 n1[kRTC], nk[kRTC], n2[kRTC];
+        for (int o = 0; o < NO; ++o) acc[c][o] = acc_t{0, 0, 0, 0};
+    C2 n1, nk, n2;
     load(lk, n1, nk, n2);
     for (int c0 = 0; c0 < C; c0 += kCC) {
         __syncthreads();
-        for (int e = threadIdx.x; e < NX * kCC * RT; e += 64 * CT) {
+        for (int e = threadIdx.x; e < NX * kCC * RT; e += 256) {
             const int xj = e / (kCC * RT), rem = e - xj * (kCC * RT);
             const int c = rem / RT, r = rem - c * RT;
             C2 v = C2{0, 0};
@@ -122,67 +118,63 @@ __global__ __launch_bounds__(64 * CT) void k_near_mfma(NearDev nd, int ldr, int 
         __syncthreads();
         const int kend = min(kCC, C - c0);
         for (int kk = 0; kk < kend; kk += 4) {
-            C2 z1[kRTC], zk[kRTC], z2[kRTC];
-#pragma unroll
-            for (int s = 0; s < kRTC; ++s) z1[s] = n1[s], zk[s] = nk[s], z2[s] = n2[s];
+            const C2 z1 = n1, zk = nk, z2 = n2;
             load(c0 + kk + 4 + lk, n1, nk, n2);   // next k-step
-            const int cl = kk + lk;                 // this lane's column in the chunk (< kCC); zero-staged past kend
-            const C2 xj = xs[0][cl][rcol];
-            const T bJ1 = part ? xj.y : xj.x, bJ2 = part ? xj.x : -xj.y;
-            T bM1 = 0, bM2 = 0;
-            if (P == 3) {
-                const C2 xm = xs[NX - 1][cl][rcol];
-                bM1 = part ? xm.y : xm.x;
-                bM2 = part ? xm.x : -xm.y;
-            }
+            if (!wact) continue;
+            const int cl = kk + lk;   // this lane's column in the chunk (< kCC); zero-staged past kend
 #pragma unroll
-            for (int s = 0; s < kRTC; ++s) {
-                if (s >= nrp) continue;
-                acc_t a = acc[s][0];
-                a = M::op(z1[s].x, bJ1, a);
-                a = M::op(z1[s].y, bJ2, a);
+            for (int ct = 0; ct < CT; ++ct) {
+                const int rc = ct * 8 + (li & 7);
+                const C2 xj = xs[0][cl][rc];
+                const T bJ1 = part ? xj.y : xj.x, bJ2 = part ? xj.x : -xj.y;
+                acc_t a = acc[ct][0];
+                a = M::op(z1.x, bJ1, a);
+                a = M::op(z1.y, bJ2, a);
                 if (P == 3) {
-                    a = M::op(zk[s].x, -bM1, a);   // - ZK X_M
-                    a = M::op(zk[s].y, -bM2, a);
-                    acc_t b = acc[s][NO - 1];
-                    b = M::op(zk[s].x, bJ1, b);
-                    b = M::op(zk[s].y, bJ2, b);
-                    b = M::op(z2[s].x, bM1, b);
-                    b = M::op(z2[s].y, bM2, b);
-                    acc[s][NO - 1] = b;
+                    const C2 xm = xs[NX - 1][cl][rc];
+                    const T bM1 = part ? xm.y : xm.x, bM2 = part ? xm.x : -xm.y;
+                    a = M::op(zk.x, -bM1, a);   // - ZK X_M
+                    a = M::op(zk.y, -bM2, a);
+                    acc_t b = acc[ct][NO - 1];
+                    b = M::op(zk.x, bJ1, b);
+                    b = M::op(zk.y, bJ2, b);
+                    b = M::op(z2.x, bM1, b);
+                    b = M::op(z2.y, bM2, b);
+                    acc[ct][NO - 1] = b;
                 }
-                acc[s][0] = a;
+                acc[ct][0] = a;
             }
         }
     }
-    // D[row(lk, v)][li]: lanes li < 8 hold Re of right-hand side wave * 8 + li, lane li + 8 its Im
+    if (!wact) return;
+    // D[row(lk, v)][li]: lanes li < 8 hold Re of right-hand side ct * 8 + li, lane li + 8 its Im
 #pragma unroll
-    for (int s = 0; s < kRTC; ++s) {
-        if (s >= nrp) continue;
+    for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int o = 0; o < NO; ++o)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const T re = acc[s][o][v];
+                const T re = acc[ct][o][v];
                 const T im = __shfl_xor(re, 8, 64);
-                const int row = (rp + s) * 16 + M::row(lk, v);
-                if (part == 0 && rcol < rv && row < nt) {
+                const int row = (rp + wave) * 16 + M::row(lk, v);
+                const int rc = ct * 8 + li;
+                if (part == 0 && rc < rv && row < nt) {
                     const size_t u = (size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0);
-                    y[u * ldr + r0 + rb0 + rcol] = make_double2((double)re, (double)im);
+                    y[u * ldr + r0 + rb0 + rc] = make_double2((double)re, (double)im);
                 }
             }
-    }
 }
 
 template <typename T>
 static void near_mfma_t(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const cplx_t<T>* Z,
                         const double2* x, double2* y) {
     if (nd.nitem == 0 || R < 1) return;
-    const int ct = R > 32 ? 8 : (R > 16 ? 4 : (R > 8 ? 2 : 1));
+    constexpr bool f64 = sizeof(T) == 8;   // FP64 accumulators: at most 32 right-hand sides per workgroup
+    const int ct = (R > 32 && !f64) ? 8 : (R > 16 ? 4 : (R > 8 ? 2 : 1));
     dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
-#define NM(P, CT) k_near_mfma<T, P, CT><<<grid, 64 * CT, 0, st>>>(nd, ldr, r0, R, Z, x, y)
+#define NM(P, CT) k_near_mfma<T, P, CT><<<grid, 256, 0, st>>>(nd, ldr, r0, R, Z, x, y)
 #define NMP(P)                                                                                                        \
-    if (ct == 8) NM(P, 8);                                                                                            \
+    if (ct == 8 && !f64) NM(P, 8);                                                                                    \
     else if (ct == 4) NM(P, 4);                                                                                       \
     else if (ct == 2) NM(P, 2);                                                                                       \
     else NM(P, 1);
