@@ -130,3 +130,26 @@ def test_bf16_preconditioned_solves():
         print(f"[bf16 solve] {inc[j]} bf16 it={st[j]['iterations']} fp32 it={st32[j]['iterations']}")
         assert st[j]["iterations"] <= 1.15 * st32[j]["iterations"] + 3
     ref.close()
+
+
+def test_bf16_graph_replay_matches_eager():
+    """The recorded (hipGraph) preconditioned batched matvec with BF16 products
+    gives the eager solve's iterations and solutions (every Arnoldi step is a
+    replay of the same recording)."""
+    op, orc, An = case("pmchwt_1lvl")
+    op.set_apply_precision(api.FP64)
+    G, h, c = op.precond_statistics(op.excitation(0.0, 0.0, 0.0), steps=30, degree=learned.DEGREE)
+    op.set_precond_poly(learned.fit_coefficients(G, h, c))
+    op.set_apply_precision(api.FP32)
+    B = np.stack([op.excitation(0.0, th, 0.0) for th in (15.0, 75.0, 135.0, 165.0)], axis=1)
+    runs = {}
+    for graphs in (False, True):
+        op.set_graphs(graphs)
+        runs[graphs] = op.solve_batch(B, tol_rel=1e-4, restart=30, itr_max=2000, precond=api.PRECOND_POLY)
+    op.set_graphs(False)
+    (Xe, se), (Xg, sg) = runs[False], runs[True]
+    for j in range(B.shape[1]):
+        print(f"[bf16 graph] column {j}: eager it={se[j]['iterations']} graph it={sg[j]['iterations']}")
+        assert se[j]["converged"] and sg[j]["converged"]
+        assert se[j]["iterations"] == sg[j]["iterations"]
+    assert relerr(Xg, Xe) <= 1e-12
