@@ -305,6 +305,10 @@ int mlfma_batch_grid(const mlfma_batch* b, double* theta_deg, double* phi_deg);
  * medium (NULL: defaults); sp: NULL for the reference drivers' GMRES(50),
  * tol_rel 1e-4, tol_abs 1e-8, n iterations (Code/RCS.h:215-219). */
 int mlfma_batch_run(mlfma_batch* b, const mlfma_params* base, const mlfma_solve_params* sp);
+/* Objects (operator groups) processed concurrently by n <= 16 host threads, each
+ * with its own operator, stream and graphs (default 1: one after another).
+ * Results do not depend on n; device memory holds n operators at a time. */
+int mlfma_batch_set_concurrency(mlfma_batch* b, int n);
 /* rcs3[3*i + {0,1,2}] = total, V, H (dBsm) on the scattering grid; returns the
  * run's status (MLFMA_ERR_STATE before mlfma_batch_run). */
 int mlfma_batch_result(const mlfma_batch* b, int task, int pol, double* rcs3, mlfma_solve_stats* stats);

@@ -104,6 +104,7 @@ def library() -> C.CDLL:
         lib.mlfma_batch_grid.argtypes = [vp, vp, vp]
         lib.mlfma_batch_run.argtypes = [vp, vp, vp]
         lib.mlfma_batch_result.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.mlfma_batch_set_concurrency.argtypes = [vp, C.c_int]
         lib.mlfma_stage.argtypes = [vp, C.c_int, vp, vp]
         lib.mlfma_get_workspace.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
         lib.mlfma_set_profiling.argtypes = [vp, C.c_int]
@@ -535,8 +536,12 @@ class Batch:
         library().mlfma_batch_grid(self._h, _p(th), _p(ph))
         return th, ph
 
-    def run(self, solve: SolveParams | None = None, **params) -> int:
-        """Runs every task on the GPU; params override operator defaults (not equation/frequency/medium)."""
+    def run(self, solve: SolveParams | None = None, concurrency: int = 1, **params) -> int:
+        """Runs every task on the GPU; params override operator defaults (not equation/frequency/medium);
+        concurrency: objects processed at once by that many host threads."""
+        rc = library().mlfma_batch_set_concurrency(self._h, concurrency)
+        if rc != 0:
+            raise MlfmaError(rc, "concurrency must be 1..16")
         base = default_params(**params) if params else None
         return library().mlfma_batch_run(self._h, None if base is None else C.byref(base),
                                          None if solve is None else C.byref(solve))

@@ -18,15 +18,21 @@
 // polarisation per group (the reference's mono mode ignores the incidence
 // angle, so every mono task of a group has the same result).  MoM (0) and FMM
 // (1) tasks are not run here (status MLFMA_ERR_ARG); CGS tasks are solved one
-// right-hand side at a time.  Results stay in memory; file export is the
-// caller's.
+// right-hand side at a time.  With a concurrency above 1, that many host
+// threads take groups (objects) from a shared counter, each with its own
+// operator, stream and recorded graphs, so objects of uneven size share the
+// GPU instead of running one after another; results do not depend on the
+// concurrency.  Results stay in memory; file export is the caller's.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -83,6 +89,7 @@ struct mlfma_batch {
     std::vector<Task> tasks;
     std::vector<double> grid_th, grid_ph;
     std::vector<RunResult> res;   // [task][pol], pol 0 = "h", 1 = "v"
+    int concurrency = 1;
     std::string err;
 
     void parse(std::istream& in) {
@@ -296,9 +303,10 @@ int mlfma_batch_run(mlfma_batch* b, const mlfma_params* base, const mlfma_solve_
         if (!groups.count(k)) order.push_back(k);
         groups[k].push_back((int)i);
     }
-    int worst = MLFMA_OK;
-    for (const Key& k : order) {
-        const std::vector<int>& ids = groups[k];
+    // one object (operator group) per call; returns its worst status
+    auto run_group = [&](const Key& k) -> int {
+        int worst = MLFMA_OK;
+        const std::vector<int>& ids = groups.at(k);
         const Task& t0 = b->tasks[ids[0]];
         mlfma_params p;
         if (base) p = *base;
@@ -317,8 +325,7 @@ int mlfma_batch_run(mlfma_batch* b, const mlfma_params* base, const mlfma_solve_
         int rc = mlfma_create_nas(&p, t0.nas.c_str(), &h);
         if (rc != MLFMA_OK) {
             for (int i : ids) b->res[2 * i].status = b->res[2 * i + 1].status = rc;
-            worst = rc;
-            continue;
+            return rc;
         }
         long long dims[6];
         mlfma_dims(h, dims);
@@ -403,8 +410,35 @@ int mlfma_batch_run(mlfma_batch* b, const mlfma_params* base, const mlfma_solve_
             }
         }
         mlfma_destroy(h);
+        return worst;
+    };
+    int worst = MLFMA_OK;
+    std::mutex mu;
+    std::atomic<int> next{0};
+    auto worker = [&] {
+        for (int g = next++; g < (int)order.size(); g = next++) {
+            const int rc = run_group(order[g]);
+            if (rc != MLFMA_OK) {
+                std::lock_guard<std::mutex> lock(mu);
+                worst = rc;
+            }
+        }
+    };
+    const int nthreads = std::min(b->concurrency, (int)order.size());
+    if (nthreads <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; ++t) pool.emplace_back(worker);
+        for (auto& th : pool) th.join();
     }
     return worst;
+}
+
+int mlfma_batch_set_concurrency(mlfma_batch* b, int n) {
+    if (!b || n < 1 || n > 16) return MLFMA_ERR_ARG;
+    b->concurrency = n;
+    return MLFMA_OK;
 }
 
 int mlfma_batch_result(const mlfma_batch* b, int task, int pol, double* rcs3, mlfma_solve_stats* stats) {

@@ -112,3 +112,38 @@ mono_dual = mono
             assert np.max(np.abs(rcs - ref)) < 1e-6
     op.close()
     b.close()
+
+
+def test_batch_concurrent_objects_equal_sequential(tmp_path):
+    """Objects of uneven size (four spheres, two materials: eight operators)
+    processed by four host threads at once give the one-after-another results:
+    each object has its own operator, stream and recordings, so the threads
+    share nothing but the GPU."""
+    paths = []
+    for k, (s, r) in enumerate([(1, 0.3), (2, 0.4), (2, 0.6), (3, 0.5)]):
+        p = tmp_path / f"obj{k}.nas"
+        geometry.write_nas(p, geometry.icosphere(s, r))
+        paths.append(str(p))
+    cfg = f"""nas_files = {'; '.join(paths)}
+frequencies = 299792458
+incident_angles = 0:0, 60:30, 120:90
+material_list = 4.0,-0.1;2.5,0.0
+sca_theta = 0:180
+sca_phi = 0:90
+step = 30
+integral_equation = 2
+"""
+    out = {}
+    for conc in (1, 4):
+        b = api.Batch(text=cfg)
+        print(f"[batch concurrency] {conc} threads over {b.n_tasks} tasks", flush=True)
+        assert b.run(api.SolveParams(api.GMRES, 2000, 50, 1e-5, 0.0, 0), concurrency=conc) == 0
+        out[conc] = [[b.result(i, pol) for pol in (0, 1)] for i in range(b.n_tasks)]
+        b.close()
+    for i, (seq, par) in enumerate(zip(out[1], out[4])):
+        for (s1, r1, t1), (s4, r4, t4) in zip(seq, par):
+            assert s1 == 0 and s4 == 0
+            assert t1["iterations"] == t4["iterations"]
+            assert np.max(np.abs(r1 - r4)) < 1e-9, i
+    with pytest.raises(api.MlfmaError):
+        api.Batch(text=cfg).run(concurrency=0)
