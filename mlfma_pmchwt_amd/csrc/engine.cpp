@@ -610,8 +610,12 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     const bool mfma_near = R >= 2 && near_mfma_;
     if (mfma_near) {
         if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
-        if (prec == MLFMA_FP32 && d_Z32_) launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
-        else launch_near_mfma64(st, R, R, 0, nd_, d_Z_, X, Y);
+        if (prec == MLFMA_FP32 && d_Z32_) {
+            ensure_xf(R);
+            launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, d_xf_, Y);
+        } else {
+            launch_near_mfma64(st, R, R, 0, nd_, d_Z_, X, Y);
+        }
         if (prof_on_) {
             MLFMA_HIP(hipEventRecord(ev_[17], st));
             MLFMA_HIP(hipEventSynchronize(ev_[17]));
@@ -649,6 +653,7 @@ void Engine::apply_device(const double2* x, double2* y, int nrhs) {
 
 void Engine::ensure_block(int R) {
     if (R > 256) throw Error(MLFMA_ERR_ARG, "at most 256 right-hand sides per call");
+    ensure_xf(R);
     if (R <= block_cap_) return;
     clear_graphs();
     dfree(d_bx_);
@@ -1471,7 +1476,18 @@ void Engine::set_poly(const std::vector<cplx>& alpha) {
     clear_graphs();   // the coefficients are kernel arguments of recorded sequences
 }
 
+// FP32 copy of the matrix-core near product's input (sized before any
+// recording: the recorded sequences hold its address)
+void Engine::ensure_xf(int R) {
+    if (!d_Z32_ || R <= xf_cap_) return;
+    clear_graphs();
+    dfree(d_xf_);
+    d_xf_ = dalloc<float2>((size_t)nunk * R);
+    xf_cap_ = R;
+}
+
 void Engine::ensure_poly(int R) {
+    ensure_xf(R);
     if (R <= poly_cap_) return;
     clear_graphs();
     dfree(d_pt_);
@@ -1494,8 +1510,12 @@ void Engine::near_block(const double2* X, double2* Y, int R, int prec, bool prec
         return;
     }
     if (R >= 2 && near_mfma_) {
-        if (prec == MLFMA_FP32 && d_Z32_) launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, Y);
-        else launch_near_mfma64(st, R, R, 0, nd_, d_Z_, X, Y);
+        if (prec == MLFMA_FP32 && d_Z32_) {
+            ensure_xf(R);
+            launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, d_xf_, Y);
+        } else {
+            launch_near_mfma64(st, R, R, 0, nd_, d_Z_, X, Y);
+        }
         return;
     }
     const int cap = (ie == MLFMA_PMCHWT) ? 4 : 8;

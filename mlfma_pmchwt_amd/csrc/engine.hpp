@@ -183,6 +183,7 @@ private:
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
     void ensure_poly(int R);
+    void ensure_xf(int R);
     void near_block(const double2* X, double2* Y, int R, int prec, bool precond = false);
     double norm(const double2* v, int n);
 
@@ -201,6 +202,8 @@ private:
     float2* d_Z32_ = nullptr;   // FP32 copy (near_precision = MLFMA_FP32)
     uint4* d_Zbf_ = nullptr;    // BF16 record copy for the preconditioner (precond_precision = MLFMA_BF16)
     long long* d_bfoff_ = nullptr;
+    float2* d_xf_ = nullptr;    // FP32 copy of the matrix-core near product's input, nunk x xf_cap_
+    int xf_cap_ = 0;
     unsigned* d_xb_ = nullptr;  // BF16 copy of the preconditioner's near-product input, nunk x poly_cap_
     // RHS-fastest block vectors of the batched matvec
     double2* d_bx_ = nullptr;

@@ -94,7 +94,7 @@ __global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __
 // A workgroup owns one work item (a target leaf's range of up to 4 16-row
 // tiles) and CT <= 4 column tiles of 8 right-hand sides (at 64 right-hand
 // sides, two workgroups of 32 with 4 waves per SIMD measured faster than one
-// of 64 with 3).  Wavefront w owns row tile w of the item and every column
+// of 64 with 3; the two share the item's records through L2).  Wavefront w owns row tile w of the item and every column
 // tile, so each record is read from memory once per workgroup, by one
 // wavefront, for up to 32 right-hand sides; the next
 // 32-column block's records are loaded while the current block's products
@@ -110,12 +110,19 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     constexpr int NO = (P == 3) ? 2 : 1;   // outputs
     constexpr int RT = 8 * CT;             // right-hand sides per workgroup
     __shared__ __align__(16) unsigned short xs[NX][2][RT][kKP];   // [current][Re, Im][rhs][column]
-    const int2 item = nd.items[blockIdx.x];
+    // Blocks go round-robin to the 8 XCDs: the ny right-hand-side groups of one
+    // item are 8 blocks apart, so they run on one XCD at about the same time
+    // and the later groups read the item's records from that XCD's L2.
+    const int ny = (R + RT - 1) / RT;
+    const int slot = blockIdx.x / (8 * ny), j8 = blockIdx.x - slot * 8 * ny;
+    const int it = slot * 8 + (j8 & 7);
+    if (it >= nd.nitem) return;
+    const int2 item = nd.items[it];
     const int t = item.x, rp = item.y;
     const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
     const int nrt = (nt + 15) >> 4;
     const int nrp = min(kRTC, nrt - rp);
-    const int rb0 = blockIdx.y * RT;
+    const int rb0 = (j8 >> 3) * RT;
     const int rv = min(RT, R - rb0);
     const int C = nd.tgt_cols[t];
     const int nkb = (C + 31) >> 5;
@@ -126,15 +133,15 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     const bool wact = wave < nrp;      // this wavefront's row tile exists
     const uint4* zrec = Zb + bf_off[t] + (size_t)(rp + (wact ? wave : 0)) * P * 128 + lane;
     const size_t kstride = (size_t)nrt * P * 128;   // records of one 32-column block
+    // records of block kb (clamped to the last block: a block past nkb is
+    // loaded but never used), without bounds branches, which would make the
+    // compiler wait for the prefetched records before the current products
     auto load = [&](int kb, uint4* zr, uint4* zi) {
+        const uint4* rec = zrec + (size_t)min(kb, nkb - 1) * kstride;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            zr[p] = zi[p] = make_uint4(0, 0, 0, 0);
-            if (wact && kb < nkb) {
-                const uint4* rec = zrec + (size_t)kb * kstride + p * 128;
-                zr[p] = rec[0];
-                zi[p] = rec[64];
-            }
+            zr[p] = rec[p * 128];
+            zi[p] = rec[p * 128 + 64];
         }
     };
     floatx4 acc[CT][NO];
@@ -142,60 +149,107 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     for (int c = 0; c < CT; ++c)
 #pragma unroll
         for (int o = 0; o < NO; ++o) acc[c][o] = floatx4{0.f, 0.f, 0.f, 0.f};
-    uint4 nr[P], ni[P];
-    load(0, nr, ni);
+    // X staging, one chunk ahead: a thread's units (current xj, 8-column group
+    // cg, right-hand side r) load the next chunk's X into registers while the
+    // current chunk's products run, and the source indices two chunks ahead.
+    constexpr int NE = NX * 8 * RT, NU = (NE + 255) / 256;
+    constexpr bool kShare = NU == 2 && NE == 512;   // both units have the same (cg, r)
+    constexpr int NCR = kShare ? 1 : NU;
+    int ucg[NU], ur[NU], uxj[NU];
+    bool uok[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int e = threadIdx.x + 256 * u, ec = min(e, NE - 1);   // spare threads repeat the last unit
+        uok[u] = e < NE;
+        uxj[u] = ec / (8 * RT);
+        const int rem = ec - uxj[u] * (8 * RT);
+        ucg[u] = rem / RT;
+        ur[u] = rem - ucg[u] * RT;
+    }
+    // Loads go to clamped (valid) addresses without branches; bit 8 u + i of
+    // xmask marks the values that are stored (the rest are staged as zero).
+    int cr[NCR][8];        // source unknowns of the columns of the chunk after the staged one
+    unsigned xv[NU][8];    // next chunk's X (BF16 Re | Im << 16)
+    unsigned xmask = 0;
+    auto load_cref = [&](int cc) {
+#pragma unroll
+        for (int u = 0; u < NCR; ++u)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cr[u][i] = cref[min(cc + ucg[u] * 8 + i, C - 1)];
+    };
+    auto load_x = [&](int cc) {
+        xmask = 0;
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int src = cr[kShare ? 0 : u][i];
+                xv[u][i] = xb[((size_t)src + (uxj[u] ? (size_t)nd.N : 0)) * R + rb0 + min(ur[u], rv - 1)];
+                if (uok[u] && cc + ucg[u] * 8 + i < C && ur[u] < rv) xmask |= 1u << (8 * u + i);
+            }
+    };
+    load_cref(0);
+    load_x(0);
+    load_cref(kKC);
+    auto step = [&](int kb2, const uint4* zr, const uint4* zi) {
+        if (!wact) return;
+        const int kof = kb2 * 32 + 8 * lk;
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int rc = ct * 8 + (li & 7);
+            const uint4 xa = *reinterpret_cast<const uint4*>(&xs[0][part][rc][kof]);       // Re X | Im X
+            const uint4 xo = *reinterpret_cast<const uint4*>(&xs[0][1 - part][rc][kof]);   // Im X | Re X
+            const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
+            floatx4 a = acc[ct][0];
+            a = mfma(zr[0], bJ1, a);
+            a = mfma(zi[0], bJ2, a);
+            if (P == 3) {
+                const uint4 ma = *reinterpret_cast<const uint4*>(&xs[NX - 1][part][rc][kof]);
+                const uint4 mo = *reinterpret_cast<const uint4*>(&xs[NX - 1][1 - part][rc][kof]);
+                const uint4 bM1 = ma, bM2 = part ? mo : neg8(mo);
+                a = mfma(zr[1], neg8(bM1), a);   // - ZK X_M
+                a = mfma(zi[1], neg8(bM2), a);
+                floatx4 b = acc[ct][NO - 1];
+                b = mfma(zr[1], bJ1, b);
+                b = mfma(zi[1], bJ2, b);
+                b = mfma(zr[2], bM1, b);
+                b = mfma(zi[2], bM2, b);
+                acc[ct][NO - 1] = b;
+            }
+            acc[ct][0] = a;
+        }
+    };
+    uint4 ar[P], ai[P], br[P], bi[P];
+    load(0, ar, ai);
     for (int c0 = 0; c0 < C; c0 += kKC) {
         __syncthreads();
-        for (int e = threadIdx.x; e < NX * 8 * RT; e += 256) {
-            const int xj = e / (8 * RT), rem = e - xj * (8 * RT);
-            const int cg = rem / RT, r = rem - cg * RT;
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            if (!uok[u]) continue;
             unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const int c = c0 + cg * 8 + i;
-                unsigned v = 0;
-                if (c < C && r < rv) v = xb[((size_t)cref[c] + (xj ? (size_t)nd.N : 0)) * R + rb0 + r];
+                const unsigned v = ((xmask >> (8 * u + i)) & 1u) ? xv[u][i] : 0u;
                 re[i >> 1] |= (v & 0xffffu) << (16 * (i & 1));
                 im[i >> 1] |= (v >> 16) << (16 * (i & 1));
             }
-            *reinterpret_cast<uint4*>(&xs[xj][0][r][cg * 8]) = make_uint4(re[0], re[1], re[2], re[3]);
-            *reinterpret_cast<uint4*>(&xs[xj][1][r][cg * 8]) = make_uint4(im[0], im[1], im[2], im[3]);
+            *reinterpret_cast<uint4*>(&xs[uxj[u]][0][ur[u]][ucg[u] * 8]) = make_uint4(re[0], re[1], re[2], re[3]);
+            *reinterpret_cast<uint4*>(&xs[uxj[u]][1][ur[u]][ucg[u] * 8]) = make_uint4(im[0], im[1], im[2], im[3]);
         }
         __syncthreads();
-        for (int kb2 = 0; kb2 < 2; ++kb2) {
-            const int kb = (c0 >> 5) + kb2;
-            if (kb >= nkb) break;
-            uint4 zr[P], zi[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) zr[p] = nr[p], zi[p] = ni[p];
-            load(kb + 1, nr, ni);   // next block
-            if (!wact) continue;
-            const int kof = kb2 * 32 + 8 * lk;
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                const int rc = ct * 8 + (li & 7);
-                const uint4 xa = *reinterpret_cast<const uint4*>(&xs[0][part][rc][kof]);       // Re X | Im X
-                const uint4 xo = *reinterpret_cast<const uint4*>(&xs[0][1 - part][rc][kof]);   // Im X | Re X
-                const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
-                floatx4 a = acc[ct][0];
-                a = mfma(zr[0], bJ1, a);
-                a = mfma(zi[0], bJ2, a);
-                if (P == 3) {
-                    const uint4 ma = *reinterpret_cast<const uint4*>(&xs[NX - 1][part][rc][kof]);
-                    const uint4 mo = *reinterpret_cast<const uint4*>(&xs[NX - 1][1 - part][rc][kof]);
-                    const uint4 bM1 = ma, bM2 = part ? mo : neg8(mo);
-                    a = mfma(zr[1], neg8(bM1), a);   // - ZK X_M
-                    a = mfma(zi[1], neg8(bM2), a);
-                    floatx4 b = acc[ct][NO - 1];
-                    b = mfma(zr[1], bJ1, b);
-                    b = mfma(zi[1], bJ2, b);
-                    b = mfma(zr[2], bM1, b);
-                    b = mfma(zi[2], bM2, b);
-                    acc[ct][NO - 1] = b;
-                }
-                acc[ct][0] = a;
-            }
-        }
+        // Every load below is issued on every pass (past the end they read
+        // clamped, finite records and indices that meet zero-staged X), so
+        // the compiler's wait counts stay exact: each wait covers the data it
+        // needs, not the loads issued after it.
+        load_x(c0 + kKC);
+        load_cref(c0 + 2 * kKC);
+        // even block in set A, odd block in set B (no copies, so the next
+        // block's records stay in flight during the current block's products)
+        const int kb = c0 >> 5;
+        load(kb + 1, br, bi);
+        step(0, ar, ai);
+        load(kb + 2, ar, ai);
+        step(1, br, bi);
     }
     if (!wact) return;
     // D[4 lk + v][li]: lanes li < 8 hold Re of right-hand side ct * 8 + li, lane li + 8 its Im
@@ -239,7 +293,8 @@ void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd,
     const size_t total = (size_t)nd.N * (nd.planes == 3 ? 2 : 1) * R;   // unknowns x right-hand sides
     k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
-    dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
+    const int ny = (R + 8 * ct - 1) / (8 * ct);
+    const unsigned grid = (unsigned)((nd.nitem + 7) / 8) * 8 * ny;   // see the block mapping in k_near_bf16
 #define NB(P, CT) k_near_bf16<P, CT><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
 #define NBP(P)                                                                                                        \
     if (ct == 4) NB(P, 4);                                                                                            \

@@ -98,9 +98,10 @@ void launch_near_b(hipStream_t st, int rb, int ldr, int r0, const NearDev& nd, c
                    double2* y);
 
 // Near product of R right-hand sides at once on the matrix cores (FP32 near
-// entries, FP32 accumulation); vectors RHS-fastest with stride ldr from r0.
+// entries, FP32 accumulation); vectors RHS-fastest with stride ldr from r0;
+// xf: workspace of (unknowns x R) float2 for the FP32 copy of x.
 void launch_near_mfma(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const float2* Z, const double2* x,
-                      double2* y);
+                      float2* xf, double2* y);
 void launch_near_mfma64(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const double2* Z,
                         const double2* x, double2* y);
 
