@@ -58,9 +58,13 @@ typedef struct mlfma_params {
     double cfie_alpha;      /* reference compiles 0.2 (Code/PhyConstant.h:15) */
     double leaf_coeff;      /* finest box = leaf_coeff * lambda0; reference 0.3 (Code/PhyConstant.h:14) */
     int far_precision;      /* MLFMA_FP64 or MLFMA_FP32 */
-    int near_precision;     /* storage of the near-field entries used by FP32-mode matvecs: MLFMA_FP64
-                               or MLFMA_FP32 (entries computed in FP64, rounded once; accumulation
-                               FP64; the FP64 copy is kept for true residuals) */
+    int near_precision;     /* storage of the near-field entries used by FP32-mode matvecs: MLFMA_FP64,
+                               MLFMA_FP32 (entries computed in FP64, rounded once; the FP64 copy is
+                               kept for true residuals) or MLFMA_BF16 (FP32 storage for single-vector
+                               products; products of 2 or more right-hand sides on the BF16 matrix
+                               cores from split pairs hi + lo of entries and vectors, three partial
+                               products, FP32 accumulation: about 2^-17 relative, within the FP64
+                               refinement of FP32-mode solves) */
     int tf_full_order;      /* 0: reference series length min(L, ceil|kR|) (Code/AlgoMLFMM.h:263); 1: L */
     int max_rhs;            /* largest batch a single apply/solve will be given */
     int device;             /* HIP device ordinal */

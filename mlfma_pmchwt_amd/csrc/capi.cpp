@@ -66,7 +66,8 @@ int check_params(const mlfma_params* p, std::string& why) {
     if (!(p->frequency_hz > 0)) return why = "frequency must be positive", MLFMA_ERR_ARG;
     if (!(p->leaf_coeff > 0)) return why = "leaf_coeff must be positive", MLFMA_ERR_ARG;
     if (p->far_precision != MLFMA_FP64 && p->far_precision != MLFMA_FP32) return why = "bad far_precision", MLFMA_ERR_ARG;
-    if (p->near_precision != MLFMA_FP64 && p->near_precision != MLFMA_FP32) return why = "bad near_precision", MLFMA_ERR_ARG;
+    if (p->near_precision != MLFMA_FP64 && p->near_precision != MLFMA_FP32 && p->near_precision != MLFMA_BF16)
+        return why = "bad near_precision", MLFMA_ERR_ARG;
     if (p->precond_precision != MLFMA_FP64 && p->precond_precision != MLFMA_BF16)
         return why = "bad precond_precision", MLFMA_ERR_ARG;
     if (p->max_rhs < 1) return why = "max_rhs must be >= 1", MLFMA_ERR_ARG;

@@ -409,17 +409,25 @@ void Engine::build_near() {
     c.alpha = P.cfie_alpha;
     launch_near_fill(st, ie, nd_, d_geo_, gauss_, c, d_Z_);
     MLFMA_HIP(hipGetLastError());
-    if (P.near_precision == MLFMA_FP32) {
+    if (P.near_precision != MLFMA_FP64) {   // FP32 storage (also the single-vector products of MLFMA_BF16)
         d_Z32_ = dalloc<float2>((size_t)near_entries);
         launch_narrow(st, (size_t)near_entries, d_Z_, d_Z32_);
         MLFMA_HIP(hipGetLastError());
     }
+    if (P.near_precision == MLFMA_BF16) {   // split-pair records of the batched matrix-core products
+        std::vector<long long> off;
+        const long long recs = bf16_near_layout(tree.leaf_ptr, tgt_cols, planes, true, off);
+        d_bsoff_ = upload(off);
+        d_Zbs_ = dalloc<uint4>((size_t)recs);
+        launch_pack_bf16(st, nd_, d_Z_, d_bsoff_, true, d_Zbs_);
+        MLFMA_HIP(hipGetLastError());
+    }
     if (P.precond_precision == MLFMA_BF16) {
         std::vector<long long> bf_off;
-        const long long recs = bf16_near_layout(tree.leaf_ptr, tgt_cols, planes, bf_off);
+        const long long recs = bf16_near_layout(tree.leaf_ptr, tgt_cols, planes, false, bf_off);
         d_bfoff_ = upload(bf_off);
         d_Zbf_ = dalloc<uint4>((size_t)recs);
-        launch_pack_bf16(st, nd_, d_Z_, d_bfoff_, d_Zbf_);
+        launch_pack_bf16(st, nd_, d_Z_, d_bfoff_, false, d_Zbf_);
         MLFMA_HIP(hipGetLastError());
     }
     d_dinv_ = dalloc<double2>(nunk);
@@ -610,7 +618,10 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     const bool mfma_near = R >= 2 && near_mfma_;
     if (mfma_near) {
         if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
-        if (prec == MLFMA_FP32 && d_Z32_) {
+        if (prec == MLFMA_FP32 && d_Zbs_) {
+            ensure_xf(R);
+            launch_near_bf16x3(st, R, R, 0, nd_, d_Zbs_, d_bsoff_, X, reinterpret_cast<uint2*>(d_xf_), Y);
+        } else if (prec == MLFMA_FP32 && d_Z32_) {
             ensure_xf(R);
             launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, d_xf_, Y);
         } else {
@@ -797,7 +808,7 @@ void Engine::solve_device(const mlfma_solve_params& sp, const double2* d_b, doub
 
 void Engine::set_apply_precision(int prec) {
     if (prec != MLFMA_FP64 && prec != MLFMA_FP32) throw Error(MLFMA_ERR_ARG, "unknown precision");
-    if (prec == MLFMA_FP32 && P.far_precision != MLFMA_FP32 && P.near_precision != MLFMA_FP32)
+    if (prec == MLFMA_FP32 && P.far_precision != MLFMA_FP32 && P.near_precision == MLFMA_FP64)
         throw Error(MLFMA_ERR_ARG, "no FP32 tables built (create with far_precision or near_precision = MLFMA_FP32)");
     apply_prec = prec;
 }
@@ -1510,7 +1521,10 @@ void Engine::near_block(const double2* X, double2* Y, int R, int prec, bool prec
         return;
     }
     if (R >= 2 && near_mfma_) {
-        if (prec == MLFMA_FP32 && d_Z32_) {
+        if (prec == MLFMA_FP32 && d_Zbs_) {
+            ensure_xf(R);
+            launch_near_bf16x3(st, R, R, 0, nd_, d_Zbs_, d_bsoff_, X, reinterpret_cast<uint2*>(d_xf_), Y);
+        } else if (prec == MLFMA_FP32 && d_Z32_) {
             ensure_xf(R);
             launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, d_xf_, Y);
         } else {

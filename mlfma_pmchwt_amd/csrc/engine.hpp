@@ -200,6 +200,8 @@ private:
     NearDev nd_{};
     double2* d_Z_ = nullptr;
     float2* d_Z32_ = nullptr;   // FP32 copy (near_precision = MLFMA_FP32)
+    uint4* d_Zbs_ = nullptr;    // split-pair BF16 records of the batched products (near_precision = MLFMA_BF16)
+    long long* d_bsoff_ = nullptr;
     uint4* d_Zbf_ = nullptr;    // BF16 record copy for the preconditioner (precond_precision = MLFMA_BF16)
     long long* d_bfoff_ = nullptr;
     float2* d_xf_ = nullptr;    // FP32 copy of the matrix-core near product's input, nunk x xf_cap_

@@ -16,6 +16,7 @@
 // 2 KB record holding the A fragments of the 64 lanes for Re Z, then for Im Z
 // (8 BF16 = 16 B per lane), so a wavefront reads each record with one
 // coalesced 16-byte load per lane.  Rows past nt and columns past C_t are zero.
+#include <type_traits>
 #include <vector>
 
 #include "cplx.hpp"
@@ -52,9 +53,17 @@ __device__ __forceinline__ floatx4 mfma(uint4 a, uint4 b, floatx4 c) {
 }
 }  // namespace
 
+__device__ __forceinline__ float bf16_float(unsigned b) { return __uint_as_float(b << 16); }
+
 // One workgroup per target leaf: writes its records from the FP64 entries.
+// S (split pairs): each record also holds the BF16 rounding of what the first
+// BF16 value leaves of the FP32 entry, hi = bf16(f), lo = bf16(f - hi), so
+// hi + lo carries about 16 significant bits; records are then 4 KB:
+// [hi Re][hi Im][lo Re][lo Im], 64 lanes x 16 B each.
+template <bool S>
 __global__ void k_pack_bf16(NearDev nd, const double2* __restrict__ Z, const long long* __restrict__ bf_off,
                             uint4* __restrict__ out) {
+    constexpr int W = S ? 256 : 128;   // uint4 per record
     const int t = blockIdx.x;
     const int nt = nd.leaf_ptr[t + 1] - nd.leaf_ptr[t];
     if (nt == 0) return;
@@ -68,16 +77,26 @@ __global__ void k_pack_bf16(NearDev nd, const double2* __restrict__ Z, const lon
         const int lane = u & 63, rec = u >> 6;   // rec = (kb * nrt + rt) * planes + p
         const int p = rec % nd.planes, rt = (rec / nd.planes) % nrt, kb = rec / (nd.planes * nrt);
         const int row = rt * 16 + (lane & 15), cb = kb * 32 + 8 * (lane >> 4);
-        unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+        unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0}, rl[4] = {0, 0, 0, 0}, il[4] = {0, 0, 0, 0};
         for (int i = 0; i < 8; ++i) {
             const int col = cb + i;
             double2 z = make_double2(0.0, 0.0);
             if (row < nt && col < C) z = Zt[p * plane + (size_t)col * nt + row];
-            re[i >> 1] |= bf16_bits((float)z.x) << (16 * (i & 1));
-            im[i >> 1] |= bf16_bits((float)z.y) << (16 * (i & 1));
+            const float fr = (float)z.x, fi = (float)z.y;
+            const unsigned hr = bf16_bits(fr), hi = bf16_bits(fi);
+            re[i >> 1] |= hr << (16 * (i & 1));
+            im[i >> 1] |= hi << (16 * (i & 1));
+            if (S) {
+                rl[i >> 1] |= bf16_bits(fr - bf16_float(hr)) << (16 * (i & 1));
+                il[i >> 1] |= bf16_bits(fi - bf16_float(hi)) << (16 * (i & 1));
+            }
         }
-        o[(size_t)rec * 128 + lane] = make_uint4(re[0], re[1], re[2], re[3]);
-        o[(size_t)rec * 128 + 64 + lane] = make_uint4(im[0], im[1], im[2], im[3]);
+        o[(size_t)rec * W + lane] = make_uint4(re[0], re[1], re[2], re[3]);
+        o[(size_t)rec * W + 64 + lane] = make_uint4(im[0], im[1], im[2], im[3]);
+        if (S) {
+            o[(size_t)rec * W + 128 + lane] = make_uint4(rl[0], rl[1], rl[2], rl[3]);
+            o[(size_t)rec * W + 192 + lane] = make_uint4(il[0], il[1], il[2], il[3]);
+        }
     }
 }
 
@@ -91,6 +110,19 @@ __global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __
     xb[e] = bf16_bits((float)v.x) | (bf16_bits((float)v.y) << 16);
 }
 
+// split pairs of x: .x = (hi Re, hi Im), .y = (lo Re, lo Im) of the FP32 value
+__global__ void k_x_bf16x2(size_t total, int R, int ldr, int r0, const double2* __restrict__ x,
+                           uint2* __restrict__ xb) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const size_t u = e / R;
+    const int r = (int)(e - u * R);
+    const double2 v = x[u * ldr + r0 + r];
+    const float fr = (float)v.x, fi = (float)v.y;
+    const unsigned hr = bf16_bits(fr), hi = bf16_bits(fi);
+    xb[e] = make_uint2(hr | (hi << 16), bf16_bits(fr - bf16_float(hr)) | (bf16_bits(fi - bf16_float(hi)) << 16));
+}
+
 // A workgroup owns one work item (a target leaf's range of up to 4 16-row
 // tiles) and CT <= 4 column tiles of 8 right-hand sides (at 64 right-hand
 // sides, two workgroups of 32 with 4 waves per SIMD measured faster than one
@@ -102,14 +134,23 @@ __global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __
 // rows of 144 B: a thread gathers 8 consecutive columns of one right-hand side
 // (each load coalesced over the wavefront's right-hand sides) and writes them
 // with two 16-byte stores, which 8 consecutive lanes spread over all 32 banks.
-template <int P, int CT>
+//
+// S (split pairs, the operator's batched products with near_precision =
+// MLFMA_BF16): Z = Zh + Zl and X = Xh + Xl in BF16, and each product is
+// Zh Xh + Zh Xl + Zl Xh (the dropped Zl Xl and the representation error are
+// about 2^-17 of |Z||X|), three matrix-core products per record.
+template <int P, int CT, bool S>
 __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, int r0, const uint4* __restrict__ Zb,
                                                    const long long* __restrict__ bf_off,
-                                                   const unsigned* __restrict__ xb, double2* __restrict__ y) {
+                                                   const void* __restrict__ xbv, double2* __restrict__ y) {
     constexpr int NX = (P == 3) ? 2 : 1;   // staged currents
     constexpr int NO = (P == 3) ? 2 : 1;   // outputs
     constexpr int RT = 8 * CT;             // right-hand sides per workgroup
-    __shared__ __align__(16) unsigned short xs[NX][2][RT][kKP];   // [current][Re, Im][rhs][column]
+    constexpr int NS = S ? 2 : 1;          // staged X parts (hi, lo)
+    constexpr int W = S ? 256 : 128;       // uint4 per record
+    using xb_t = typename std::conditional<S, uint2, unsigned>::type;
+    const xb_t* __restrict__ xb = static_cast<const xb_t*>(xbv);
+    __shared__ __align__(16) unsigned short xs[NS][NX][2][RT][kKP];   // [part][current][Re, Im][rhs][column]
     // Blocks go round-robin to the 8 XCDs: the ny right-hand-side groups of one
     // item are 8 blocks apart, so they run on one XCD at about the same time
     // and the later groups read the item's records from that XCD's L2.
@@ -131,17 +172,22 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;          // 0: real-part column, 1: imaginary-part column
     const bool wact = wave < nrp;      // this wavefront's row tile exists
-    const uint4* zrec = Zb + bf_off[t] + (size_t)(rp + (wact ? wave : 0)) * P * 128 + lane;
-    const size_t kstride = (size_t)nrt * P * 128;   // records of one 32-column block
+    const uint4* zrec = Zb + bf_off[t] + (size_t)(rp + (wact ? wave : 0)) * P * W + lane;
+    const size_t kstride = (size_t)nrt * P * W;   // records of one 32-column block
     // records of block kb (clamped to the last block: a block past nkb is
     // loaded but never used), without bounds branches, which would make the
     // compiler wait for the prefetched records before the current products
+    // zr/zi[p] hold the hi parts of plane p, zr/zi[P + p] the lo parts (S)
     auto load = [&](int kb, uint4* zr, uint4* zi) {
         const uint4* rec = zrec + (size_t)min(kb, nkb - 1) * kstride;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            zr[p] = rec[p * 128];
-            zi[p] = rec[p * 128 + 64];
+            zr[p] = rec[p * W];
+            zi[p] = rec[p * W + 64];
+            if (S) {
+                zr[P + p] = rec[p * W + 128];
+                zi[P + p] = rec[p * W + 192];
+            }
         }
     };
     floatx4 acc[CT][NO];
@@ -169,7 +215,7 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     // Loads go to clamped (valid) addresses without branches; bit 8 u + i of
     // xmask marks the values that are stored (the rest are staged as zero).
     int cr[NCR][8];        // source unknowns of the columns of the chunk after the staged one
-    unsigned xv[NU][8];    // next chunk's X (BF16 Re | Im << 16)
+    xb_t xv[NU][8];        // next chunk's X (BF16 Re | Im << 16; S: .x hi, .y lo)
     unsigned xmask = 0;
     auto load_cref = [&](int cc) {
 #pragma unroll
@@ -191,50 +237,61 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     load_cref(0);
     load_x(0);
     load_cref(kKC);
+    // products of one 32-column block; zr/zi as filled by load
     auto step = [&](int kb2, const uint4* zr, const uint4* zi) {
         if (!wact) return;
         const int kof = kb2 * 32 + 8 * lk;
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
             const int rc = ct * 8 + (li & 7);
-            const uint4 xa = *reinterpret_cast<const uint4*>(&xs[0][part][rc][kof]);       // Re X | Im X
-            const uint4 xo = *reinterpret_cast<const uint4*>(&xs[0][1 - part][rc][kof]);   // Im X | Re X
-            const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
-            floatx4 a = acc[ct][0];
-            a = mfma(zr[0], bJ1, a);
-            a = mfma(zi[0], bJ2, a);
-            if (P == 3) {
-                const uint4 ma = *reinterpret_cast<const uint4*>(&xs[NX - 1][part][rc][kof]);
-                const uint4 mo = *reinterpret_cast<const uint4*>(&xs[NX - 1][1 - part][rc][kof]);
-                const uint4 bM1 = ma, bM2 = part ? mo : neg8(mo);
-                a = mfma(zr[1], neg8(bM1), a);   // - ZK X_M
-                a = mfma(zi[1], neg8(bM2), a);
-                floatx4 b = acc[ct][NO - 1];
-                b = mfma(zr[1], bJ1, b);
-                b = mfma(zi[1], bJ2, b);
-                b = mfma(zr[2], bM1, b);
-                b = mfma(zi[2], bM2, b);
-                acc[ct][NO - 1] = b;
+            floatx4 a = acc[ct][0], b = acc[ct][NO - 1];
+            // (Z part, X part) pairs: (hi, hi), and with S also (hi, lo), (lo, hi)
+#pragma unroll
+            for (int q = 0; q < (S ? 3 : 1); ++q) {
+                const int zp = q == 2 ? P : 0, xp = q == 1 ? 1 : 0;
+                const uint4 xa = *reinterpret_cast<const uint4*>(&xs[xp][0][part][rc][kof]);       // Re X | Im X
+                const uint4 xo = *reinterpret_cast<const uint4*>(&xs[xp][0][1 - part][rc][kof]);   // Im X | Re X
+                const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
+                a = mfma(zr[zp], bJ1, a);
+                a = mfma(zi[zp], bJ2, a);
+                if (P == 3) {
+                    const uint4 ma = *reinterpret_cast<const uint4*>(&xs[xp][NX - 1][part][rc][kof]);
+                    const uint4 mo = *reinterpret_cast<const uint4*>(&xs[xp][NX - 1][1 - part][rc][kof]);
+                    const uint4 bM1 = ma, bM2 = part ? mo : neg8(mo);
+                    a = mfma(zr[zp + 1], neg8(bM1), a);   // - ZK X_M
+                    a = mfma(zi[zp + 1], neg8(bM2), a);
+                    b = mfma(zr[zp + 1], bJ1, b);
+                    b = mfma(zi[zp + 1], bJ2, b);
+                    b = mfma(zr[zp + 2], bM1, b);
+                    b = mfma(zi[zp + 2], bM2, b);
+                }
             }
             acc[ct][0] = a;
+            if (P == 3) acc[ct][NO - 1] = b;
         }
     };
-    uint4 ar[P], ai[P], br[P], bi[P];
+    uint4 ar[NS * P], ai[NS * P], br[NS * P], bi[NS * P];
     load(0, ar, ai);
     for (int c0 = 0; c0 < C; c0 += kKC) {
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
             if (!uok[u]) continue;
-            unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const unsigned v = ((xmask >> (8 * u + i)) & 1u) ? xv[u][i] : 0u;
-                re[i >> 1] |= (v & 0xffffu) << (16 * (i & 1));
-                im[i >> 1] |= (v >> 16) << (16 * (i & 1));
+            for (int h = 0; h < NS; ++h) {
+                unsigned re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    unsigned v;
+                    if constexpr (S) v = h ? xv[u][i].y : xv[u][i].x;
+                    else v = xv[u][i];
+                    v = ((xmask >> (8 * u + i)) & 1u) ? v : 0u;
+                    re[i >> 1] |= (v & 0xffffu) << (16 * (i & 1));
+                    im[i >> 1] |= (v >> 16) << (16 * (i & 1));
+                }
+                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][0][ur[u]][ucg[u] * 8]) = make_uint4(re[0], re[1], re[2], re[3]);
+                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][1][ur[u]][ucg[u] * 8]) = make_uint4(im[0], im[1], im[2], im[3]);
             }
-            *reinterpret_cast<uint4*>(&xs[uxj[u]][0][ur[u]][ucg[u] * 8]) = make_uint4(re[0], re[1], re[2], re[3]);
-            *reinterpret_cast<uint4*>(&xs[uxj[u]][1][ur[u]][ucg[u] * 8]) = make_uint4(im[0], im[1], im[2], im[3]);
         }
         __syncthreads();
         // Every load below is issued on every pass (past the end they read
@@ -271,31 +328,35 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
 }
 
 long long bf16_near_layout(const std::vector<int>& leaf_ptr, const std::vector<int>& tgt_cols, int planes,
-                           std::vector<long long>& bf_off) {
+                           bool split, std::vector<long long>& bf_off) {
     const int nleaf = (int)tgt_cols.size();
     bf_off.assign(nleaf + 1, 0);
     for (int t = 0; t < nleaf; ++t) {
         const int nt = leaf_ptr[t + 1] - leaf_ptr[t];
         const long long recs = nt == 0 ? 0 : (long long)((tgt_cols[t] + 31) / 32) * ((nt + 15) / 16) * planes;
-        bf_off[t + 1] = bf_off[t] + recs * 128;
+        bf_off[t + 1] = bf_off[t] + recs * (split ? 256 : 128);
     }
     return bf_off[nleaf];
 }
 
-void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const long long* bf_off, uint4* out) {
+void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const long long* bf_off, bool split,
+                      uint4* out) {
     if (nd.nleaf == 0) return;
-    k_pack_bf16<<<nd.nleaf, 256, 0, st>>>(nd, Z, bf_off, out);
+    if (split) k_pack_bf16<true><<<nd.nleaf, 256, 0, st>>>(nd, Z, bf_off, out);
+    else k_pack_bf16<false><<<nd.nleaf, 256, 0, st>>>(nd, Z, bf_off, out);
 }
 
-void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
-                      const long long* bf_off, const double2* x, unsigned* xb, double2* y) {
+template <bool S>
+static void near_bf16_t(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                        const long long* bf_off, const double2* x, void* xb, double2* y) {
     if (nd.nitem == 0 || R < 1) return;
     const size_t total = (size_t)nd.N * (nd.planes == 3 ? 2 : 1) * R;   // unknowns x right-hand sides
-    k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
+    if (S) k_x_bf16x2<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, (uint2*)xb);
+    else k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, (unsigned*)xb);
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     const int ny = (R + 8 * ct - 1) / (8 * ct);
     const unsigned grid = (unsigned)((nd.nitem + 7) / 8) * 8 * ny;   // see the block mapping in k_near_bf16
-#define NB(P, CT) k_near_bf16<P, CT><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
+#define NB(P, CT) k_near_bf16<P, CT, S><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
 #define NBP(P)                                                                                                        \
     if (ct == 4) NB(P, 4);                                                                                            \
     else if (ct == 2) NB(P, 2);                                                                                       \
@@ -307,6 +368,16 @@ void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd,
     }
 #undef NBP
 #undef NB
+}
+
+void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                      const long long* bf_off, const double2* x, unsigned* xb, double2* y) {
+    near_bf16_t<false>(st, R, ldr, r0, nd, Zb, bf_off, x, xb, y);
+}
+
+void launch_near_bf16x3(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                        const long long* bf_off, const double2* x, uint2* xb, double2* y) {
+    near_bf16_t<true>(st, R, ldr, r0, nd, Zb, bf_off, x, xb, y);
 }
 
 }  // namespace mlfma

@@ -119,13 +119,19 @@ void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int r
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
 
-// BF16 near-field copy for the polynomial preconditioner: per-target record
-// offsets (in 16-byte units; returns the total), packing from the FP64
-// entries, and the batched product (x rounded to BF16 into xb, nu x R).
+// BF16 near-field copies: per-target record offsets (in 16-byte units;
+// returns the total), packing from the FP64 entries, and the batched products.
+// Single BF16 (the polynomial preconditioner): x rounded to BF16 into xb,
+// nu x R.  Split pairs (split = true; the operator's batched products with
+// near_precision = MLFMA_BF16): hi + lo BF16 of entries and of x (xb nu x R
+// pairs), three partial products.
 long long bf16_near_layout(const std::vector<int>& leaf_ptr, const std::vector<int>& tgt_cols, int planes,
-                           std::vector<long long>& bf_off);
-void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const long long* bf_off, uint4* out);
+                           bool split, std::vector<long long>& bf_off);
+void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const long long* bf_off, bool split,
+                      uint4* out);
 void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
                       const long long* bf_off, const double2* x, unsigned* xb, double2* y);
+void launch_near_bf16x3(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                        const long long* bf_off, const double2* x, uint2* xb, double2* y);
 
 }  // namespace mlfma
