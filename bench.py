@@ -139,8 +139,9 @@ def headline(args):
 
     # ---- mixed + batched + learned, end to end
     t_all = time.perf_counter()
+    near_prec = api.BF16 if args.near == "bf16x3" else api.FP32
     op = api.Operator(mesh, integral_equation=api.PMCHWT, eps_r_re=eps[0], eps_r_im=eps[1], far_precision=api.FP32,
-                      near_precision=api.FP32, precond_precision=api.BF16, max_rhs=R, verbose=1)
+                      near_precision=near_prec, precond_precision=api.BF16, max_rhs=R, verbose=1)
     setup_mixed = time.perf_counter() - t_all
     B = np.stack([op.excitation(*a) for a in inc], axis=1)
     t0 = time.perf_counter()
@@ -199,6 +200,9 @@ def headline(args):
         "mixed": {"total_s": round(mixed_total, 3), "setup_s": round(setup_mixed, 3), "predict_s": round(predict_s, 3),
                   "autotune_s": round(tune_s, 3), "tuned": tune, "graphs": True,
                   "preconditioner_near_products": "bf16 matrix cores, fp32 accumulation",
+                  "operator_near_products": ("split-pair bf16 matrix cores (hi + lo, three partial products), "
+                                             "fp32 accumulation" if args.near == "bf16x3"
+                                             else "fp32 matrix cores"),
                   "solve_s": round(st[0]["seconds"], 3), "iterations_min": min(its), "iterations_max": max(its),
                   "converged": conv, "precond_fallbacks": st[0]["precond_fallbacks"],
                   "max_true_rel_residual": max(s["rel_residual"] for s in st),
@@ -236,6 +240,8 @@ def main():
     ap.add_argument("--workload", default="pec_cfie", choices=["pec_cfie", "pmchwt_batch"],
                     help="pec_cfie: matvec/s on the 300k-RWG CFIE sphere (default); pmchwt_batch: end-to-end solve "
                          "of 64 incidences on the 300k-unknown PMCHWT sphere, mixed + batched + learned vs plain FP64")
+    ap.add_argument("--near", default="bf16x3", choices=["bf16x3", "fp32"],
+                    help="pmchwt_batch: the mixed path's batched near products (split-pair BF16 or FP32)")
     ap.add_argument("--plain-sample", type=int, default=4, help="pmchwt_batch: incidences solved one at a time in FP64")
     ap.add_argument("--plain-cap", type=int, default=3000, help="pmchwt_batch: iteration cap of the unpreconditioned solves")
     args = ap.parse_args()

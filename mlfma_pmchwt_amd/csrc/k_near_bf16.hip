@@ -168,7 +168,8 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     const int C = nd.tgt_cols[t];
     const int nkb = (C + 31) >> 5;
     const int* cref = nd.colref + nd.col_ptr[t];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wavefront index as a scalar, so tests on it are uniform branches
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;          // 0: real-part column, 1: imaginary-part column
     const bool wact = wave < nrp;      // this wavefront's row tile exists
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
         const int e = threadIdx.x + 256 * u, ec = min(e, NE - 1);   // spare threads repeat the last unit
-        uok[u] = e < NE;
+        uok[u] = NE % 256 == 0 || e < NE;
         uxj[u] = ec / (8 * RT);
         const int rem = ec - uxj[u] * (8 * RT);
         ucg[u] = rem / RT;
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
             for (int i = 0; i < 8; ++i) {
                 const int src = cr[kShare ? 0 : u][i];
                 xv[u][i] = xb[((size_t)src + (uxj[u] ? (size_t)nd.N : 0)) * R + rb0 + min(ur[u], rv - 1)];
-                if (uok[u] && cc + ucg[u] * 8 + i < C && ur[u] < rv) xmask |= 1u << (8 * u + i);
+                xmask |= (unsigned)(uok[u] & (cc + ucg[u] * 8 + i < C) & (ur[u] < rv)) << (8 * u + i);
             }
     };
     load_cref(0);

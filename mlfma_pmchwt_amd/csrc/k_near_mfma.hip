@@ -93,7 +93,8 @@ __global__ __launch_bounds__(256) void k_near_mfma(NearDev nd, int ldr, int r0, 
     const int* cref = nd.colref + nd.col_ptr[t];
     const int plane = C * nt;
     const C2* Zt = Z + nd.tgt_off[t];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wavefront index as a scalar, so tests on it are uniform branches
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;   // 0: real-part column, 1: imaginary-part column
     const bool wact = wave < nrp;   // this wavefront's row tile exists
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) void k_near_mfma(NearDev nd, int ldr, int r0, 
         const int rem = ec - xj * (kCC * RT);
         c = rem / RT;
         r = rem - c * RT;
-        return e < NE;
+        return NE % 256 == 0 || e < NE;
     };
     auto load_cref = [&](int cc) {
 #pragma unroll
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) void k_near_mfma(NearDev nd, int ldr, int r0, 
             int xj, c, r;
             const bool ok = unit(u, xj, c, r);
             pv[u] = xf[((size_t)cr[u] + (xj ? (size_t)nd.N : 0)) * R + rb0 + min(r, rv - 1)];
-            if (ok && cc + c < C && r < rv) pmask |= 1u << u;
+            pmask |= (unsigned)(ok & (cc + c < C) & (r < rv)) << u;
         }
     };
     if constexpr (PF) {
