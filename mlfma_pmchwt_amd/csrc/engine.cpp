@@ -504,6 +504,24 @@ void Engine::translate(int w, int l, int nc) {
 template <typename R>
 void Engine::disaggregate(double2* y, int ldr, int r0, int rb) {
     const int L = tree.max_level;
+    if (std::is_same<R, float>::value && eall_R_ > 0) {   // deferred: this pass's columns of Eall only
+        const bool pm = ie == MLFMA_PMCHWT;
+        const FarDev<float>& F1 = far<float>(0);
+        if (!pm) {
+            const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));   // Code/MLFMM.cpp:226
+            const double2 z = make_double2(0, 0);
+            launch_beff_all(st, nleaf, rb, wd_[0].K[L], false, F1.Bm[L], F1.kp0, d2(c), z, z, z, eall_R_, r0,
+                            d_Eall_[0]);
+        } else {
+            const FarDev<float>& F2 = far<float>(1);
+            const cplx c1 = med.k1 * med.k1 / (16.0 * kPi * kPi), c2 = med.k2 * med.k2 / (16.0 * kPi * kPi);
+            launch_beff_all(st, nleaf, rb, wd_[0].K[L], true, F1.Bm[L], F1.kp0, d2(c1), d2(c1), d2(c1), d2(c1),
+                            eall_R_, r0, d_Eall_[0]);
+            launch_beff_all(st, nleaf, rb, wd_[1].K[L], true, F2.Bm[L], F2.kp0, d2(c2 * med.etai), d2(c2), d2(c2),
+                            d2(c2 / med.etai), eall_R_, r0, d_Eall_[1]);
+        }
+        return;
+    }
     if (rb >= 2 && disagg_mfma_ && beff_buf<R>(0)) {
         const bool pm = ie == MLFMA_PMCHWT;
         const FarDev<R>& F1 = far<R>(0);
@@ -635,13 +653,47 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
             prof_ms[3] += ms;
         }
     }
-    for (int r0 = 0; r0 < R;) {
-        int rb = 1;
-        while (rb * 2 <= cap && r0 + rb * 2 <= R) rb *= 2;
-        apply_pass(X, Y, R, r0, rb, prec, !mfma_near);
-        if (prof_on_) collect_profile(rb);
-        r0 += rb;
+    // FP32 far field with the matrix-core disaggregation: the passes write
+    // their effective matrices into Eall and one product reads each pattern
+    // row once for all R (per-pass products when profiling, so the stage
+    // timings keep their meaning)
+    const bool all = f32 && mfma_near && disagg_mfma_ && !prof_on_;
+    if (all) {
+        ensure_eall(R);
+        eall_R_ = R;
     }
+    try {
+        for (int r0 = 0; r0 < R;) {
+            int rb = 1;
+            while (rb * 2 <= cap && r0 + rb * 2 <= R) rb *= 2;
+            apply_pass(X, Y, R, r0, rb, prec, !mfma_near);
+            if (prof_on_) collect_profile(rb);
+            r0 += rb;
+        }
+    } catch (...) {
+        eall_R_ = 0;
+        throw;
+    }
+    eall_R_ = 0;
+    if (all) {
+        const int L = tree.max_level;
+        const bool pm = ie == MLFMA_PMCHWT;
+        launch_disagg_all(st, R, nd_, pm, wd_[0].K[L], far<float>(0).Vf, d_Eall_[0], pm ? wd_[1].K[L] : 0,
+                          pm ? far<float>(1).Vf : nullptr, d_Eall_[1], Y);
+    }
+}
+
+// All-RHS effective matrices of the FP32 far field (sized before any
+// recording: the recorded sequences hold their addresses)
+void Engine::ensure_eall(int R) {
+    if (P.far_precision != MLFMA_FP32 || R <= eall_cap_) return;
+    clear_graphs();
+    const int L = tree.max_level, no = ie == MLFMA_PMCHWT ? 2 : 1;
+    for (int w = 0; w < nw; ++w) {
+        dfree(d_Eall_[w]);
+        d_Eall_[w] = dalloc<float2>((size_t)nleaf * no * 3 * wd_[w].K[L] * R);
+    }
+    eall_cap_ = R;
 }
 
 // Column-major device vectors (vector j at x + j * nunk), as the C ABI passes them.
@@ -665,6 +717,7 @@ void Engine::apply_device(const double2* x, double2* y, int nrhs) {
 void Engine::ensure_block(int R) {
     if (R > 256) throw Error(MLFMA_ERR_ARG, "at most 256 right-hand sides per call");
     ensure_xf(R);
+    ensure_eall(R);
     if (R <= block_cap_) return;
     clear_graphs();
     dfree(d_bx_);
@@ -1499,6 +1552,7 @@ void Engine::ensure_xf(int R) {
 
 void Engine::ensure_poly(int R) {
     ensure_xf(R);
+    ensure_eall(R);
     if (R <= poly_cap_) return;
     clear_graphs();
     dfree(d_pt_);

@@ -184,6 +184,7 @@ private:
     void precond_apply(const double2* x, double2* y);
     void ensure_poly(int R);
     void ensure_xf(int R);
+    void ensure_eall(int R);
     void near_block(const double2* X, double2* Y, int R, int prec, bool precond = false);
     double norm(const double2* v, int n);
 
@@ -204,6 +205,9 @@ private:
     long long* d_bsoff_ = nullptr;
     uint4* d_Zbf_ = nullptr;    // BF16 record copy for the preconditioner (precond_precision = MLFMA_BF16)
     long long* d_bfoff_ = nullptr;
+    float2* d_Eall_[2] = {nullptr, nullptr};   // all-RHS effective matrices (FP32), nleaf x no x 3K x eall_cap_
+    int eall_cap_ = 0;
+    int eall_R_ = 0;   // > 0 while a batched FP32 matvec defers its disaggregation to one all-RHS product
     float2* d_xf_ = nullptr;    // FP32 copy of the matrix-core near product's input, nunk x xf_cap_
     int xf_cap_ = 0;
     unsigned* d_xb_ = nullptr;  // BF16 copy of the preconditioner's near-product input, nunk x poly_cap_

@@ -194,6 +194,179 @@ void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int r
                                                    y);
 }
 
+// ---------------------------------------------------------------- all right-hand sides at once (FP32)
+// The far-field passes write their effective matrices into one block
+// Eall[w][leaf][o][q][R] (q = d K + k as in E, right-hand sides fastest,
+// pass columns r0 .. r0 + rb - 1), and one product per matvec then reads
+// each pattern row once for all R right-hand sides instead of once per pass.
+__global__ void k_beff_all(int nleaf, int rb, int K, int pmchwt, const float2* __restrict__ Bm,
+                           const float* __restrict__ kp, double2 ca, double2 cb, double2 cc, double2 cd, int R,
+                           int r0, float2* __restrict__ Eall) {
+    using M = Mx<float>;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nleaf * rb * K) return;
+    const int rr = (int)(e % rb);   // right-hand side fastest: the writes below are contiguous in it
+    const int k = (int)((e / rb) % K), leaf = (int)(e / ((size_t)K * rb));
+    const int ncur = pmchwt ? 2 : 1, no = pmchwt ? 2 : 1;
+    const size_t Q = 3 * (size_t)K;
+    const float2* BJ = Bm + ((size_t)leaf * rb * ncur + (size_t)rr * ncur) * Q;
+    float2* EJ = Eall + (size_t)leaf * no * Q * R + r0 + rr;   // element (o, q) at EJ[(o Q + q) R]
+    if (!pmchwt) {
+        for (int d = 0; d < 3; ++d) EJ[((size_t)d * K + k) * R] = M::mulc(ca, BJ[(size_t)d * K + k]);
+        return;
+    }
+    const float2* BM = BJ + Q;
+    float2* EM = EJ + Q * R;
+    const float kx = kp[k], ky = kp[K + k], kz = kp[2 * K + k];
+    float2 j[3], m[3];
+    for (int d = 0; d < 3; ++d) j[d] = BJ[(size_t)d * K + k], m[d] = BM[(size_t)d * K + k];
+    const float2 cj[3] = {csub(cscale(j[2], ky), cscale(j[1], kz)), csub(cscale(j[0], kz), cscale(j[2], kx)),
+                          csub(cscale(j[1], kx), cscale(j[0], ky))};
+    const float2 cm[3] = {csub(cscale(m[2], ky), cscale(m[1], kz)), csub(cscale(m[0], kz), cscale(m[2], kx)),
+                          csub(cscale(m[1], kx), cscale(m[0], ky))};
+    for (int d = 0; d < 3; ++d) {
+        EJ[((size_t)d * K + k) * R] = csub(M::mulc(ca, j[d]), M::mulc(cb, cm[d]));
+        EM[((size_t)d * K + k) * R] = cadd(M::mulc(cc, cj[d]), M::mulc(cd, m[d]));
+    }
+}
+
+void launch_beff_all(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, const float2* Bm, const float* kp,
+                     double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, float2* Eall) {
+    if (nleaf == 0) return;
+    const size_t total = (size_t)nleaf * rb * K;
+    k_beff_all<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(nleaf, rb, K, pmchwt ? 1 : 0, Bm, kp, ca, cb, cc, cd,
+                                                              R, r0, Eall);
+}
+
+// Y[perm[i] (+N for output 1)][r] += sum_w V_w[i] . Eall_w[leaf][o][.][r] for all R.
+// The work items of the matrix-core near product (a leaf's range of up to 4
+// 16-row tiles, largest first); wavefront w owns row tile w and CT column
+// tiles of 8 right-hand sides, the pattern values of the next k-step are
+// loaded while the current one's products run (alternating register sets,
+// clamped branch-free loads), and Eall is staged in LDS kQ rows at a time,
+// the next chunk loaded into registers during the current one's products.
+template <int NO, int CT>
+__global__ __launch_bounds__(256) void k_disagg_all(NearDev nd, int R, int nw, int K1, const float2* __restrict__ V1,
+                                                    const float2* __restrict__ E1, int K2,
+                                                    const float2* __restrict__ V2,
+                                                    const float2* __restrict__ E2, double2* __restrict__ y) {
+    using M = Mx<float>;
+    constexpr int RT = 8 * CT;
+    constexpr int kQ = 16;
+    constexpr int NE = NO * kQ * RT, NU = (NE + 255) / 256;
+    __shared__ float2 es[NO][kQ][RT];
+    const int2 item = nd.items[blockIdx.x];
+    const int t = item.x, rp = item.y;
+    const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
+    const int nrp = min(4, ((nt + 15) >> 4) - rp);
+    const int rb0 = blockIdx.y * RT;
+    const int rv = min(RT, R - rb0);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int part = li >> 3;
+    const bool wact = wave < nrp;
+    const int arow = min((rp + (wact ? wave : 0)) * 16 + li, nt - 1);
+    floatx4 acc[CT][NO];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) acc[c][o] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float2 pe[NU];
+    unsigned pmask = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int Q = 3 * (w ? K2 : K1);
+        const float2* Vr = (w ? V2 : V1) + ((size_t)i0 + arow) * Q;   // this lane's pattern row
+        const float2* E = (w ? E2 : E1) + (size_t)t * NO * Q * R;
+        auto load_e = [&](int q0) {
+            pmask = 0;
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u, ec = min(e, NE - 1);
+                const int o = ec / (kQ * RT), rem = ec - o * (kQ * RT);
+                const int qq = rem / RT, r = rem - qq * RT;
+                pe[u] = E[((size_t)o * Q + min(q0 + qq, Q - 1)) * R + rb0 + min(r, rv - 1)];
+                pmask |= (unsigned)((NE % 256 == 0 || e < NE) & (q0 + qq < Q) & (r < rv)) << u;
+            }
+        };
+        auto step = [&](int kk, const float2& v) {
+            if (!wact) return;
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int o = 0; o < NO; ++o) {
+                    const float2 ev = es[o][kk + lk][ct * 8 + (li & 7)];
+                    const float b1 = part ? ev.y : ev.x, b2 = part ? ev.x : -ev.y;
+                    floatx4 a = acc[ct][o];
+                    a = M::op(v.x, b1, a);
+                    a = M::op(v.y, b2, a);
+                    acc[ct][o] = a;
+                }
+        };
+        load_e(0);
+        float2 va = Vr[min(lk, Q - 1)], vb;
+        for (int q0 = 0; q0 < Q; q0 += kQ) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u;
+                if (NE % 256 == 0 || e < NE) {
+                    const int o = e / (kQ * RT), rem = e - o * (kQ * RT);
+                    const int qq = rem / RT, r = rem - qq * RT;
+                    es[o][qq][r] = ((pmask >> u) & 1u) ? pe[u] : make_float2(0.f, 0.f);
+                }
+            }
+            __syncthreads();
+            load_e(q0 + kQ);
+#pragma unroll
+            for (int kk = 0; kk < kQ; kk += 8) {
+                vb = Vr[min(q0 + kk + 4 + lk, Q - 1)];
+                step(kk, va);
+                va = Vr[min(q0 + kk + 8 + lk, Q - 1)];
+                step(kk + 4, vb);
+            }
+        }
+    }
+    if (!wact) return;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int o = 0; o < NO; ++o)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float re = acc[ct][o][v];
+                const float im = __shfl_xor(re, 8, 64);
+                const int row = (rp + wave) * 16 + M::row(lk, v);
+                const int rc = ct * 8 + li;
+                if (part == 0 && rc < rv && row < nt) {
+                    double2* yp = y + ((size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0)) * R + rb0 + rc;
+                    double2 tt = *yp;
+                    tt.x += (double)re;
+                    tt.y += (double)im;
+                    *yp = tt;
+                }
+            }
+}
+
+void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const float2* V1,
+                       const float2* E1, int K2, const float2* V2, const float2* E2, double2* y) {
+    if (nd.nitem == 0 || R < 1) return;
+    const int ct = R > 32 ? 8 : (R > 16 ? 4 : (R > 8 ? 2 : 1));
+    dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
+#define DA(NO, CT) k_disagg_all<NO, CT><<<grid, 256, 0, st>>>(nd, R, NO, K1, V1, E1, K2, V2, E2, y)
+#define DAN(NO)                                                                                                       \
+    if (ct == 8) DA(NO, 8);                                                                                           \
+    else if (ct == 4) DA(NO, 4);                                                                                      \
+    else if (ct == 2) DA(NO, 2);                                                                                      \
+    else DA(NO, 1);
+    if (pmchwt) {
+        DAN(2)
+    } else {
+        DAN(1)
+    }
+#undef DAN
+#undef DA
+}
+
 #define MLFMA_DISAGG_MFMA_INSTANTIATE(T)                                                                            \
     template void launch_beff<T>(hipStream_t, int, int, int, bool, const cplx_t<T>*, const T*, double2, double2,   \
                                  double2, double2, cplx_t<T>*);                                                     \

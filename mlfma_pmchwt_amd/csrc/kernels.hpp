@@ -115,6 +115,13 @@ template <typename T>
 void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int rb, const int* order,
                         const int* leaf_ptr, const int* perm, bool pmchwt, int K1, const cplx_t<T>* V1,
                         const cplx_t<T>* E1, int K2, const cplx_t<T>* V2, const cplx_t<T>* E2, double2* y);
+// All right-hand sides at once (FP32): the passes' effective matrices go into
+// Eall[leaf][o][3K][R] at columns r0 .. r0 + rb - 1, then one product per
+// matvec over the near product's work items (nd.items, leaf rows).
+void launch_beff_all(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, const float2* Bm, const float* kp,
+                     double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, float2* Eall);
+void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const float2* V1,
+                       const float2* E1, int K2, const float2* V2, const float2* E2, double2* y);
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
