@@ -469,8 +469,14 @@ void Engine::far_pass(int w, const double2* x, int ldr, int r0, int rb) {
     const int L = tree.max_level;
     const int nc = ncur * rb;
     mark(6 * w + 0);
-    launch_aggregate_b<R>(st, ncur, rb, nleaf, D.K[L], N, ldr, r0, d_leaf_order_, d_leaf_ptr_, d_perm_, F.Vs, x,
-                          F.Sm[L]);
+    cplx_t<R>* leaf_ws = F.Sm[L];
+    if constexpr (std::is_same<R, float>::value) {
+        if (eall_R_ > 0)   // aggregated for all right-hand sides already: this pass's leaf workspaces
+            F.Sm[L] = d_Sall_[w] + (size_t)nleaf * ncur * r0 * 3 * D.K[L];
+    }
+    if (F.Sm[L] == leaf_ws)
+        launch_aggregate_b<R>(st, ncur, rb, nleaf, D.K[L], N, ldr, r0, d_leaf_order_, d_leaf_ptr_, d_perm_, F.Vs, x,
+                              F.Sm[L]);
     mark(6 * w + 1);
     mark(6 * w + 2);
     for (int l = L - 1; l >= 2; --l)
@@ -479,6 +485,7 @@ void Engine::far_pass(int w, const double2* x, int ldr, int r0, int rb) {
     mark(6 * w + 3);
     mark(6 * w + 4);
     for (int l = 2; l <= L; ++l) translate<R>(w, l, nc);
+    F.Sm[L] = leaf_ws;
     mark(6 * w + 5);
 }
 
@@ -653,13 +660,19 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
             prof_ms[3] += ms;
         }
     }
-    // FP32 far field with the matrix-core disaggregation: the passes write
-    // their effective matrices into Eall and one product reads each pattern
-    // row once for all R (per-pass products when profiling, so the stage
-    // timings keep their meaning)
+    // FP32 far field with the matrix-core disaggregation: one product
+    // aggregates all R into the passes' leaf workspaces, the passes write
+    // their effective matrices into Eall, and one product disaggregates all R,
+    // so each pattern row is read once per matvec in each direction
+    // (per-pass products when profiling, so the stage timings keep their
+    // meaning)
     const bool all = f32 && mfma_near && disagg_mfma_ && !prof_on_;
     if (all) {
         ensure_eall(R);
+        const int L = tree.max_level;
+        for (int w = 0; w < nw; ++w)
+            launch_aggregate_all(st, ncur, nleaf, wd_[w].K[L], N, R, cap, d_leaf_ptr_, d_perm_, far<float>(w).Vs, X,
+                                 d_Sall_[w]);
         eall_R_ = R;
     }
     try {
@@ -691,7 +704,9 @@ void Engine::ensure_eall(int R) {
     const int L = tree.max_level, no = ie == MLFMA_PMCHWT ? 2 : 1;
     for (int w = 0; w < nw; ++w) {
         dfree(d_Eall_[w]);
+        dfree(d_Sall_[w]);
         d_Eall_[w] = dalloc<float2>((size_t)nleaf * no * 3 * wd_[w].K[L] * R);
+        d_Sall_[w] = dalloc<float2>((size_t)nleaf * ncur * 3 * wd_[w].K[L] * R);
     }
     eall_cap_ = R;
 }

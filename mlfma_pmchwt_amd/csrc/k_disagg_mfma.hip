@@ -367,6 +367,136 @@ void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, in
 #undef DA
 }
 
+// ---------------------------------------------------------------- aggregation, all right-hand sides (FP32)
+// Code/ProcessMLFMM.h:33-56 (mexp) for all R right-hand sides in one product:
+// per leaf, S (3K x ncur R) = V^T (3K x nt) X (nt x ncur R) on the matrix
+// cores, so each pattern row is read once per matvec.  Column c is current
+// c % ncur of right-hand side c / ncur.  The result goes straight into the
+// leaf workspaces of every far-field pass: pass (r0, rb) (the split of R into
+// passes of the batched matvec, rb the largest power of two <= cap that
+// fits) at Sall + nleaf ncur r0 Q, element (leaf, rr ncur + cur, q) as in a
+// single pass's workspace.  A workgroup owns one leaf and 4 16-row q tiles
+// (one per wavefront) and CT column tiles; X is staged in LDS kJ rows at a
+// time, the next chunk in flight during the current one's products.
+template <int NCUR, int CT>
+__global__ __launch_bounds__(256) void k_aggregate_all(int nleaf, int K, int N, int R, int cap,
+                                                       const int* __restrict__ leaf_ptr,
+                                                       const int* __restrict__ perm, const float2* __restrict__ Vs,
+                                                       const double2* __restrict__ x, float2* __restrict__ Sall) {
+    using M = Mx<float>;
+    constexpr int CC = 8 * CT;   // columns per workgroup
+    constexpr int kJ = 16;
+    constexpr int NE = kJ * CC, NU = (NE + 255) / 256;
+    __shared__ float2 xs[kJ][CC];
+    const int Q = 3 * K, QG = (Q + 63) / 64;
+    const int leaf = blockIdx.x / QG, qg = blockIdx.x - leaf * QG;
+    const int i0 = leaf_ptr[leaf], nt = leaf_ptr[leaf + 1] - i0;
+    const int ncol = NCUR * R, cb0 = blockIdx.y * CC;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int part = li >> 3;
+    const int q = (qg * 4 + wave) * 16 + li;   // this lane's row of the A operand (V^T)
+    const int qc = min(q, Q - 1);
+    floatx4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (nt > 0) {
+        float2 px[NU];
+        unsigned pmask = 0;
+        auto load_x = [&](int j0) {
+            pmask = 0;
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u, ec = min(e, NE - 1);
+                const int jj = ec / CC, cc = ec - jj * CC;
+                const int c = min(cb0 + cc, ncol - 1);
+                const int r = c / NCUR, cur = c - r * NCUR;
+                const double2 v = x[((size_t)cur * N + perm[i0 + min(j0 + jj, nt - 1)]) * R + r];
+                px[u] = make_float2((float)v.x, (float)v.y);
+                pmask |= (unsigned)((NE % 256 == 0 || e < NE) & (j0 + jj < nt) & (cb0 + cc < ncol)) << u;
+            }
+        };
+        const float2* Vq = Vs + (size_t)i0 * Q + qc;   // element (j, q) at Vq[j Q]
+        auto vload = [&](int j) { return Vq[(size_t)min(j, nt - 1) * Q]; };
+        auto step = [&](int kk, const float2& v) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const float2 xv = xs[kk + lk][ct * 8 + (li & 7)];
+                const float b1 = part ? xv.y : xv.x, b2 = part ? xv.x : -xv.y;
+                floatx4 a = acc[ct];
+                a = M::op(v.x, b1, a);
+                a = M::op(v.y, b2, a);
+                acc[ct] = a;
+            }
+        };
+        load_x(0);
+        float2 va = vload(lk), vb;
+        for (int j0 = 0; j0 < nt; j0 += kJ) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u;
+                if (NE % 256 == 0 || e < NE) {
+                    const int jj = e / CC, cc = e - jj * CC;
+                    xs[jj][cc] = ((pmask >> u) & 1u) ? px[u] : make_float2(0.f, 0.f);
+                }
+            }
+            __syncthreads();
+            load_x(j0 + kJ);
+#pragma unroll
+            for (int kk = 0; kk < kJ; kk += 8) {
+                vb = vload(j0 + kk + 4 + lk);
+                step(kk, va);
+                va = vload(j0 + kk + 8 + lk);
+                step(kk + 4, vb);
+            }
+        }
+    }
+    // D[4 lk + v][li]: lanes li < 8 hold Re of column ct * 8 + li, lane li + 8 its Im
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int c = cb0 + ct * 8 + li;
+        const int r = c / NCUR, cur = c - r * NCUR;
+        int p0 = 0, rb = 1;   // the pass holding right-hand side r
+        for (;;) {
+            rb = 1;
+            while (rb * 2 <= cap && p0 + rb * 2 <= R) rb *= 2;
+            if (r < p0 + rb || p0 + rb >= R) break;
+            p0 += rb;
+        }
+        float2* S = Sall + (size_t)nleaf * NCUR * p0 * Q + ((size_t)leaf * rb + (r - p0)) * NCUR * Q +
+                    (size_t)cur * Q;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float re = acc[ct][v];
+            const float im = __shfl_xor(re, 8, 64);
+            const int qq = (qg * 4 + wave) * 16 + M::row(lk, v);
+            if (part == 0 && c < ncol && qq < Q) S[qq] = make_float2(re, im);
+        }
+    }
+}
+
+void launch_aggregate_all(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, const int* leaf_ptr,
+                          const int* perm, const float2* Vs, const double2* x, float2* Sall) {
+    if (nleaf == 0 || R < 1) return;
+    const int ncol = ncur * R;
+    const int ct = ncol > 32 ? 8 : (ncol > 16 ? 4 : (ncol > 8 ? 2 : 1));
+    dim3 grid(nleaf * ((3 * K + 63) / 64), (ncol + 8 * ct - 1) / (8 * ct));
+#define AA(NC, CT) k_aggregate_all<NC, CT><<<grid, 256, 0, st>>>(nleaf, K, N, R, cap, leaf_ptr, perm, Vs, x, Sall)
+#define AAN(NC)                                                                                                       \
+    if (ct == 8) AA(NC, 8);                                                                                           \
+    else if (ct == 4) AA(NC, 4);                                                                                      \
+    else if (ct == 2) AA(NC, 2);                                                                                      \
+    else AA(NC, 1);
+    if (ncur == 2) {
+        AAN(2)
+    } else {
+        AAN(1)
+    }
+#undef AAN
+#undef AA
+}
+
 #define MLFMA_DISAGG_MFMA_INSTANTIATE(T)                                                                            \
     template void launch_beff<T>(hipStream_t, int, int, int, bool, const cplx_t<T>*, const T*, double2, double2,   \
                                  double2, double2, cplx_t<T>*);                                                     \

@@ -122,6 +122,12 @@ void launch_beff_all(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, cons
                      double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, float2* Eall);
 void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const float2* V1,
                        const float2* E1, int K2, const float2* V2, const float2* E2, double2* y);
+// Aggregation of all R right-hand sides (FP32 patterns, x RHS-fastest with
+// stride R) into the leaf workspaces of every pass of the batched matvec
+// (passes of rb = largest power of two <= cap, pass (r0, rb) at
+// Sall + nleaf ncur r0 3K).
+void launch_aggregate_all(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, const int* leaf_ptr,
+                          const int* perm, const float2* Vs, const double2* x, float2* Sall);
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
