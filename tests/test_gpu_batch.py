@@ -189,3 +189,29 @@ def test_cluster_translation_matches_per_box(name):
         print(f"[cluster] {name} {'batched' if k else 'single'} fp64 {e64:.2e} fp32 {e32:.2e}")
         assert e64 <= 1e-13 and e32 <= 2e-6
     assert relerr(out[(4, api.FP64)][0], orc.apply(X[:, 0])) <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["efie_ml", "pmchwt_lossy_ml", "pmchwt_1lvl"])
+@pytest.mark.parametrize("R", [5, 13, 21])
+def test_all_rhs_far_ends_match_per_pass(name, R):
+    """FP32 batched matvecs aggregate all right-hand sides into every pass's
+    leaf workspaces in one matrix-core product and disaggregate them in
+    another (tuning bit 2 on); with it off each pass aggregates and
+    disaggregates its own right-hand sides with the vector kernels.  Passes of
+    8, 4 and 1 (R = 5, 13, 21) exercise the pass lookup of the aggregation; the
+    two agree to FP32 rounding, and both meet the oracle's FP64 operator at the
+    FP32 bar."""
+    _, op, orc = pair(name)
+    X = np.stack([seeded_x(op.n, 1300 + j) for j in range(R)], axis=1)
+    op.set_apply_precision(api.FP32)
+    out = {}
+    for bits in (7, 5):
+        op.set_tuning(api.FP32, 0, bits)
+        out[bits] = op.apply(X)
+    op.set_tuning(api.FP32, 0, 7)
+    op.set_apply_precision(api.FP64)
+    e = relerr(out[7], out[5])
+    e_or = max(relerr(out[7][:, j], orc.apply(X[:, j])) for j in (0, R - 1))
+    print(f"[all-rhs] {name} R={R}: all-RHS vs per-pass {e:.2e}, vs oracle {e_or:.2e}")
+    assert e <= 2e-6
+    assert e_or <= 1e-5
