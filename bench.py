@@ -223,6 +223,49 @@ def headline(args):
     print(json.dumps(out))
 
 
+def multi_object(args):
+    """Several objects of uneven size in one batch run (the reference's batch
+    configuration format): eight PMCHWT spheres of 3k-35k RWG, 6 incidences x 2
+    polarisations each, GMRES(50) to 1e-4 (the reference drivers' settings) in
+    FP32 mode, first one object after another, then `--concurrency` objects at
+    once on that many host threads.  The results must not depend on it."""
+    import tempfile
+    import numpy as np
+    from mlfma_pmchwt_amd import api, geometry
+    objs = [(10, 0.3), (12, 0.35), (14, 0.4), (16, 0.45), (18, 0.5), (20, 0.55), (22, 0.6), (24, 0.65)]
+    tmp = tempfile.mkdtemp(prefix="mlfma_objects_")
+    paths = []
+    for k, (n, r) in enumerate(objs):
+        p = os.path.join(tmp, f"object{k}.nas")
+        geometry.write_nas(p, geometry.geodesic_sphere(n, r))
+        paths.append(p)
+    cfg = (f"nas_files = {'; '.join(paths)}\nfrequencies = 299792458\nincident_theta_range = 0:150:30\n"
+           "material_list = 4.0,-0.1\nsca_theta = 0:180\nsca_phi = 0:0\nstep = 2\nintegral_equation = 2\n")
+    sp = api.SolveParams(api.GMRES, 2000, 50, 1e-4, 0.0, 0)
+    runs = {}
+    for conc in (1, args.concurrency):
+        b = api.Batch(text=cfg)
+        t0 = time.perf_counter()
+        rc = b.run(sp, concurrency=conc, far_precision=api.FP32, near_precision=api.FP32)
+        dt = time.perf_counter() - t0
+        res = [[b.result(i, pol) for pol in (0, 1)] for i in range(b.n_tasks)]
+        b.close()
+        its = [r[2]["iterations"] for t in res for r in t]
+        log(f"concurrency {conc}: {dt:.2f} s, status {rc}, iterations {min(its)}..{max(its)}")
+        runs[conc] = (dt, res)
+    (t1, r1), (tk, rk) = runs[1], runs[args.concurrency]
+    same_its = all(a[2]["iterations"] == c[2]["iterations"] for ta, tc in zip(r1, rk) for a, c in zip(ta, tc))
+    max_db = max(float(np.max(np.abs(a[1] - c[1]))) for ta, tc in zip(r1, rk) for a, c in zip(ta, tc))
+    print(json.dumps({
+        "metric": "multi-object batch wall s (8 PMCHWT spheres, 12 runs each)", "value": round(tk, 3), "unit": "s",
+        "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "f32",
+        "config": {"workload": "8 PMCHWT spheres eps_r=4-0.1j, geodesic n=10..24 (3,000-17,280 RWG), 6 theta x 2 "
+                               "polarisations each, GMRES(50) to 1e-4, FP32 mode",
+                   "objects": [{"geodesic_n": n, "radius_lambda": r, "n_rwg": 30 * n * n} for n, r in objs]},
+        "sequential_s": round(t1, 3), "concurrency": args.concurrency, "concurrent_s": round(tk, 3),
+        "speedup": round(t1 / tk, 3), "same_iterations": same_its, "max_rcs_diff_db": max_db}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,9 +280,11 @@ def main():
     ap.add_argument("--batch", type=int, default=32, help="right-hand sides of the batched-matvec measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB device to device once (counter calibration run)")
-    ap.add_argument("--workload", default="pec_cfie", choices=["pec_cfie", "pmchwt_batch"],
+    ap.add_argument("--concurrency", type=int, default=4, help="multi_object: objects processed at once")
+    ap.add_argument("--workload", default="pec_cfie", choices=["pec_cfie", "pmchwt_batch", "multi_object"],
                     help="pec_cfie: matvec/s on the 300k-RWG CFIE sphere (default); pmchwt_batch: end-to-end solve "
-                         "of 64 incidences on the 300k-unknown PMCHWT sphere, mixed + batched + learned vs plain FP64")
+                         "of 64 incidences on the 300k-unknown PMCHWT sphere, mixed + batched + learned vs plain FP64; "
+                         "multi_object: eight objects of uneven size in one batch run, sequential vs concurrent")
     ap.add_argument("--near", default="bf16x3", choices=["bf16x3", "fp32"],
                     help="pmchwt_batch: the mixed path's batched near products (split-pair BF16 or FP32)")
     ap.add_argument("--plain-sample", type=int, default=4, help="pmchwt_batch: incidences solved one at a time in FP64")
@@ -260,6 +305,9 @@ def main():
         sys.exit(1)
     if args.workload == "pmchwt_batch":
         headline(args)
+        return
+    if args.workload == "multi_object":
+        multi_object(args)
         return
 
     t0 = time.perf_counter()
