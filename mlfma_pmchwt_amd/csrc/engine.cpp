@@ -246,6 +246,12 @@ void Engine::build_wave(int w) {
     if (P.far_precision == MLFMA_FP32) {
         build_far<float>(w, T);
         if (rb_max<float>() > 1) D.E = dalloc<float2>((size_t)nleaf * rb_max<float>() * no * 3 * D.K[L]);
+        if (P.near_precision == MLFMA_BF16) {   // split-pair planes of the receive patterns
+            const int Q = 3 * D.K[L], Qp = (Q + 31) & ~31;
+            d_Vb_[w] = dalloc<unsigned short>((size_t)4 * N * Qp);
+            launch_split_rows(st, N, Q, Qp, far<float>(w).Vf, d_Vb_[w]);
+            MLFMA_HIP(hipGetLastError());
+        }
     }
 }
 
@@ -691,8 +697,12 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     if (all) {
         const int L = tree.max_level;
         const bool pm = ie == MLFMA_PMCHWT;
-        launch_disagg_all(st, R, nd_, pm, wd_[0].K[L], far<float>(0).Vf, d_Eall_[0], pm ? wd_[1].K[L] : 0,
-                          pm ? far<float>(1).Vf : nullptr, d_Eall_[1], Y);
+        if (d_Vb_[0])   // split-pair BF16 (near_precision = MLFMA_BF16)
+            launch_disagg_bf3(st, R, nd_, pm, wd_[0].K[L], d_Vb_[0], d_Eall_[0], pm ? wd_[1].K[L] : 0,
+                              pm ? d_Vb_[1] : nullptr, d_Eall_[1], Y);
+        else
+            launch_disagg_all(st, R, nd_, pm, wd_[0].K[L], far<float>(0).Vf, d_Eall_[0], pm ? wd_[1].K[L] : 0,
+                              pm ? far<float>(1).Vf : nullptr, d_Eall_[1], Y);
     }
 }
 

@@ -206,6 +206,7 @@ private:
     uint4* d_Zbf_ = nullptr;    // BF16 record copy for the preconditioner (precond_precision = MLFMA_BF16)
     long long* d_bfoff_ = nullptr;
     float2* d_Eall_[2] = {nullptr, nullptr};   // all-RHS effective matrices (FP32), nleaf x no x 3K x eall_cap_
+    unsigned short* d_Vb_[2] = {nullptr, nullptr};   // split-pair BF16 receive-pattern planes (near_precision = MLFMA_BF16)
     float2* d_Sall_[2] = {nullptr, nullptr};   // all-RHS leaf radiation workspaces of the passes, nleaf x ncur x 3K x eall_cap_
     int eall_cap_ = 0;
     int eall_R_ = 0;   // > 0 while a batched FP32 matvec defers its disaggregation to one all-RHS product

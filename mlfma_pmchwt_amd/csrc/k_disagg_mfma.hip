@@ -367,6 +367,202 @@ void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, in
 #undef DA
 }
 
+// ---------------------------------------------------------------- split-pair BF16 disaggregation
+// The same all-RHS product with split-pair BF16 operands (far_precision =
+// MLFMA_FP32 with near_precision = MLFMA_BF16): each FP32 pattern value f is
+// hi = bf16(f) plus lo = bf16(f - hi), stored as four planes [Re hi, Im hi,
+// Re lo, Im lo][N][Qp] (Qp = 3K rounded up to 32, zero-padded), and Eall is
+// split the same way when staged; the product is Vh Eh + Vh El + Vl Eh on
+// v_mfma_f32_16x16x32_bf16 (lane l holds A[l % 16][8 (l / 16) + i] and
+// B[8 (l / 16) + i][l % 16], i < 8, and D[4 (l / 16) + v][l % 16]), about
+// 2^-17 of |V||E| from the FP32 product, FP32 accumulation.
+namespace {
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ unsigned bf16b(float f) {   // round to nearest even (finite inputs)
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ float bf16f(unsigned b) { return __uint_as_float(b << 16); }
+__device__ __forceinline__ uint4 neg8b(uint4 v) {
+    v.x ^= 0x80008000u;
+    v.y ^= 0x80008000u;
+    v.z ^= 0x80008000u;
+    v.w ^= 0x80008000u;
+    return v;
+}
+__device__ __forceinline__ floatx4 mfb(uint4 a, uint4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
+}  // namespace
+
+// planes[p][i][q] from V[i][q] (FP32 complex), q < Qp (zero past Q)
+__global__ void k_split_rows(int N, int Q, int Qp, const float2* __restrict__ V, unsigned short* __restrict__ planes) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * Qp) return;
+    const size_t i = e / Qp;
+    const int q = (int)(e - i * Qp);
+    const float2 v = q < Q ? V[i * Q + q] : make_float2(0.f, 0.f);
+    const unsigned hr = bf16b(v.x), hi = bf16b(v.y);
+    const size_t plane = (size_t)N * Qp;
+    planes[e] = (unsigned short)hr;
+    planes[plane + e] = (unsigned short)hi;
+    planes[2 * plane + e] = (unsigned short)bf16b(v.x - bf16f(hr));
+    planes[3 * plane + e] = (unsigned short)bf16b(v.y - bf16f(hi));
+}
+
+void launch_split_rows(hipStream_t st, int N, int Q, int Qp, const float2* V, unsigned short* planes) {
+    const size_t total = (size_t)N * Qp;
+    if (total == 0) return;
+    k_split_rows<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(N, Q, Qp, V, planes);
+}
+
+template <int NO, int CT>
+__global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, int K1,
+                                                    const unsigned short* __restrict__ P1,
+                                                    const float2* __restrict__ E1, int K2,
+                                                    const unsigned short* __restrict__ P2,
+                                                    const float2* __restrict__ E2, double2* __restrict__ y) {
+    constexpr int RT = 8 * CT;
+    constexpr int kQ = 32, kP = kQ + 8;   // staged q per chunk, padded LDS row (80 B)
+    constexpr int NE = NO * kQ * RT, NU = (NE + 255) / 256;
+    __shared__ __align__(16) unsigned short es[2][NO][2][RT][kP];   // [hi, lo][output][Re, Im][rhs][q]
+    const int2 item = nd.items[blockIdx.x];
+    const int t = item.x, rp = item.y;
+    const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
+    const int nrp = min(4, ((nt + 15) >> 4) - rp);
+    const int rb0 = blockIdx.y * RT;
+    const int rv = min(RT, R - rb0);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int part = li >> 3;
+    const bool wact = wave < nrp;
+    const int arow = min((rp + (wact ? wave : 0)) * 16 + li, nt - 1);
+    floatx4 acc[CT][NO];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) acc[c][o] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float2 pe[NU];
+    unsigned pmask = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int Q = 3 * (w ? K2 : K1), Qp = (Q + 31) & ~31;
+        const unsigned short* Pl = w ? P2 : P1;
+        const size_t plane = (size_t)nd.N * Qp;
+        const unsigned short* arow_p = Pl + ((size_t)i0 + arow) * Qp + 8 * lk;   // this lane's 8 q of plane 0
+        const float2* E = (w ? E2 : E1) + (size_t)t * NO * Q * R;
+        auto load_a = [&](int q0, uint4* a) {
+            const int qc = min(q0, Qp - kQ);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) a[p] = *reinterpret_cast<const uint4*>(arow_p + p * plane + qc);
+        };
+        auto load_e = [&](int q0) {
+            pmask = 0;
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u, ec = min(e, NE - 1);
+                const int o = ec / (kQ * RT), rem = ec - o * (kQ * RT);
+                const int qq = rem / RT, r = rem - qq * RT;
+                pe[u] = E[((size_t)o * Q + min(q0 + qq, Q - 1)) * R + rb0 + min(r, rv - 1)];
+                pmask |= (unsigned)((NE % 256 == 0 || e < NE) & (q0 + qq < Q) & (r < rv)) << u;
+            }
+        };
+        // one 32-q chunk: stage Eall's (split when stored), then the next
+        // chunk's loads, then the products from the current patterns (cur);
+        // chunks alternate between two register sets, so nothing is copied
+        // and the next patterns stay in flight during the products
+        auto chunk = [&](int q0, const uint4* cur, uint4* nxt) {
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = threadIdx.x + 256 * u;
+                if (NE % 256 == 0 || e < NE) {
+                    const int o = e / (kQ * RT), rem = e - o * (kQ * RT);
+                    const int qq = rem / RT, r = rem - qq * RT;
+                    const float2 v = ((pmask >> u) & 1u) ? pe[u] : make_float2(0.f, 0.f);
+                    const unsigned hr = bf16b(v.x), hi = bf16b(v.y);
+                    es[0][o][0][r][qq] = (unsigned short)hr;
+                    es[0][o][1][r][qq] = (unsigned short)hi;
+                    es[1][o][0][r][qq] = (unsigned short)bf16b(v.x - bf16f(hr));
+                    es[1][o][1][r][qq] = (unsigned short)bf16b(v.y - bf16f(hi));
+                }
+            }
+            __syncthreads();
+            load_e(q0 + kQ);
+            load_a(q0 + kQ, nxt);
+            if (!wact) return;
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const int rc = ct * 8 + (li & 7);
+#pragma unroll
+                for (int o = 0; o < NO; ++o) {
+                    floatx4 d = acc[ct][o];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {   // E part: hi, then lo (lo meets only the hi patterns)
+                        const uint4 xa = *reinterpret_cast<const uint4*>(&es[h][o][part][rc][8 * lk]);
+                        const uint4 xo = *reinterpret_cast<const uint4*>(&es[h][o][1 - part][rc][8 * lk]);
+                        const uint4 b1 = xa, b2 = part ? xo : neg8b(xo);
+                        d = mfb(cur[0], b1, d);
+                        d = mfb(cur[1], b2, d);
+                        if (h == 0) {
+                            d = mfb(cur[2], b1, d);
+                            d = mfb(cur[3], b2, d);
+                        }
+                    }
+                    acc[ct][o] = d;
+                }
+            }
+        };
+        uint4 aa[4], ab[4];
+        load_e(0);
+        load_a(0, aa);
+        // chunks in pairs (a chunk past Qp reads clamped patterns and meets zero-staged Eall)
+        for (int q0 = 0; q0 < Qp; q0 += 2 * kQ) {
+            chunk(q0, aa, ab);
+            chunk(q0 + kQ, ab, aa);
+        }
+    }
+    if (!wact) return;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int o = 0; o < NO; ++o)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float re = acc[ct][o][v];
+                const float im = __shfl_xor(re, 8, 64);
+                const int row = (rp + wave) * 16 + 4 * lk + v;
+                const int rc = ct * 8 + li;
+                if (part == 0 && rc < rv && row < nt) {
+                    double2* yp = y + ((size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0)) * R + rb0 + rc;
+                    double2 tt = *yp;
+                    tt.x += (double)re;
+                    tt.y += (double)im;
+                    *yp = tt;
+                }
+            }
+}
+
+void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const unsigned short* P1,
+                       const float2* E1, int K2, const unsigned short* P2, const float2* E2, double2* y) {
+    if (nd.nitem == 0 || R < 1) return;
+    const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
+    dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
+#define DB(NO, CT) k_disagg_bf3<NO, CT><<<grid, 256, 0, st>>>(nd, R, NO, K1, P1, E1, K2, P2, E2, y)
+#define DBN(NO)                                                                                                       \
+    if (ct == 4) DB(NO, 4);                                                                                           \
+    else if (ct == 2) DB(NO, 2);                                                                                      \
+    else DB(NO, 1);
+    if (pmchwt) {
+        DBN(2)
+    } else {
+        DBN(1)
+    }
+#undef DBN
+#undef DB
+}
+
 // ---------------------------------------------------------------- aggregation, all right-hand sides (FP32)
 // Code/ProcessMLFMM.h:33-56 (mexp) for all R right-hand sides in one product:
 // per leaf, S (3K x ncur R) = V^T (3K x nt) X (nt x ncur R) on the matrix

@@ -122,6 +122,12 @@ void launch_beff_all(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, cons
                      double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, float2* Eall);
 void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const float2* V1,
                        const float2* E1, int K2, const float2* V2, const float2* E2, double2* y);
+// Split-pair BF16 disaggregation of all R: pattern planes [Re hi, Im hi,
+// Re lo, Im lo][N][Qp] (Qp = 3K rounded up to 32) from the FP32 patterns,
+// and the product from Eall (FP32, split when staged).
+void launch_split_rows(hipStream_t st, int N, int Q, int Qp, const float2* V, unsigned short* planes);
+void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const unsigned short* P1,
+                       const float2* E1, int K2, const unsigned short* P2, const float2* E2, double2* y);
 // Aggregation of all R right-hand sides (FP32 patterns, x RHS-fastest with
 // stride R) into the leaf workspaces of every pass of the batched matvec
 // (passes of rb = largest power of two <= cap, pass (r0, rb) at
