@@ -64,7 +64,8 @@ typedef struct mlfma_params {
                                products; products of 2 or more right-hand sides on the BF16 matrix
                                cores from split pairs hi + lo of entries and vectors, three partial
                                products, FP32 accumulation: about 2^-17 relative, within the FP64
-                               refinement of FP32-mode solves) */
+                               refinement of FP32-mode solves; with far_precision = MLFMA_FP32 the
+                               batched disaggregation runs on split pairs of the patterns too) */
     int tf_full_order;      /* 0: reference series length min(L, ceil|kR|) (Code/AlgoMLFMM.h:263); 1: L */
     int max_rhs;            /* largest batch a single apply/solve will be given */
     int device;             /* HIP device ordinal */
