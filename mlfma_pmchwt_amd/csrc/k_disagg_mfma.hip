@@ -428,11 +428,17 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
     constexpr int kQ = 32, kP = kQ + 8;   // staged q per chunk, padded LDS row (80 B)
     constexpr int NE = NO * kQ * RT, NU = (NE + 255) / 256;
     __shared__ __align__(16) unsigned short es[2][NO][2][RT][kP];   // [hi, lo][output][Re, Im][rhs][q]
-    const int2 item = nd.items[blockIdx.x];
+    // the ny right-hand-side groups of one item run 8 blocks apart, on one
+    // XCD, so the later groups read the item's pattern planes from its L2
+    const int ny = (R + RT - 1) / RT;
+    const int slot = blockIdx.x / (8 * ny), j8 = blockIdx.x - slot * 8 * ny;
+    const int it = slot * 8 + (j8 & 7);
+    if (it >= nd.nitem) return;
+    const int2 item = nd.items[it];
     const int t = item.x, rp = item.y;
     const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
     const int nrp = min(4, ((nt + 15) >> 4) - rp);
-    const int rb0 = blockIdx.y * RT;
+    const int rb0 = (j8 >> 3) * RT;
     const int rv = min(RT, R - rb0);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
@@ -548,7 +554,8 @@ void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, in
                        const float2* E1, int K2, const unsigned short* P2, const float2* E2, double2* y) {
     if (nd.nitem == 0 || R < 1) return;
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
-    dim3 grid(nd.nitem, (R + 8 * ct - 1) / (8 * ct));
+    const int ny = (R + 8 * ct - 1) / (8 * ct);
+    const unsigned grid = (unsigned)((nd.nitem + 7) / 8) * 8 * ny;   // see the block mapping in k_disagg_bf3
 #define DB(NO, CT) k_disagg_bf3<NO, CT><<<grid, 256, 0, st>>>(nd, R, NO, K1, P1, E1, K2, P2, E2, y)
 #define DBN(NO)                                                                                                       \
     if (ct == 4) DB(NO, 4);                                                                                           \
