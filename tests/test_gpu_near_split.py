@@ -125,7 +125,7 @@ def test_split_batched_solves():
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("R", [5, 21])
+@pytest.mark.parametrize("R", [5, 21, 40])
 def test_split_disaggregation_matches_fp32(name, R):
     """With near_precision = BF16 the all-RHS disaggregation also runs on split
     pairs (patterns split once at set-up, the effective matrices when staged).
