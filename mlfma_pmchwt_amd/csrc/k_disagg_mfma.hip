@@ -592,9 +592,15 @@ __global__ __launch_bounds__(256) void k_aggregate_all(int nleaf, int K, int N, 
     constexpr int NE = kJ * CC, NU = (NE + 255) / 256;
     __shared__ float2 xs[kJ][CC];
     const int Q = 3 * K, QG = (Q + 63) / 64;
-    const int leaf = blockIdx.x / QG, qg = blockIdx.x - leaf * QG;
+    // the column groups of one (leaf, sample group) run 8 blocks apart, on one
+    // XCD, so the later ones read its pattern rows from that XCD's L2
+    const int ncol = NCUR * R, ncg = (ncol + CC - 1) / CC;
+    const int slot = blockIdx.x / (8 * ncg), j8 = blockIdx.x - slot * 8 * ncg;
+    const int lq = slot * 8 + (j8 & 7);
+    if (lq >= nleaf * QG) return;
+    const int leaf = lq / QG, qg = lq - leaf * QG;
     const int i0 = leaf_ptr[leaf], nt = leaf_ptr[leaf + 1] - i0;
-    const int ncol = NCUR * R, cb0 = blockIdx.y * CC;
+    const int cb0 = (j8 >> 3) * CC;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;
@@ -684,7 +690,8 @@ void launch_aggregate_all(hipStream_t st, int ncur, int nleaf, int K, int N, int
     if (nleaf == 0 || R < 1) return;
     const int ncol = ncur * R;
     const int ct = ncol > 32 ? 8 : (ncol > 16 ? 4 : (ncol > 8 ? 2 : 1));
-    dim3 grid(nleaf * ((3 * K + 63) / 64), (ncol + 8 * ct - 1) / (8 * ct));
+    const int nlq = nleaf * ((3 * K + 63) / 64), ncg = (ncol + 8 * ct - 1) / (8 * ct);
+    const unsigned grid = (unsigned)((nlq + 7) / 8) * 8 * ncg;   // see the block mapping in k_aggregate_all
 #define AA(NC, CT) k_aggregate_all<NC, CT><<<grid, 256, 0, st>>>(nleaf, K, N, R, cap, leaf_ptr, perm, Vs, x, Sall)
 #define AAN(NC)                                                                                                       \
     if (ct == 8) AA(NC, 8);                                                                                           \
