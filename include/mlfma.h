@@ -49,6 +49,14 @@ typedef struct mlfma_handle mlfma_handle;
 #define MLFMA_FP32 1
 #define MLFMA_BF16 2
 
+/* mlfma_params.tf_mode bits.  MLFMA_TF_COMPLEX_ARG is the accuracy mode for lossy media: the
+ * spherical Hankel functions of the translation series are evaluated at the complex kR, as the
+ * Green's function exp(-jkR)/R of a lossy medium requires; the reference's real |kR| drops the
+ * medium's decay between distant boxes (25% matvec error on a 6-level lossy PMCHWT tree, 1.3% in
+ * this mode).  MLFMA_TF_FULL_ORDER keeps all L terms (unstable where L > |kR|; study only). */
+#define MLFMA_TF_FULL_ORDER 1
+#define MLFMA_TF_COMPLEX_ARG 2
+
 typedef struct mlfma_params {
     int integral_equation;  /* MLFMA_EFIE / MLFMA_CFIE / MLFMA_PMCHWT */
     int gauss_points;       /* Dunavant rule: 1,3,4,6,7,12,13,16 (Code/GaussPoints.cpp:286-301) */
@@ -66,7 +74,9 @@ typedef struct mlfma_params {
                                products, FP32 accumulation: about 2^-17 relative, within the FP64
                                refinement of FP32-mode solves; with far_precision = MLFMA_FP32 the
                                batched disaggregation runs on split pairs of the patterns too) */
-    int tf_full_order;      /* 0: reference series length min(L, ceil|kR|) (Code/AlgoMLFMM.h:263); 1: L */
+    int tf_mode;            /* translation operators, a bit set of MLFMA_TF_*: 0 = the reference's series
+                               (parity mode, Code/AlgoMLFMM.h:262-265): length min(L, ceil|kR|), Hankel
+                               functions at the real |kR| even for a lossy medium */
     int max_rhs;            /* largest batch a single apply/solve will be given */
     int device;             /* HIP device ordinal */
     int verbose;            /* 1: setup timings on stderr */

@@ -20,6 +20,7 @@ FP64, FP32, BF16 = 0, 1, 2
 PRECOND_NONE, PRECOND_BLOCK_DIAG, PRECOND_POLY = 0, 1, 2
 PRECOND_FEATURES = 20
 GMRES, CGS = 0, 1
+TF_FULL_ORDER, TF_COMPLEX_ARG = 1, 2   # mlfma_params.tf_mode bits (include/mlfma.h)
 STAGE_AGGREGATE, STAGE_M2M, STAGE_M2L_L2L, STAGE_FAR, STAGE_NEAR = range(5)
 
 ERRORS = {1: "MLFMA_ERR_ARG", 2: "MLFMA_ERR_HIP", 3: "MLFMA_ERR_OOM", 4: "MLFMA_ERR_NONFINITE",
@@ -30,7 +31,7 @@ class Params(C.Structure):
     _fields_ = [("integral_equation", C.c_int), ("gauss_points", C.c_int), ("frequency_hz", C.c_double),
                 ("eps_r_re", C.c_double), ("eps_r_im", C.c_double), ("mu_r_re", C.c_double),
                 ("mu_r_im", C.c_double), ("cfie_alpha", C.c_double), ("leaf_coeff", C.c_double),
-                ("far_precision", C.c_int), ("near_precision", C.c_int), ("tf_full_order", C.c_int),
+                ("far_precision", C.c_int), ("near_precision", C.c_int), ("tf_mode", C.c_int),
                 ("max_rhs", C.c_int),
                 ("device", C.c_int), ("verbose", C.c_int), ("precond_precision", C.c_int)]
 

@@ -93,7 +93,7 @@ void mlfma_params_default(mlfma_params* p) {
     p->far_precision = MLFMA_FP64;
     p->near_precision = MLFMA_FP64;
     p->precond_precision = MLFMA_FP64;
-    p->tf_full_order = 0;
+    p->tf_mode = 0;
     p->max_rhs = 1;
     p->device = 0;
     p->verbose = 0;

@@ -71,16 +71,17 @@ Engine::~Engine() {
     for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : ev_)
         if (e) hipEventDestroy(e);
-    for (void* a : allocs_) hipFree(a);
+    for (auto& a : allocs_) hipFree(a.first);
     if (st) hipStreamDestroy(st);
 }
 
 template <typename T>
 T* Engine::dalloc(size_t count) {
+    if (capturing_) throw Error(MLFMA_ERR_STATE, "device allocation inside a recorded matvec");
     if (count == 0) count = 1;
     void* p = nullptr;
     MLFMA_HIP(hipMalloc(&p, count * sizeof(T)));
-    allocs_.push_back(p);
+    allocs_[p] = count * sizeof(T);
     dev_bytes += count * sizeof(T);
     return static_cast<T*>(p);
 }
@@ -112,8 +113,8 @@ void Engine::build(int np, const int* ids, const double* xyz, int nf, const int*
         max_leaf = std::max(max_leaf, tree.leaf_ptr[t + 1] - tree.leaf_ptr[t]);
         for (int i = tree.leaf_ptr[t]; i < tree.leaf_ptr[t + 1]; ++i) leaf_of[i] = t;
     }
-    build_wave_tables(tree, med.k1, std::abs(med.k1), P.tf_full_order != 0, W[0]);
-    if (nw == 2) build_wave_tables(tree, med.k2, std::abs(med.k2), P.tf_full_order != 0, W[1]);
+    build_wave_tables(tree, med.k1, std::abs(med.k1), P.tf_mode, W[0]);
+    if (nw == 2) build_wave_tables(tree, med.k2, std::abs(med.k2), P.tf_mode, W[1]);
     const double t1 = wall();
 
     std::vector<double> l1, l2, l3, w;
@@ -762,13 +763,16 @@ void Engine::run_graph(const GraphKey& key, const std::function<void()>& body) {
     if (it == graphs_.end()) {
         hipGraph_t g = nullptr;
         MLFMA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
         try {
             body();
         } catch (...) {
+            capturing_ = false;
             hipStreamEndCapture(st, &g);
             if (g) hipGraphDestroy(g);
             throw;
         }
+        capturing_ = false;
         MLFMA_HIP(hipStreamEndCapture(st, &g));
         hipGraphExec_t ex = nullptr;
         const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -780,6 +784,7 @@ void Engine::run_graph(const GraphKey& key, const std::function<void()>& body) {
 }
 
 void Engine::clear_graphs() {
+    if (capturing_) throw Error(MLFMA_ERR_STATE, "workspace resize inside a recorded matvec");
     if (!graphs_.empty()) hipStreamSynchronize(st);
     for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     graphs_.clear();
@@ -867,8 +872,12 @@ void Engine::set_tuning(int prec, int rb, int bits) {
 
 void Engine::dfree(void* p) {
     if (!p) return;
-    auto it = std::find(allocs_.begin(), allocs_.end(), p);
-    if (it != allocs_.end()) allocs_.erase(it);
+    if (capturing_) throw Error(MLFMA_ERR_STATE, "device free inside a recorded matvec");
+    auto it = allocs_.find(p);
+    if (it != allocs_.end()) {
+        dev_bytes -= it->second;
+        allocs_.erase(it);
+    }
     MLFMA_HIP(hipStreamSynchronize(st));
     MLFMA_HIP(hipFree(p));
 }
@@ -1189,6 +1198,10 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
     if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
     bool pc = sp.precond == 2;
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "batched solve: 1..256 right-hand sides");
+    // every workspace a batched matvec of R may use is sized here, before any
+    // recorded matvec could reach it (a resize inside a capture is an error)
+    ensure_xf(R);
+    ensure_eall(R);
     const size_t nR = (size_t)n * R;
     stats_fallbacks_ = 0;
     if (bV_cols_ < m + 1 || bV_R_ != R) {

@@ -6,6 +6,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mlfma.h"
@@ -188,7 +189,8 @@ private:
     void near_block(const double2* X, double2* Y, int R, int prec, bool precond = false);
     double norm(const double2* v, int n);
 
-    std::vector<void*> allocs_;
+    std::unordered_map<void*, size_t> allocs_;   // device allocation -> bytes
+    bool capturing_ = false;                      // inside run_graph's stream capture
     std::vector<LevelDev> lv_;
     WaveDev wd_[2];
     int* d_perm_ = nullptr;

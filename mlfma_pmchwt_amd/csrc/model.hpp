@@ -106,12 +106,13 @@ void build_basis(const Mesh& m, Basis& b);
 void build_octree(const Basis& b, double wavelength, double leaf_coeff, Octree& t);
 
 Medium make_medium(double freq, cplx epsr, cplx mur, bool dielectric);
-void build_wave_tables(const Octree& t, cplx k, double kabs, bool full_order, WaveTables& w);
+void build_wave_tables(const Octree& t, cplx k, double kabs, int tf_mode, WaveTables& w);
 
 // Dunavant rule (weights halved), Code/GaussPoints.cpp:29-284
 bool dunavant(int n, std::vector<double>& l1, std::vector<double>& l2, std::vector<double>& l3,
               std::vector<double>& w);
 void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
 void spherical_jy(int L, double x, std::vector<double>& j, std::vector<double>& y);
+void spherical_h2(int L, cplx z, std::vector<cplx>& h);
 
 }  // namespace mlfma

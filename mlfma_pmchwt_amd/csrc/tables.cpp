@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "mlfma.h"
 #include "model.hpp"
 
 namespace mlfma {
@@ -125,6 +126,18 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
     }
 }
 
+// h2_l(z) = j_l(z) - j y_l(z) for complex z: h2_0 = j e^{-jz}/z, h2_1 = e^{-jz}(j/z^2 - 1/z),
+// then h_{l+1} = (2l+1)/z h_l - h_{l-1} upward (stable: the Hankel functions are the dominant
+// solution of the recurrence once l passes |z|).
+void spherical_h2(int L, cplx z, std::vector<cplx>& h) {
+    h.assign(L + 1, cplx(0, 0));
+    const cplx j(0.0, 1.0);
+    const cplx e = std::exp(-j * z);
+    h[0] = j * e / z;
+    if (L > 0) h[1] = e * (j / (z * z) - 1.0 / z);
+    for (int l = 1; l < L; ++l) h[l + 1] = (2.0 * l + 1.0) / z * h[l] - h[l - 1];
+}
+
 // j_l by Miller's backward recurrence (normalised on j0 or j1), y_l forward.
 void spherical_jy(int L, double x, std::vector<double>& j, std::vector<double>& y) {
     j.assign(L + 1, 0.0);
@@ -176,7 +189,7 @@ static double lagrange(double x0, double x1, double x2, double x3) {
     return ((x0 - x2) * (x0 - x3)) / ((x1 - x2) * (x1 - x3));
 }
 
-void build_wave_tables(const Octree& t, cplx k, double kabs, bool full_order, WaveTables& W) {
+void build_wave_tables(const Octree& t, cplx k, double kabs, int tf_mode, WaveTables& W) {
     const int L = t.max_level;
     W.k = k;
     W.grid.assign(L + 1, Grid());
@@ -269,7 +282,8 @@ void build_wave_tables(const Octree& t, cplx k, double kabs, bool full_order, Wa
     }
 
     // Code/AlgoMLFMM.h:192-295: sum_{l<=L'} (-j)^l (2l+1) h2_l(|kR|) P_l(Rhat.khat),
-    // L' = min(L, ceil|kR|) (real Hankel argument even when k is complex).
+    // L' = min(L, ceil|kR|) (real Hankel argument even when k is complex).  tf_mode bit 1
+    // keeps L' = L; bit 2 evaluates h2_l at the complex kR (accuracy mode for lossy media).
     for (int l = 2; l <= L; ++l) {
         const Grid& g = W.grid[l];
         const auto& vecs = t.lv[l].drvec;
@@ -280,12 +294,18 @@ void build_wave_tables(const Octree& t, cplx k, double kabs, bool full_order, Wa
             double R = std::sqrt(vecs[v].x * vecs[v].x + vecs[v].y * vecs[v].y + vecs[v].z * vecs[v].z);
             double rx = vecs[v].x / R, ry = vecs[v].y / R, rz = vecs[v].z / R;
             cplx kR = k * R;
-            int Lv = full_order ? g.L : std::min(g.L, (int)std::ceil(std::abs(kR)));
-            std::vector<double> jb, yb;
-            spherical_jy(Lv, std::abs(kR), jb, yb);
+            int Lv = (tf_mode & MLFMA_TF_FULL_ORDER) ? g.L : std::min(g.L, (int)std::ceil(std::abs(kR)));
+            std::vector<cplx> h2(Lv + 1);
+            if (tf_mode & MLFMA_TF_COMPLEX_ARG) {
+                spherical_h2(Lv, kR, h2);
+            } else {
+                std::vector<double> jb, yb;
+                spherical_jy(Lv, std::abs(kR), jb, yb);
+                for (int n = 0; n <= Lv; ++n) h2[n] = cplx(jb[n], -yb[n]);
+            }
             std::vector<cplx> coef(Lv + 1);
             const cplx mj[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
-            for (int n = 0; n <= Lv; ++n) coef[n] = mj[n & 3] * cplx(jb[n], -yb[n]) * (2.0 * n + 1.0);
+            for (int n = 0; n <= Lv; ++n) coef[n] = mj[n & 3] * h2[n] * (2.0 * n + 1.0);
             for (int q = 0; q < K; ++q) {
                 double x = rx * g.kp[q].x + ry * g.kp[q].y + rz * g.kp[q].z;
                 double p0 = 1.0, p1 = x;

@@ -115,6 +115,7 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
 double legendre_p(int l, double x);
 double legendre_p_prime(int l, double x);
 void sph_jy(int L, double x, std::vector<double>& j, std::vector<double>& y);
+void sph_h2(int L, cd z, std::vector<cd>& h);
 
 Wave make_wave(double freq, cd epsr, cd mur, bool dielectric);
 PlaneWave init_pw(double pol_deg, double th_deg, double ph_deg);

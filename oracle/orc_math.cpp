@@ -159,6 +159,21 @@ void sph_jy(int L, double x, std::vector<double>& j, std::vector<double>& y) {
     for (int l = 0; l <= L; ++l) j[l] = f[l] * scale;
 }
 
+// Spherical Hankel functions of the second kind h2_l(z), l = 0..L, for a complex
+// argument z (accuracy mode of the translation operator; the reference evaluates
+// its series at the real |kR|, Code/AlgoMLFMM.h:262-265).  Closed forms
+// h2_0 = j e^{-jz}/z, h2_1 = e^{-jz}(j/z^2 - 1/z) and the upward recurrence
+// h_{l+1} = (2l+1)/z h_l - h_{l-1}, which is stable for the Hankel functions
+// (they are the dominant solution once l exceeds |z|).
+void sph_h2(int L, cd z, std::vector<cd>& h) {
+    h.assign(L + 1, cd(0, 0));
+    const cd j(0.0, 1.0);
+    cd e = std::exp(-j * z);
+    h[0] = j * e / z;
+    if (L >= 1) h[1] = e * (j / (z * z) - 1.0 / z);
+    for (int l = 1; l < L; ++l) h[l + 1] = (2.0 * l + 1.0) / z * h[l] - h[l - 1];
+}
+
 // Code/EMSource.cpp:6-32 -- k1 = 2 pi / lambda; k2 = k1 sqrt(eps mu);
 // etai = sqrt(mu/eps); eta0 = 119.9169832 pi.
 Wave make_wave(double freq, cd epsr, cd mur, bool dielectric) {

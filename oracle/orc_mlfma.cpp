@@ -119,7 +119,8 @@ void Oracle::build_interp() {
 
 // Code/AlgoMLFMM.h:192-295 -- TF = sum_{l<=L'} (-j)^l (2l+1) h2_l(|kR|) P_l(Rhat.khat),
 // L' = min(L_level, ceil|kR|), Hankel argument real |kR| even for complex k.
-static void build_tf_tables(const Tree& t, const std::vector<Grid>& g, cd k, bool full,
+// mode bit 1: L' = L_level; bit 2: h2_l at the complex kR (accuracy mode).
+static void build_tf_tables(const Tree& t, const std::vector<Grid>& g, cd k, int mode,
                             std::vector<std::vector<std::vector<cd>>>& TF) {
     int span = t.maxLevel - 1;
     TF.assign(span, {});
@@ -137,9 +138,15 @@ static void build_tf_tables(const Tree& t, const std::vector<Grid>& g, cd k, boo
             r[1] /= R;
             r[2] /= R;
             cd kR = k * R;
-            int L = full ? G.L : std::min(G.L, (int)std::ceil(std::abs(kR)));
-            std::vector<double> jv, yv;
-            sph_jy(L, std::abs(kR), jv, yv);
+            int L = (mode & 1) ? G.L : std::min(G.L, (int)std::ceil(std::abs(kR)));
+            std::vector<cd> h(L + 1);
+            if (mode & 2) {
+                sph_h2(L, kR, h);
+            } else {
+                std::vector<double> jv, yv;
+                sph_jy(L, std::abs(kR), jv, yv);
+                for (int l = 0; l <= L; ++l) h[l] = cd(jv[l], -yv[l]);
+            }
             std::vector<double> P(L + 1);
             for (int q = 0; q < K; ++q) {
                 double x = r[0] * G.kp[q].x + r[1] * G.kp[q].y + r[2] * G.kp[q].z;
@@ -147,7 +154,7 @@ static void build_tf_tables(const Tree& t, const std::vector<Grid>& g, cd k, boo
                 if (L >= 1) P[1] = x;
                 for (int l = 1; l < L; ++l) P[l + 1] = ((2.0 * l + 1.0) * x * P[l] - l * P[l - 1]) / (l + 1.0);
                 cd sum(0, 0);
-                for (int l = 0; l <= L; ++l) sum += jl[l & 3] * cd(jv[l], -yv[l]) * (2.0 * l + 1.0) * P[l];
+                for (int l = 0; l <= L; ++l) sum += jl[l & 3] * h[l] * (2.0 * l + 1.0) * P[l];
                 TF[level][v][q] = sum;
             }
         }
@@ -155,8 +162,8 @@ static void build_tf_tables(const Tree& t, const std::vector<Grid>& g, cd k, boo
 }
 
 void Oracle::build_tf() {
-    build_tf_tables(tree, g1, wave.k1, tfFull != 0, TF1);
-    if (ie == 2) build_tf_tables(tree, g2, wave.k2, tfFull != 0, TF2);
+    build_tf_tables(tree, g1, wave.k1, tfFull, TF1);
+    if (ie == 2) build_tf_tables(tree, g2, wave.k2, tfFull, TF2);
 }
 
 // Code/ProcessFMM.h:17-180 -- radiation (Vsmi) and receiving (Vfmj) patterns of

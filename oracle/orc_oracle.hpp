@@ -10,7 +10,10 @@ using Ws = std::vector<std::vector<std::vector<C3>>>;   // [level][node][k], lev
 struct Oracle {
     int ie = 2;              // 0 EFIE, 1 CFIE, 2 PMCHWT (Code/RCSExportConfig.h:17)
     double alpha = 0.2;      // CFIE weight (Code/PhyConstant.h:15)
-    int tfFull = 0;          // 1: translation series to L_level (not the reference's truncation)
+    // Translation-operator mode, a bit set.  0 = the reference's series (parity):
+    // length min(L, ceil|kR|), Hankel functions at the real |kR|.  Bit 1: length L.
+    // Bit 2: Hankel functions at the complex kR (accuracy mode for lossy media).
+    int tfFull = 0;
     Mesh mesh;
     std::vector<Rwg> rwgs;
     Tree tree;

@@ -83,18 +83,21 @@ def _p(a):
 
 
 class Oracle:
-    """mode: 1 patterns, 2 near field, 4 synthetic values (timing only)."""
+    """mode: 1 patterns, 2 near field, 4 synthetic values (timing only).
+
+    tf_mode: 0 = the reference's translation series (parity); bit 1 = all L terms;
+    bit 2 = Hankel functions at the complex kR (accuracy mode for lossy media)."""
 
     def __init__(self, mesh, ie=2, freq=299792458.0, eps=(1.0, 0.0), mu=(1.0, 0.0), alpha=0.2, leaf=0.3,
-                 gauss=7, tf_full=0, mode=3):
+                 gauss=7, tf_mode=0, mode=3):
         lib = library()
         if isinstance(mesh, (str, Path)):
             h = lib.orc_create_nas(str(mesh).encode(), ie, freq, eps[0], eps[1], mu[0], mu[1], alpha, leaf,
-                                   gauss, tf_full, mode)
+                                   gauss, tf_mode, mode)
         else:
             h = lib.orc_create(mesh.n_points, _p(mesh.point_ids), _p(mesh.xyz), mesh.n_faces, _p(mesh.face_ids),
                                _p(mesh.faces), ie, freq, eps[0], eps[1], mu[0], mu[1], alpha, leaf, gauss,
-                               tf_full, mode)
+                               tf_mode, mode)
         if not h:
             raise RuntimeError("oracle construction failed")
         self._h = C.c_void_p(h)

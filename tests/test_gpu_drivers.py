@@ -42,6 +42,27 @@ def test_monostatic_equals_per_direction(ie):
     op.close()
 
 
+def test_monostatic_graphs_fp32_after_smaller_batch():
+    """A monostatic sweep with recorded matvecs in FP32 mode after an earlier call sized the
+    FP32 workspaces for fewer right-hand sides: the workspaces are sized before any recording,
+    and the sweep equals the same sweep run eagerly."""
+    mesh = geometry.icosphere(2, 0.5)
+    th, ph = api.scatter_grid(0, 180, 0, 0, 30)
+    out = []
+    for graphs in (False, True):
+        op = api.Operator(mesh, integral_equation=api.PMCHWT, eps_r_re=4.0, eps_r_im=-0.1, far_precision=api.FP32,
+                          near_precision=api.FP32, max_rhs=8)
+        op.set_apply_precision(api.FP32)
+        op.apply(np.ones((op.n, 2), dtype=np.complex128))   # FP32 workspaces sized for 2
+        op.set_graphs(graphs)
+        rcs, st = op.monostatic(th, ph, pol=0.0, tol_rel=1e-6, tol_abs=0.0)
+        assert all(s["converged"] for s in st)
+        out.append(rcs)
+        op.close()
+    print(f"[mono] graphs vs eager, FP32 mode: max {np.max(np.abs(out[0] - out[1])):.2e} dB")
+    assert np.max(np.abs(out[0] - out[1])) < 1e-6
+
+
 def _write_meshes(tmp_path):
     paths = []
     for k, (s, r) in enumerate([(2, 0.4), (2, 0.6)]):

@@ -164,7 +164,7 @@ def test_addition_theorem():
     This fixes the sign conventions of TF against the e^{-jkR}/R kernel of the
     near field (Code/Operator.h:803)."""
     m = geometry.icosphere(2, 1.2)
-    o = pyoracle.Oracle(m, ie=0, tf_full=1, mode=0)
+    o = pyoracle.Oracle(m, ie=0, tf_mode=1, mode=0)
     k = o.wave()["k1"].real
     lvl = o.max_level
     L, th, ph, wgl, dphi, kp = o.grid(1, lvl)
@@ -181,6 +181,97 @@ def test_addition_theorem():
             Rd = np.linalg.norm(X + d)
             exact = np.exp(-1j * k * Rd) / Rd
             assert abs(approx - exact) / abs(exact) < 1e-3
+
+
+def test_tf_complex_argument_closed_form():
+    """Accuracy mode (tf_mode = 2): h2_l at the complex kR of the lossy interior medium, via
+    SciPy's spherical_jn / spherical_yn of a complex argument; the same series length."""
+    m = geometry.icosphere(2, 1.2)
+    o = pyoracle.Oracle(m, ie=2, eps=(2.5, -0.4), tf_mode=2, mode=0)
+    k = o.wave()["k2"]
+    assert abs(k.imag) > 0.1
+    for lvl in range(2, o.max_level + 1):
+        L, _, _, _, _, kp = o.grid(2, lvl)
+        vecs = o.drvec(lvl)
+        tf = o.tf(2, lvl)
+        for v in range(0, vecs.shape[0], 23):
+            R = np.linalg.norm(vecs[v])
+            z = k * R
+            Lv = min(L, math.ceil(abs(z)))
+            ls = np.arange(Lv + 1)
+            h2 = special.spherical_jn(ls, z) - 1j * special.spherical_yn(ls, z)
+            P = special.eval_legendre(ls[:, None], (kp @ (vecs[v] / R))[None, :])
+            ref = np.sum(((-1j) ** ls * (2 * ls + 1) * h2)[:, None] * P, axis=0)
+            assert relerr(tf[v], ref) < 1e-11
+
+
+@pytest.mark.parametrize("tf_mode,ok", [(3, True), (1, False)])
+def test_addition_theorem_lossy(tf_mode, ok):
+    """Lossy medium (complex k): only Hankel functions at the complex kR reproduce the decaying
+    kernel e^{-jk|X+d|}/|X+d| through the plane-wave expansion; the real |kR| of the
+    reference (Code/AlgoMLFMM.h:262-265) misses it by the factor of the medium's decay."""
+    m = geometry.icosphere(2, 1.2)
+    o = pyoracle.Oracle(m, ie=2, eps=(2.5, -0.4), tf_mode=tf_mode, mode=0)
+    k = o.wave()["k2"]
+    lvl = o.max_level
+    L, th, ph, wgl, dphi, kp = o.grid(2, lvl)
+    w = np.repeat(wgl, 2 * L) * dphi
+    vecs = o.drvec(lvl)
+    box = o.level(lvl)[2][0]
+    rng = np.random.default_rng(5)
+    errs = []
+    for v in range(0, vecs.shape[0], 29):
+        X = vecs[v]
+        T = o.tf(2, lvl)[v]
+        for _ in range(3):
+            d = (rng.random(3) - 0.5) * box * 0.9
+            approx = (-1j * k / (4 * np.pi)) * np.sum(w * np.exp(-1j * k * (kp @ d)) * T)
+            Rd = np.linalg.norm(X + d)
+            exact = np.exp(-1j * k * Rd) / Rd
+            errs.append(abs(approx - exact) / abs(exact))
+    if ok:
+        assert max(errs) < 1e-3
+    else:
+        assert min(errs) > 0.05
+
+
+def test_interpolation_rows_reference_rule():
+    """Every row of every interpolation table against the rule the reference documents for
+    find_interp_pos / computeInterpolationMatrix (README/MLFMM插值函数_Fortran对比分析.md:202-255;
+    Code/AlgoMLFMM.h:141-189, 298-368): the nearest coarse sample is the middle tap; at either end
+    of theta or phi the neighbour index wraps (n-1, 0, 1 and n-2, n-1, 0); the three Lagrange
+    weights use the raw, non-unwrapped coordinates of those taps.  The wrap rows are closed-form
+    values of that rule, so unwrapping phi (or clamping theta) fails here."""
+    m = geometry.icosphere(2, 1.2)
+    for wn, eps in ((1, (1.0, 0.0)), (2, (4.0, 0.0))):
+        o = pyoracle.Oracle(m, ie=2, eps=eps, mode=0)
+        wrapped = 0
+        for lvl in range(2, o.max_level):
+            taps, w = o.interp(wn, lvl)
+            Lc, thc, phc, _, _, _ = o.grid(wn, lvl + 1)
+            Lp, thp, php, _, _, _ = o.grid(wn, lvl)
+
+            def rule(x, a):
+                n = len(a)
+                c = int(np.argmin(np.abs(a - x)))
+                idx = [(c - 1) % n, c, (c + 1) % n]
+                xs = a[idx]
+                lw = [(x - xs[1]) * (x - xs[2]) / ((xs[0] - xs[1]) * (xs[0] - xs[2])),
+                      (x - xs[0]) * (x - xs[2]) / ((xs[1] - xs[0]) * (xs[1] - xs[2])),
+                      (x - xs[0]) * (x - xs[1]) / ((xs[2] - xs[0]) * (xs[2] - xs[1]))]
+                return idx, np.array(lw), c in (0, n - 1)
+
+            for t in range(Lp):
+                ti, tw, twrap = rule(thp[t], thc)
+                for p in range(2 * Lp):
+                    pi_, pw, pwrap = rule(php[p], phc)
+                    row = t * 2 * Lp + p
+                    exp_taps = [ti[a] * 2 * Lc + pi_[b] for a in range(3) for b in range(3)]
+                    exp_w = np.outer(tw, pw).ravel()
+                    assert list(taps[row]) == exp_taps
+                    assert np.max(np.abs(w[row] - exp_w)) < 1e-12 * max(1.0, np.max(np.abs(exp_w)))
+                    wrapped += twrap or pwrap
+        assert wrapped > 0
 
 
 def test_interpolation_rows():
