@@ -174,7 +174,8 @@ typedef struct mlfma_solve_params {
     double tol_abs;
     int precond;        /* 0 none (reference), 1 near-field block-diagonal, 2 near-field polynomial
                            (coefficients from mlfma_set_precond_poly, e.g. predicted by the network);
-                           mlfma_solve_batch accepts 0 and 2 */
+                           mlfma_solve_batch accepts all three; a solve replaces a stalling
+                           preconditioner by the next plainer one (2 -> 1 -> 0) */
 } mlfma_solve_params;
 
 typedef struct mlfma_solve_stats {
@@ -196,7 +197,7 @@ int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const doub
  * doubles); stats has nrhs entries; returns MLFMA_ERR_NOT_CONVERGED if any
  * right-hand side did not converge.  Replaces the sequential per-incidence
  * MLFMM::matrix_solver calls of the monostatic sweeps (Code/RCS.h:272-325,
- * 428-497) and of batch mode (Code/BatchRunner.h:422-484); precond must be 0. */
+ * 428-497) and of batch mode (Code/BatchRunner.h:422-484); precond may be 0, 1 or 2. */
 int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int nrhs, const double* rhs, double* x,
                       mlfma_solve_stats* stats);
 

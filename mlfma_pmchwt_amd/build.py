@@ -60,7 +60,19 @@ def build(incremental: bool = False, jobs: int | None = None) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_caller()
     return LIB
+
+
+def build_cpp_caller() -> Path:
+    """The C++ drop-in caller test program (tests/cpp/dropin_gmres.cpp) against libmlfma.so."""
+    exe = OUT / "dropin_gmres"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "dropin_gmres.cpp"),
+           "-o", str(exe), f"-L{PKG}", "-lmlfma", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return exe
 
 
 if __name__ == "__main__":

@@ -1103,13 +1103,17 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
             if (err < sp.tol_rel || err < sp.tol_abs) converged = true;
             if (total >= sp.itr_max) break;
             if (err > 1000.0 * sp.tol_rel && cyc > 8) break;   // Code/GMRES.h:208-214
-            // fallback: a polynomial preconditioner that does not reduce the true
-            // residual by 10% over a whole cycle is dropped (plain GMRES from here)
-            if (!converged && precond_kind_ == 2 && err > 0.9 * err_prev) {
-                precond_kind_ = 0;
-                pc = false;
+            // fallback: a preconditioner that does not reduce the true residual by
+            // 10% over a whole cycle is replaced by the next plainer one: the
+            // polynomial by the near-field block-diagonal, that by none
+            if (!converged && pc && err > 0.9 * err_prev) {
+                precond_kind_ = precond_kind_ == 2 ? 1 : 0;
+                if (precond_kind_ == 1) precond_setup();
+                pc = precond_kind_ != 0;
                 ++stats_fallbacks_;
-                if (P.verbose) std::fprintf(stderr, "[mlfma] polynomial preconditioner stalled: continuing without\n");
+                if (P.verbose)
+                    std::fprintf(stderr, "[mlfma] preconditioner stalled: continuing with %s\n",
+                                 pc ? "the block-diagonal one" : "none");
             }
             err_cycle = err;
         }
@@ -1193,10 +1197,11 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
     const int m = std::max(1, sp.restart);
     const double eps = 1e-16;
     if (sp.solver != MLFMA_SOLVER_GMRES) throw Error(MLFMA_ERR_ARG, "batched solve supports GMRES only");
-    if (sp.precond != 0 && sp.precond != 2)
-        throw Error(MLFMA_ERR_ARG, "batched solve: precond must be 0 or 2 (polynomial)");
+    if (sp.precond < 0 || sp.precond > 2) throw Error(MLFMA_ERR_ARG, "batched solve: precond must be 0, 1 or 2");
     if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
-    bool pc = sp.precond == 2;
+    int pkind = sp.precond;   // 0 none, 1 near-field block-diagonal, 2 polynomial
+    if (pkind == 1) precond_setup();
+    bool pc = pkind != 0;
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "batched solve: 1..256 right-hand sides");
     // every workspace a batched matvec of R may use is sized here, before any
     // recorded matvec could reach it (a resize inside a capture is an error)
@@ -1220,6 +1225,8 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         d_bG_ = dalloc<double2>(nR);
         clear_graphs();   // recorded sequences refer to the freed block buffers
     }
+    // the flexible basis z_k = M^-1 v_k is sized whenever a preconditioner may run
+    // (a stalling polynomial falls back to the block-diagonal one mid-solve)
     if (pc && (bZf_cols_ < m || bZf_nR_ != nR)) {
         dfree(d_bZf_);
         d_bZf_ = dalloc<double2>(nR * m);
@@ -1304,9 +1311,9 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
                 MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bV_ + (size_t)k * nR, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
                 const int prec = apply_prec;
                 if (pc) {
-                    ensure_poly(R);
-                    run_graph(GraphKey{1, R, prec}, [&] {
-                        poly_apply(d_bG_, d_bZ_, R, prec);
+                    if (pkind == 2) ensure_poly(R);
+                    run_graph(GraphKey{pkind == 2 ? 1 : 3, R, prec}, [&] {
+                        precond_block(pkind, d_bG_, d_bZ_, R, prec);
                         apply_block(d_bZ_, d_bW_, R, prec);
                     });
                     MLFMA_HIP(hipMemcpyAsync(d_bZf_ + (size_t)k * nR, d_bZ_, nR * sizeof(double2),
@@ -1315,7 +1322,7 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
                     run_graph(GraphKey{2, R, prec}, [&] { apply_block(d_bG_, d_bW_, R, prec); });
                 }
             } else if (pc) {   // flexible: keep z_k = M^-1 v_k
-                poly_apply(d_bV_ + (size_t)k * nR, d_bZf_ + (size_t)k * nR, R, apply_prec);
+                precond_block(pkind, d_bV_ + (size_t)k * nR, d_bZf_ + (size_t)k * nR, R, apply_prec);
                 apply_block(d_bZf_ + (size_t)k * nR, d_bW_, R, apply_prec);
             } else {
                 apply_block(d_bV_ + (size_t)k * nR, d_bW_, R, apply_prec);
@@ -1409,11 +1416,16 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
             std::fprintf(stderr, "[mlfma] batched gmres cycle: %d of %d converged, max true residual %.3e\n", nd, R,
                          emax);
         }
-        // fallback as in solve(): drop a stalling polynomial preconditioner
+        // fallback as in solve(): a stalling preconditioner is replaced by the
+        // next plainer one (polynomial -> block-diagonal -> none)
         if (pc && !all_done() && !progress) {
-            pc = false;
+            pkind = pkind == 2 ? 1 : 0;
+            if (pkind == 1) precond_setup();
+            pc = pkind != 0;
             ++stats_fallbacks_;
-            if (P.verbose) std::fprintf(stderr, "[mlfma] polynomial preconditioner stalled: continuing without\n");
+            if (P.verbose)
+                std::fprintf(stderr, "[mlfma] preconditioner stalled: continuing with %s\n",
+                             pc ? "the block-diagonal one" : "none");
         }
         err_cycle = err;
         bool stalled = true;   // Code/GMRES.h:208-214, all right-hand sides
@@ -1436,7 +1448,7 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
             launch_gather_cols(st, n, R, Rc, d_idx, d_X, Xc);
             mlfma_solve_params sp2 = sp;
             sp2.itr_max = sp.itr_max - total;
-            if (!pc) sp2.precond = 0;   // a preconditioner dropped above stays dropped
+            sp2.precond = pkind;   // a preconditioner replaced above stays replaced
             const int fb = stats_fallbacks_;
             std::vector<mlfma_solve_stats> st2(Rc);
             if (P.verbose) std::fprintf(stderr, "[mlfma] batched gmres: continuing with %d of %d right-hand sides\n", Rc, R);
@@ -1564,6 +1576,17 @@ void Engine::precond_apply(const double2* x, double2* y) {
         return;
     }
     launch_block_diag(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_Doff_, d_Dinv_, max_leaf, x, y);
+}
+
+// Preconditioner `kind` on an RHS-fastest block of R vectors (apply precision for
+// the polynomial's near products; the block-diagonal inverses are FP64).
+void Engine::precond_block(int kind, const double2* x, double2* y, int R, int prec) {
+    if (kind == 2) {
+        poly_apply(x, y, R, prec);
+        return;
+    }
+    launch_block_diag_b(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, R, d_leaf_ptr_, d_perm_, d_Doff_, d_Dinv_, max_leaf,
+                        x, y);
 }
 
 // ---------------------------------------------------------------- polynomial preconditioner

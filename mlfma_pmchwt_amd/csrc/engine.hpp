@@ -183,6 +183,7 @@ private:
     std::vector<double2> near_block(int p);   // planes x (ns x nt column-major) of pair p
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
+    void precond_block(int kind, const double2* x, double2* y, int R, int prec);
     void ensure_poly(int R);
     void ensure_xf(int R);
     void ensure_eall(int R);

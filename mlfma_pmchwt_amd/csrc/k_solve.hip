@@ -1,5 +1,7 @@
 // Krylov-solver vector kernels (FP64).  Reductions use a fixed block partition
 // and a fixed combination order, so repeated runs give identical results.
+#include <type_traits>
+
 #include "cplx.hpp"
 #include "setup.hpp"
 
@@ -316,6 +318,55 @@ __global__ __launch_bounds__(256) void k_block_diag(int planes, int N, int max_l
         const int r = perm[i0 + (row % nt)];
         y[(row < nt) ? r : N + r] = a;
     }
+}
+
+// Batched form on RHS-fastest blocks (element (u, r) at u * R + r): workgroup
+// (leaf t, group g) applies leaf t's inverse to right-hand sides RB g .. RB g + RB - 1,
+// so each inverse entry is read once per RB right-hand sides (RB as large as the
+// leaf's block of RB right-hand sides fits 64 KiB of LDS).
+template <int RB>
+__global__ __launch_bounds__(256) void k_block_diag_b(int planes, int N, int R, const int* __restrict__ leaf_ptr,
+                                                      const int* __restrict__ perm, const long long* __restrict__ off,
+                                                      const double2* __restrict__ Dinv, const double2* __restrict__ x,
+                                                      double2* __restrict__ y) {
+    extern __shared__ double2 xs[];   // [m][RB]
+    const int t = blockIdx.x, r0 = blockIdx.y * RB;
+    const int nr = min(RB, R - r0);
+    const int i0 = leaf_ptr[t], nt = leaf_ptr[t + 1] - i0;
+    const int m = planes == 1 ? nt : 2 * nt;
+    for (int e = threadIdx.x; e < m * RB; e += blockDim.x) {
+        const int j = e / RB, q = e % RB;
+        const int u = (j < nt) ? perm[i0 + j] : N + perm[i0 + j - nt];
+        xs[e] = q < nr ? x[(size_t)u * R + r0 + q] : make_double2(0, 0);
+    }
+    __syncthreads();
+    const double2* D = Dinv + off[t];
+    for (int row = threadIdx.x; row < m; row += blockDim.x) {
+        double2 a[RB];
+        for (int q = 0; q < RB; ++q) a[q] = make_double2(0, 0);
+        for (int j = 0; j < m; ++j) {
+            const double2 d = D[(size_t)j * m + row];
+            for (int q = 0; q < RB; ++q) cfma(a[q], d, xs[j * RB + q]);
+        }
+        const int u = (row < nt) ? perm[i0 + row] : N + perm[i0 + row - nt];
+        for (int q = 0; q < nr; ++q) y[(size_t)u * R + r0 + q] = a[q];
+    }
+}
+
+void launch_block_diag_b(hipStream_t st, int planes, int nleaf, int N, int R, const int* leaf_ptr, const int* perm,
+                         const long long* inv_off, const double2* Dinv, int max_leaf, const double2* x, double2* y) {
+    const size_t m = planes == 1 ? max_leaf : 2 * (size_t)max_leaf;
+    const size_t lds = 65536;
+    auto go = [&](auto rb) {
+        constexpr int RB = decltype(rb)::value;
+        dim3 grid(nleaf, (R + RB - 1) / RB);
+        k_block_diag_b<RB><<<grid, 256, m * RB * sizeof(double2), st>>>(planes, N, R, leaf_ptr, perm, inv_off, Dinv,
+                                                                         x, y);
+    };
+    if (m * 8 * sizeof(double2) <= lds) go(std::integral_constant<int, 8>{});
+    else if (m * 4 * sizeof(double2) <= lds) go(std::integral_constant<int, 4>{});
+    else if (m * 2 * sizeof(double2) <= lds) go(std::integral_constant<int, 2>{});
+    else go(std::integral_constant<int, 1>{});
 }
 
 void launch_block_diag(hipStream_t st, int planes, int nleaf, int N, const int* leaf_ptr, const int* perm,

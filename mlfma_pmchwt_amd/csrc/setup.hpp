@@ -71,5 +71,8 @@ void launch_scatter_cols(hipStream_t st, int n, int R, int Rc, const int* idx, c
 // block-diagonal preconditioner: y_t = D_t^{-1} x_t per leaf (dense inverse blocks)
 void launch_block_diag(hipStream_t st, int planes, int nleaf, int N, const int* leaf_ptr, const int* perm,
                        const long long* inv_off, const double2* Dinv, int max_leaf, const double2* x, double2* y);
+// the same on RHS-fastest blocks of R right-hand sides (element (u, r) at u * R + r)
+void launch_block_diag_b(hipStream_t st, int planes, int nleaf, int N, int R, const int* leaf_ptr, const int* perm,
+                         const long long* inv_off, const double2* Dinv, int max_leaf, const double2* x, double2* y);
 
 }  // namespace mlfma

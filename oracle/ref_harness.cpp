@@ -2,7 +2,9 @@
 //
 // Checks the oracle against the parts of the reference that compile here
 // without any stand-in: Code/ProcessMLFMM.h (stages), Code/GMRES.h,
-// Code/CGS.h, Code/GaussPoints.cpp and Code/EMSource.cpp (+ Code/Complex3D.cpp).
+// Code/CGS.h, Code/GaussPoints.cpp and Code/EMSource.cpp (+ Code/Complex3D.cpp),
+// and Code/RWGGenerator.cpp (which needs only -fpermissive: its file-static
+// FindPoint differs in linkage from the declaration in Code/NASReader.h).
 // Everything else in the reference needs Boost.Math, which this image lacks.
 //
 // The reference headers are included from /root/reference/Code at build time
@@ -14,6 +16,7 @@
 //   ref_harness stages <nas> <ie> <eps_re> <eps_im> <seed>
 //   ref_harness solvers <nas> <ie> <eps_re> <eps_im>
 //   ref_harness solution <nas> <ie> <eps_re> <eps_im> <out.bin>
+//   ref_harness rwg <nas> [out.bin]   (out: int32 n x 6 file-order indices, then float64 n lengths)
 #include "PhyConstant.h"
 #include "MyStruct.h"
 #include "OCTree.h"
@@ -22,6 +25,7 @@
 #include "CGS.h"
 #include "GaussPoints.h"
 #include "EMSource.h"
+#include "RWGGenerator.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -337,6 +341,54 @@ int cmd_solution(const char* nas, int ie, double er, double ei, const char* out)
     return 0;
 }
 
+// RWG numbering: the reference's RWGGenerator::GenerateRWG on the mesh the
+// oracle read, against the oracle's build_rwg: same count and, per function in
+// order, the same common-edge vertex ids, positive / negative face ids, free
+// vertex ids and edge length (bit-exact).
+int cmd_rwg(const char* nas, const char* out) {
+    orc::Mesh m;
+    if (orc::read_nas(nas, m) != 0) return 2;
+    std::vector<Point> pts(m.p.size());
+    for (size_t i = 0; i < pts.size(); ++i) pts[i] = Point{m.pid[i], m.p[i].x, m.p[i].y, m.p[i].z};
+    std::vector<FaceElement> faces(m.fv.size());
+    for (size_t f = 0; f < faces.size(); ++f) {
+        faces[f].faceId = m.fid[f];
+        faces[f].vertex1 = &pts[m.fv[f][0]];
+        faces[f].vertex2 = &pts[m.fv[f][1]];
+        faces[f].vertex3 = &pts[m.fv[f][2]];
+    }
+    std::vector<RWGBase> ref;
+    if (!RWGGenerator::GenerateRWG(faces, pts, ref)) return 3;
+    std::vector<orc::Rwg> orw;
+    orc::build_rwg(m, orw);
+    long mism = 0;
+    const size_t n = std::min(ref.size(), orw.size());
+    for (size_t i = 0; i < n; ++i) {
+        const RWGBase& a = ref[i];
+        const orc::Rwg& b = orw[i];
+        if (a.commonEdgeVertex1->id != m.pid[b.e0] || a.commonEdgeVertex2->id != m.pid[b.e1] ||
+            a.positiveFace->faceId != m.fid[b.fpos] || a.negativeFace->faceId != m.fid[b.fneg] ||
+            a.freeVertexPositive->id != m.pid[b.vpos] || a.freeVertexNegative->id != m.pid[b.vneg] ||
+            a.edgeLength != b.len || a.rwgid != (int)i)
+            ++mism;
+    }
+    if (out) {   // the reference's numbering as indices: pos face, neg face, edge v1, edge v2, free pos, free neg
+        std::FILE* f = std::fopen(out, "wb");
+        if (!f) return 4;
+        for (const RWGBase& a : ref) {
+            const int v[6] = {(int)(a.positiveFace - faces.data()), (int)(a.negativeFace - faces.data()),
+                              (int)(a.commonEdgeVertex1 - pts.data()), (int)(a.commonEdgeVertex2 - pts.data()),
+                              (int)(a.freeVertexPositive - pts.data()), (int)(a.freeVertexNegative - pts.data())};
+            std::fwrite(v, sizeof(int), 6, f);
+        }
+        for (const RWGBase& a : ref) std::fwrite(&a.edgeLength, sizeof(double), 1, f);
+        std::fclose(f);
+    }
+    std::printf("{\"cmd\":\"rwg\",\"ref_count\":%zu,\"orc_count\":%zu,\"mismatches\":%ld}\n", ref.size(), orw.size(),
+                mism);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -349,6 +401,7 @@ int main(int argc, char** argv) {
                           (unsigned)std::atoi(argv[6]));
     if (c == "solution" && argc >= 7)
         return cmd_solution(argv[2], std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]), argv[6]);
+    if (c == "rwg" && argc >= 3) return cmd_rwg(argv[2], argc >= 4 ? argv[3] : nullptr);
     if (c == "solvers" && argc >= 6) return cmd_solvers(argv[2], std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]));
     return 1;
 }

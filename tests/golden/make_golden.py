@@ -24,6 +24,22 @@ from mlfma_pmchwt_amd import geometry  # noqa: E402
 OUT = Path(__file__).resolve().parent
 
 
+def rwg_meshes():
+    """Meshes whose RWG numbering is taken from the reference's RWGGenerator::GenerateRWG
+    (Code/RWGGenerator.cpp:39-113), including one with shuffled, non-contiguous point and
+    face ids and shuffled face order (the numbering keys on ids, the faces' roles on order)."""
+    rng = np.random.default_rng(21)
+    s = geometry.icosphere(2, 1.2)
+    perm = rng.permutation(s.n_points)
+    ids = (perm * 7 + 3).astype(np.int32)               # point i gets id ids[i]
+    fperm = rng.permutation(s.n_faces)
+    faces = ids[s.faces[fperm] - 1]
+    shuffled = geometry.Mesh(s.xyz, faces, point_ids=ids, face_ids=(fperm * 5 + 11).astype(np.int32))
+    return {"icosphere_s2": geometry.icosphere(2, 1.2), "cube": geometry.cube(1.0, 5),
+            "cylinder": geometry.cylinder(0.4, 1.5, 16, 6, 3),
+            "ellipsoid": geometry.ellipsoid((1.5, 1.0, 0.5), frequency=6), "shuffled_ids": shuffled}
+
+
 def main():
     if not pyoracle.HARNESS.exists():
         pyoracle.build(ref=True)
@@ -38,6 +54,16 @@ def main():
             res["stages"].append(pyoracle.harness("stages", ml, ie, er, ei, 11))
         for ie in (0, 1):
             res["solvers"].append(pyoracle.harness("solvers", s2, ie, 1.0, 0.0))
+        rwg = {}
+        for name, m in rwg_meshes().items():
+            p, out = Path(d) / f"{name}.nas", Path(d) / f"{name}.bin"
+            geometry.write_nas(p, m)
+            res.setdefault("rwg", []).append(dict(pyoracle.harness("rwg", p, out), mesh=name))
+            raw = out.read_bytes()
+            n = len(raw) // (6 * 4 + 8)
+            rwg[f"{name}_ints"] = np.frombuffer(raw[:24 * n], dtype=np.int32).reshape(n, 6)
+            rwg[f"{name}_len"] = np.frombuffer(raw[24 * n:], dtype=np.float64)
+        np.savez_compressed(OUT / "rwg_reference.npz", **rwg)
         (OUT / "reference_checks.json").write_text(json.dumps(res, indent=1) + "\n")
         binf = Path(d) / "sol.bin"
         pyoracle.harness("solution", s2, 0, 1.0, 0.0, binf)

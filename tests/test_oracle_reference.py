@@ -87,3 +87,36 @@ def test_golden_solution_vectors():
     assert np.max(np.abs(b - g["rhs"])) <= 1e-12 * np.max(np.abs(b))
     x, st = orc.gmres(b, restart=int(g["restart"]), tol_rel=float(g["tol"]), tol_abs=1e-12)
     assert np.linalg.norm(x - g["x_ref"]) <= 1e-12 * np.linalg.norm(g["x_ref"])
+
+
+def _rwg_meshes():
+    import sys
+    sys.path.insert(0, str(GOLD))
+    from make_golden import rwg_meshes
+    return rwg_meshes()
+
+
+@needs_harness
+def test_rwg_numbering_matches_reference(tmp_path):
+    """Code/RWGGenerator.cpp compiled from the reference (oracle/Makefile `ref`) against the
+    oracle's build_rwg on the same NAS files: count and every field of every function."""
+    for name, m in _rwg_meshes().items():
+        p = tmp_path / f"{name}.nas"
+        geometry.write_nas(p, m)
+        r = pyoracle.harness("rwg", p)
+        assert r["ref_count"] == r["orc_count"] > 0, name
+        assert r["mismatches"] == 0, name
+
+
+def test_golden_rwg_numbering(tmp_path):
+    """The reference's RWG numbering (tests/golden/rwg_reference.npz, written by the compiled
+    Code/RWGGenerator.cpp): the oracle reproduces faces, edge and free vertices and edge
+    lengths bit for bit, also with shuffled non-contiguous ids and shuffled face order."""
+    gold = np.load(GOLD / "rwg_reference.npz")
+    for name, m in _rwg_meshes().items():
+        p = tmp_path / f"{name}.nas"
+        geometry.write_nas(p, m)
+        o = pyoracle.Oracle(p, ie=0, mode=0)
+        ints, ln, _ = o.rwg()
+        assert np.array_equal(ints, gold[f"{name}_ints"]), name
+        assert np.array_equal(ln, gold[f"{name}_len"]), name
