@@ -35,7 +35,12 @@ __device__ __forceinline__ double2 expmjk(double2 k, double R) {
 }
 
 // Analytic integrals over a flat source triangle of 1/R and 1/R^3 (vector and
-// scalar parts) for one observation point; Code/Operator.h:72-305 and :585-674.
+// scalar parts) for one observation point on that triangle (the identical-face
+// terms); Code/Operator.h:72-305 and :585-674.  The point's height D above the
+// triangle's plane is exactly 0: the reference computes D = n.(r - v) in floating
+// point, and that rounding noise (~1e-16 |r|) picks between the two branches of the
+// 1/R^3 scalar integral, one of which divides an O(1) angle difference by the noise;
+// D = 0 gives the reference's value wherever its noise stays below 10 eps0.
 struct SingFrame {
     R3 rho[3], l[3], u[3], n;
 };
@@ -77,7 +82,7 @@ __device__ void sing_point(R3 r, const double* t, const SingFrame& f, bool want3
             const R3 pm = sub3(dp, scl3(f.l[e], Lp));
             P0v = {pm.x / P0, pm.y / P0, pm.z / P0};
         }
-        const double D = dot3(f.n, sub3(r, mk(t + 3 * e)));
+        const double D = 0.0;
         const double R0 = sqrt(P0 * P0 + D * D), Rp = sqrt(Pp * Pp + D * D), Rm = sqrt(Pm * Pm + D * D);
         const double LOGp = (Rp + Lp < th) ? 0.0 : log(Rp + Lp);
         const double LOGm = (Rm + Lm < th) ? 0.0 : log(Rm + Lm);

@@ -71,7 +71,14 @@ SingFrame sing_frame(const Tri& s) {
     return f;
 }
 
-SingTerms sing_terms(const V3& r, const Tri& s, const SingFrame& fr, bool want3) {
+// on_face: r is a point of the source triangle itself (the identical-face terms of
+// the Galerkin entries).  Its height D above the triangle's plane is then exactly 0;
+// the reference computes D = n.(r - v) in floating point (Code/Operator.h:585-674),
+// whose rounding noise (|D| ~ 1e-16 |r|) decides between the two branches of the
+// 1/R^3 scalar integral, one of which divides an O(1) angle difference by that noise.
+// The exact D = 0 gives the reference's value wherever its noise stays below
+// 10 eps0, and a noise-free value where it does not.
+SingTerms sing_terms(const V3& r, const Tri& s, const SingFrame& fr, bool want3, bool on_face) {
     const double th = 50 * kEps0;
     SingTerms out;
     const V3& n = s.n;
@@ -95,7 +102,7 @@ SingTerms sing_terms(const V3& r, const Tri& s, const SingFrame& fr, bool want3)
             V3 pom = sub(dp, scale(fr.l[i], Lpl[i]));
             P0v[i] = {pom.x / P0[i], pom.y / P0[i], pom.z / P0[i]};
         }
-        D[i] = dot(n, sub(r, *sv[i]));
+        D[i] = on_face ? 0.0 : dot(n, sub(r, *sv[i]));
     }
     for (int i = 0; i < 3; ++i) {
         R0[i] = std::sqrt(P0[i] * P0[i] + D[i] * D[i]);
@@ -202,7 +209,7 @@ cd op_L(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp) {
                 for (int a = 0; a < gp.n; ++a) {
                     V3 rf = gpoint(gp, a, F);
                     double wF = gp.w[a];
-                    SingTerms st = sing_terms(rf, S, fr, false);
+                    SingTerms st = sing_terms(rf, S, fr, false, true);
                     V3 RWGf = scale(sub(rf, F.vf), recAf);
                     V3 Ikon = scale(add(st.Ivec, scale(sub(st.rho, rho0N), st.Isca)), recAs);
                     alok += kJ * k * wF * 2.0 * F.area *
@@ -254,7 +261,7 @@ cd op_EFIE(const Mesh& m, const Rwg& f, const Rwg& s, cd k, cd eta, const Gauss&
                 for (int a = 0; a < gp.n; ++a) {
                     V3 rf = gpoint(gp, a, F);
                     double wF = gp.w[a];
-                    SingTerms st = sing_terms(rf, S, fr, false);
+                    SingTerms st = sing_terms(rf, S, fr, false, true);
                     V3 RWGf = scale(sub(rf, F.vf), recAf);
                     V3 Ikon = scale(add(st.Ivec, scale(sub(st.rho, rho0N), st.Isca)), recAs);
                     alok += wF * 2.0 * F.area *
@@ -311,7 +318,7 @@ cd op_K(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp) {
                 for (int a = 0; a < gp.n; ++a) {
                     V3 rf = gpoint(gp, a, F);
                     double wF = gp.w[a];
-                    SingTerms st = sing_terms(rf, S, fr, true);
+                    SingTerms st = sing_terms(rf, S, fr, true, true);
                     V3 RWGf = scale(sub(rf, F.vf), recAf);
                     V3 Ri = sub(rf, S.vf);
                     cd Ik[3];
@@ -379,7 +386,7 @@ cd op_MFIE(const Mesh& m, const Rwg& f, const Rwg& s, cd k, const Gauss& gp) {
                 for (int a = 0; a < gp.n; ++a) {
                     V3 rf = gpoint(gp, a, F);
                     double wF = gp.w[a];
-                    SingTerms st = sing_terms(rf, S, fr, true);
+                    SingTerms st = sing_terms(rf, S, fr, true, true);
                     V3 RWGf = scale(sub(rf, F.vf), recAf);
                     V3 Ri = sub(rf, S.vf);
                     cd Ik[3];
@@ -412,7 +419,7 @@ void sing_probe(const double* tri9, const double* n3, const double* r3, double* 
     S.area = 0;
     S.face = 0;
     SingFrame fr = sing_frame(S);
-    SingTerms st = sing_terms({r3[0], r3[1], r3[2]}, S, fr, true);
+    SingTerms st = sing_terms({r3[0], r3[1], r3[2]}, S, fr, true, false);
     const double o[11] = {st.Ivec.x, st.Ivec.y, st.Ivec.z, st.Isca, st.Ivec3.x, st.Ivec3.y, st.Ivec3.z, st.Isca3,
                           st.rho.x, st.rho.y, st.rho.z};
     for (int i = 0; i < 11; ++i) out[i] = o[i];
