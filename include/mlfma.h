@@ -273,6 +273,18 @@ int mlfma_excitation(mlfma_handle* h, double pol_deg, double theta_deg, double p
 int mlfma_rcs(mlfma_handle* h, const double* I, int n_angles, const double* theta_deg, const double* phi_deg,
               double pol_deg, double* out);
 
+/* Allocation-free variant of mlfma_rcs.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).  Every buffer is
+ * the caller's device memory: I_dev the solved currents (n interleaved complex), theta_dev and
+ * phi_dev the n_angles observation angles in degrees, out_dev 3 doubles per angle (total, V, H in
+ * dBsm, as mlfma_rcs).  The directions and partial sums live in a file-scope device array of
+ * 8 MiB per GPU, held until the process ends; a call needs 3*8 + 48*ceil(N/256) bytes per angle
+ * (about 300 angles per call at 150k RWG functions) and returns MLFMA_ERR_ARG beyond that.  All
+ * handles on one GPU share that array: calls that overlap on one GPU race.  The call completes
+ * before it returns (the handle's stream is synchronised). */
+int mlfma_rcs_device_noalloc(mlfma_handle* h, const void* I_dev, int n_angles, const double* theta_dev,
+                             const double* phi_dev, double* out_dev);
+
 /* n_inc plane-wave right-hand sides at once, column-major into v (column j at
  * v + 2*n*j, interleaved complex); pol/theta/phi per column, as
  * mlfma_excitation.  The batched form of RHS::computeV_* (Code/Vm.h:150-203)

@@ -328,6 +328,14 @@ int mlfma_rcs(mlfma_handle* h, const double* I, int nang, const double* th, cons
     });
 }
 
+int mlfma_rcs_device_noalloc(mlfma_handle* h, const void* I_dev, int n_angles, const double* theta_dev,
+                             const double* phi_dev, double* out_dev) {
+    if (!h || !I_dev || n_angles < 1 || !theta_dev || !phi_dev || !out_dev) return MLFMA_ERR_ARG;
+    return guarded(h, [&] {
+        h->eng->rcs_noalloc(static_cast<const double2*>(I_dev), n_angles, theta_dev, phi_dev, out_dev);
+    });
+}
+
 int mlfma_excitation_batch(mlfma_handle* h, int n_inc, const double* pol, const double* th, const double* ph,
                            double* v) {
     if (!h || n_inc < 1 || !pol || !th || !ph || !v) return MLFMA_ERR_ARG;

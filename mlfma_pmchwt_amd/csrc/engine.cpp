@@ -95,6 +95,7 @@ T* Engine::upload(const std::vector<T>& v) {
 
 void Engine::build(int np, const int* ids, const double* xyz, int nf, const int* fids, const int* fvid) {
     const double t0 = wall();
+    rcs_scratch_init(P.device);   // this device's copy of the allocation-free RCS scratch
     std::string err;
     if (!build_mesh(np, ids, xyz, nf, fids, fvid, mesh, err)) throw Error(MLFMA_ERR_MESH, err);
     build_basis(mesh, basis);
@@ -1875,6 +1876,17 @@ void Engine::rcs_multi(const double2* d_I, size_t istride, int nang, const doubl
         out[3 * a + 1] = 10.0 * std::log10(std::norm(Ev) / (4.0 * kPi));
         out[3 * a + 2] = 10.0 * std::log10(std::norm(Eh) / (4.0 * kPi));
     }
+}
+
+// The same far field and dB values as rcs(), with every buffer either the
+// caller's (currents, angles, output: device memory) or the file-scope scratch
+// array of k_rcs_noalloc.hip; no allocation of any kind.
+void Engine::rcs_noalloc(const double2* d_I, int nang, const double* d_th, const double* d_ph, double* d_out) {
+    const cplx coeff = (ie == MLFMA_PMCHWT) ? cplx(0, 1) * med.k1 : cplx(0, 1) * med.k1 * med.eta1;
+    launch_rcs_noalloc(st, P.device, N, ie == MLFMA_PMCHWT, d_geo_, d_perm_, gauss_, d2(med.k1), d2(coeff), nang,
+                       d_th, d_ph, d_I, d_out);
+    MLFMA_HIP(hipGetLastError());
+    MLFMA_HIP(hipStreamSynchronize(st));
 }
 
 void Engine::get_workspace(int wn, int cur, int which, int level, int node, double* out) {

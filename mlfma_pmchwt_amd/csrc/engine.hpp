@@ -104,6 +104,8 @@ public:
     void precond_statistics(const double2* d_b, int steps, int degree, double2* G, double2* h, double* c);
     void excitation(double pol, double th, double ph, double2* d_v);
     void rcs(const double2* d_I, int nang, const double* th, const double* ph, double pol, double* out);
+    // RCS of device currents at device angles into device output, allocating nothing
+    void rcs_noalloc(const double2* d_I, int nang, const double* d_th, const double* d_ph, double* d_out);
     void rcs_multi(const double2* d_I, size_t istride, int nang, const double* th, const double* ph, double pol,
                    double* out);
     void excitation_batch(int n_inc, const double* pol, const double* th, const double* ph, double2* d_V);

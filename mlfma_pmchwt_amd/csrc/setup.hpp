@@ -32,6 +32,14 @@ void launch_farfield(hipStream_t st, int N, int pmchwt, const RwgGeo* geo, const
                      double2 k, int nang, const double* dirs, const double2* I, size_t istride, double2* partial,
                      int nblocks);
 
+// Allocation-free RCS (k_rcs_noalloc.hip): directions and partial sums in a
+// file-scope device array, looked up once per device by rcs_scratch_init.
+void rcs_scratch_init(int device);
+size_t rcs_scratch_capacity();
+void launch_rcs_noalloc(hipStream_t st, int device, int N, int pmchwt, const RwgGeo* geo, const int* perm,
+                        const GaussRule& g, double2 k1, double2 coeff, int nang, const double* th_dev,
+                        const double* ph_dev, const double2* I_dev, double* out_dev);
+
 // Krylov vector kernels (k_solve.hip)
 // partial[j*nblk + b] = sum over block b of conj(V_j) . w, j < m
 void launch_multidot(hipStream_t st, int n, int m, const double2* V, size_t ldv, const double2* w, double2* partial,
