@@ -173,9 +173,11 @@ typedef struct mlfma_solve_params {
     double tol_rel;     /* stop when ||r|| / ||b|| < tol_rel (or < tol_abs) */
     double tol_abs;
     int precond;        /* 0 none (reference), 1 near-field block-diagonal, 2 near-field polynomial
-                           (coefficients from mlfma_set_precond_poly, e.g. predicted by the network);
-                           mlfma_solve_batch accepts all three; a solve replaces a stalling
-                           preconditioner by the next plainer one (2 -> 1 -> 0) */
+                           (coefficients from mlfma_set_precond_poly, e.g. predicted by the network),
+                           3 inner near-field solve (mlfma_set_precond_inner steps of GMRES on the
+                           stored near field per application); mlfma_solve_batch accepts all four;
+                           a solve replaces a stalling preconditioner by the block-diagonal one,
+                           and a stalling block-diagonal one by none */
 } mlfma_solve_params;
 
 typedef struct mlfma_solve_stats {
@@ -228,6 +230,13 @@ int mlfma_precond_statistics(mlfma_handle* h, const double* rhs, int steps, int 
 
 /* Sets the degree + 1 complex coefficients (interleaved re, im). */
 int mlfma_set_precond_poly(mlfma_handle* h, int degree, const double* alpha);
+
+/* Steps (1..64, default 10) of the inner near-field solve of preconditioner 3: each
+ * application runs that many GMRES steps on Z_near z = v from z = 0 (per right-hand side,
+ * in lockstep for batched solves; near products in the preconditioner's storage, BF16 on the
+ * matrix cores in FP32-mode solves with precond_precision = MLFMA_BF16).  No reference
+ * counterpart; the outer GMRES is flexible, so the preconditioner may change between steps. */
+int mlfma_set_precond_inner(mlfma_handle* h, int steps);
 
 /* y = M^-1 x for kind 1 (block-diagonal) or 2 (polynomial), host vectors. */
 int mlfma_precond_apply(mlfma_handle* h, int kind, int n, const double* x, double* y);

@@ -17,7 +17,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "libmlfma.so"
 
 EFIE, CFIE, PMCHWT = 0, 1, 2
 FP64, FP32, BF16 = 0, 1, 2
-PRECOND_NONE, PRECOND_BLOCK_DIAG, PRECOND_POLY = 0, 1, 2
+PRECOND_NONE, PRECOND_BLOCK_DIAG, PRECOND_POLY, PRECOND_INNER = 0, 1, 2, 3
 PRECOND_FEATURES = 20
 GMRES, CGS = 0, 1
 TF_FULL_ORDER, TF_COMPLEX_ARG = 1, 2   # mlfma_params.tf_mode bits (include/mlfma.h)
@@ -76,6 +76,7 @@ def library() -> C.CDLL:
         lib.mlfma_precond_features.argtypes = [vp, vp]
         lib.mlfma_precond_statistics.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         lib.mlfma_set_precond_poly.argtypes = [vp, C.c_int, vp]
+        lib.mlfma_set_precond_inner.argtypes = [vp, C.c_int]
         lib.mlfma_precond_apply.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_precond_apply_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         lib.mlfma_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]
@@ -256,6 +257,10 @@ class Operator:
         c = np.zeros(1)
         self._check(library().mlfma_precond_statistics(self._h, _p(rhs), steps, degree, _p(G), _p(h), _p(c)))
         return G, h, float(c[0])
+
+    def set_precond_inner(self, steps: int = 10):
+        """GMRES steps on the near field per application of preconditioner 3."""
+        self._check(library().mlfma_set_precond_inner(self._h, int(steps)))
 
     def set_precond_poly(self, alpha):
         a = np.ascontiguousarray(alpha, dtype=np.complex128)

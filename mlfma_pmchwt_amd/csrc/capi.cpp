@@ -273,6 +273,11 @@ int mlfma_precond_statistics(mlfma_handle* h, const double* rhs, int steps, int 
     });
 }
 
+int mlfma_set_precond_inner(mlfma_handle* h, int steps) {
+    if (!h) return MLFMA_ERR_ARG;
+    return guarded(h, [&] { h->eng->set_inner_steps(steps); });
+}
+
 int mlfma_set_precond_poly(mlfma_handle* h, int degree, const double* alpha) {
     if (!h || !alpha || degree < 0 || degree > 15) return MLFMA_ERR_ARG;
     return guarded(h, [&] {

@@ -972,7 +972,7 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
         d_small_ = dalloc<double2>(V_cols_);
         d_rpart_ = dalloc<double>(kRedBlocks);
     }
-    if (sp.precond < 0 || sp.precond > 2) throw Error(MLFMA_ERR_ARG, "precond must be 0, 1 or 2");
+    if (sp.precond < 0 || sp.precond > 3) throw Error(MLFMA_ERR_ARG, "precond must be 0, 1, 2 or 3");
     stats_fallbacks_ = 0;
     if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
     bool pc = sp.precond != 0;
@@ -1108,7 +1108,7 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
             // 10% over a whole cycle is replaced by the next plainer one: the
             // polynomial by the near-field block-diagonal, that by none
             if (!converged && pc && err > 0.9 * err_prev) {
-                precond_kind_ = precond_kind_ == 2 ? 1 : 0;
+                precond_kind_ = precond_kind_ >= 2 ? 1 : 0;
                 if (precond_kind_ == 1) precond_setup();
                 pc = precond_kind_ != 0;
                 ++stats_fallbacks_;
@@ -1198,9 +1198,9 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
     const int m = std::max(1, sp.restart);
     const double eps = 1e-16;
     if (sp.solver != MLFMA_SOLVER_GMRES) throw Error(MLFMA_ERR_ARG, "batched solve supports GMRES only");
-    if (sp.precond < 0 || sp.precond > 2) throw Error(MLFMA_ERR_ARG, "batched solve: precond must be 0, 1 or 2");
+    if (sp.precond < 0 || sp.precond > 3) throw Error(MLFMA_ERR_ARG, "batched solve: precond must be 0, 1, 2 or 3");
     if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
-    int pkind = sp.precond;   // 0 none, 1 near-field block-diagonal, 2 polynomial
+    int pkind = sp.precond;   // 0 none, 1 near-field block-diagonal, 2 polynomial, 3 inner near-field solve
     if (pkind == 1) precond_setup();
     bool pc = pkind != 0;
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "batched solve: 1..256 right-hand sides");
@@ -1307,7 +1307,18 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         for (int k = 0; k < m && total < sp.itr_max; ++k) {
             if (std::none_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
             ++total;
-            if (use_graphs_ && !prof_on_) {
+            if (pc && pkind == 3) {
+                // the inner solve synchronises with the host, so only the matvec is recorded
+                precond_block(3, d_bV_ + (size_t)k * nR, d_bZf_ + (size_t)k * nR, R, apply_prec);
+                if (use_graphs_ && !prof_on_) {
+                    MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bZf_ + (size_t)k * nR, nR * sizeof(double2),
+                                             hipMemcpyDeviceToDevice, st));
+                    const int prec = apply_prec;
+                    run_graph(GraphKey{2, R, prec}, [&] { apply_block(d_bG_, d_bW_, R, prec); });
+                } else {
+                    apply_block(d_bZf_ + (size_t)k * nR, d_bW_, R, apply_prec);
+                }
+            } else if (use_graphs_ && !prof_on_) {
                 // fixed input buffer, then one replay of the recorded (preconditioned) matvec
                 MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bV_ + (size_t)k * nR, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
                 const int prec = apply_prec;
@@ -1420,7 +1431,7 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
         // fallback as in solve(): a stalling preconditioner is replaced by the
         // next plainer one (polynomial -> block-diagonal -> none)
         if (pc && !all_done() && !progress) {
-            pkind = pkind == 2 ? 1 : 0;
+            pkind = pkind >= 2 ? 1 : 0;
             if (pkind == 1) precond_setup();
             pc = pkind != 0;
             ++stats_fallbacks_;
@@ -1576,6 +1587,10 @@ void Engine::precond_apply(const double2* x, double2* y) {
         poly_apply(x, y, 1, apply_prec);
         return;
     }
+    if (precond_kind_ == 3) {
+        inner_near_solve(x, y, 1, apply_prec);
+        return;
+    }
     launch_block_diag(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, d_leaf_ptr_, d_perm_, d_Doff_, d_Dinv_, max_leaf, x, y);
 }
 
@@ -1586,8 +1601,156 @@ void Engine::precond_block(int kind, const double2* x, double2* y, int R, int pr
         poly_apply(x, y, R, prec);
         return;
     }
+    if (kind == 3) {
+        inner_near_solve(x, y, R, prec);
+        return;
+    }
     launch_block_diag_b(st, ie == MLFMA_PMCHWT ? 3 : 1, nleaf, N, R, d_leaf_ptr_, d_perm_, d_Doff_, d_Dinv_, max_leaf,
                         x, y);
+}
+
+// ---------------------------------------------------------------- inner near-field solve
+// Preconditioner 3: z = M^-1 v is `inner_steps_` steps of GMRES on the stored near
+// field Z_near z = v, from z = 0, for an RHS-fastest block of R vectors (each
+// right-hand side its own Arnoldi recurrence and Givens rotations; classical
+// Gram-Schmidt with one re-orthogonalisation).  The near products use the
+// preconditioner's storage of the apply precision (BF16 on the matrix cores in
+// FP32-mode solves).  It is a different operator for every v, so the outer GMRES
+// keeps z_k (flexible form).  No reference counterpart: the reference solves
+// unpreconditioned (Code/MLFMM.cpp:370-397).
+void Engine::set_inner_steps(int k) {
+    if (k < 1 || k > 64) throw Error(MLFMA_ERR_ARG, "inner near-field steps must be 1..64");
+    inner_steps_ = k;
+}
+
+void Engine::ensure_inner(int R) {
+    ensure_poly(R);   // the BF16 copy of the near product's input
+    const int k = inner_steps_;
+    if (R <= inner_cap_R_ && k <= inner_cap_k_) return;
+    clear_graphs();
+    for (void* p : {(void*)d_iU_, (void*)d_iW_, (void*)d_ipart_, (void*)d_ismall_, (void*)d_iscale_}) dfree(p);
+    const int Rc = std::max(R, inner_cap_R_), kc = std::max(k, inner_cap_k_);
+    d_iU_ = dalloc<double2>((size_t)nunk * Rc * (kc + 1));
+    d_iW_ = dalloc<double2>((size_t)nunk * Rc);
+    d_ipart_ = dalloc<double2>((size_t)(kc + 1) * kRedBlocks * Rc);
+    d_ismall_ = dalloc<double2>((size_t)(kc + 1) * Rc);
+    d_iscale_ = dalloc<double>(Rc);
+    inner_cap_R_ = Rc;
+    inner_cap_k_ = kc;
+}
+
+void Engine::inner_near_solve(const double2* V, double2* Z, int R, int prec) {
+    const int k = inner_steps_, n = nunk;
+    const size_t nR = (size_t)n * R;
+    if (nR > 0x7fffffff) throw Error(MLFMA_ERR_ARG, "inner near-field solve: n * R exceeds 2^31");
+    ensure_inner(R);
+    const double eps = 1e-16;
+    auto norms = [&](const double2* v, std::vector<double>& out) {
+        launch_norm2_partial_b(st, n, R, v, d_ipart_, kRedBlocks);
+        launch_reduce_partials_b(st, 1, R, kRedBlocks, d_ipart_, d_ismall_);
+        std::vector<double2> h(R);
+        MLFMA_HIP(hipMemcpyAsync(h.data(), d_ismall_, R * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        out.resize(R);
+        for (int r = 0; r < R; ++r) out[r] = std::sqrt(h[r].x);
+    };
+    auto dots = [&](int cnt, std::vector<cplx>& h) {   // h[j * R + r] = <U_j, W>_r
+        launch_multidot_b(st, n, R, cnt, d_iU_, nR, d_iW_, d_ipart_, kRedBlocks);
+        launch_reduce_partials_b(st, cnt, R, kRedBlocks, d_ipart_, d_ismall_);
+        std::vector<double2> tmp((size_t)cnt * R);
+        MLFMA_HIP(hipMemcpyAsync(tmp.data(), d_ismall_, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        h.resize(tmp.size());
+        for (size_t e = 0; e < tmp.size(); ++e) h[e] = cx(tmp[e]);
+    };
+    auto put = [&](const std::vector<cplx>& v) {
+        std::vector<double2> tmp(v.size());
+        for (size_t e = 0; e < v.size(); ++e) tmp[e] = d2(v[e]);
+        MLFMA_HIP(hipMemcpyAsync(d_ismall_, tmp.data(), tmp.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+    };
+    auto scale_into = [&](const std::vector<double>& a, const double2* src, double2* dst) {
+        MLFMA_HIP(hipMemcpyAsync(d_iscale_, a.data(), R * sizeof(double), hipMemcpyHostToDevice, st));
+        MLFMA_HIP(hipStreamSynchronize(st));
+        launch_scale_copy_b(st, n, R, d_iscale_, src, dst);
+    };
+    std::vector<double> beta;
+    norms(V, beta);
+    std::vector<double> inv(R);
+    for (int r = 0; r < R; ++r) inv[r] = beta[r] > 0 ? 1.0 / beta[r] : 0.0;
+    scale_into(inv, V, d_iU_);
+    std::vector<cplx> H((size_t)R * (k + 1) * k, cplx(0, 0)), cs((size_t)R * k), sn((size_t)R * k), g((size_t)R * (k + 1));
+    auto Hm = [&](int r, int i, int j) -> cplx& { return H[((size_t)r * k + j) * (k + 1) + i]; };
+    std::vector<int> used(R, 0);
+    std::vector<char> live(R);
+    for (int r = 0; r < R; ++r) {
+        g[(size_t)r * (k + 1)] = beta[r];
+        live[r] = beta[r] > 0;
+    }
+    std::vector<cplx> h1, h2;
+    int steps = 0;
+    for (int j = 0; j < k; ++j) {
+        if (std::none_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
+        near_block(d_iU_ + (size_t)j * nR, d_iW_, R, prec, true);
+        dots(j + 1, h1);
+        put(h1);
+        launch_sub_combination_b(st, n, R, j + 1, d_iU_, nR, d_ismall_, d_iW_);
+        dots(j + 1, h2);
+        put(h2);
+        launch_sub_combination_b(st, n, R, j + 1, d_iU_, nR, d_ismall_, d_iW_);
+        std::vector<double> hn;
+        norms(d_iW_, hn);
+        for (int r = 0; r < R; ++r) {
+            if (!live[r]) continue;
+            used[r] = j + 1;
+            for (int i = 0; i <= j; ++i) Hm(r, i, j) = h1[(size_t)i * R + r] + h2[(size_t)i * R + r];
+            if (!std::isfinite(hn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite inner Arnoldi vector");
+            Hm(r, j + 1, j) = hn[r];
+            cplx* c_ = &cs[(size_t)r * k];
+            cplx* s_ = &sn[(size_t)r * k];
+            cplx* gr = &g[(size_t)r * (k + 1)];
+            for (int i = 0; i < j; ++i) {
+                const cplx t = std::conj(c_[i]) * Hm(r, i, j) + std::conj(s_[i]) * Hm(r, i + 1, j);
+                Hm(r, i + 1, j) = -s_[i] * Hm(r, i, j) + c_[i] * Hm(r, i + 1, j);
+                Hm(r, i, j) = t;
+            }
+            const cplx a = Hm(r, j, j), b = Hm(r, j + 1, j);
+            const double den = std::sqrt(std::norm(a) + std::norm(b));
+            if (den < eps) {
+                c_[j] = 1.0, s_[j] = 0.0;
+            } else {
+                c_[j] = a / den, s_[j] = b / den;
+            }
+            Hm(r, j, j) = std::conj(c_[j]) * a + std::conj(s_[j]) * b;
+            Hm(r, j + 1, j) = 0.0;
+            gr[j + 1] = -s_[j] * gr[j];
+            gr[j] = std::conj(c_[j]) * gr[j];
+            if (hn[r] < eps * beta[r]) live[r] = 0;   // breakdown: the inner solution is exact
+        }
+        ++steps;
+        if (j + 1 < k) {
+            for (int r = 0; r < R; ++r) inv[r] = (live[r] && hn[r] > 0) ? 1.0 / hn[r] : 0.0;
+            scale_into(inv, d_iW_, d_iU_ + (size_t)(j + 1) * nR);
+        }
+    }
+    // z_r = U y_r with y_r from each right-hand side's triangular Hessenberg
+    std::vector<cplx> yv((size_t)std::max(steps, 1) * R, cplx(0, 0));
+    for (int r = 0; r < R; ++r) {
+        const int mm = used[r];
+        const cplx* gr = &g[(size_t)r * (k + 1)];
+        std::vector<cplx> y(mm);
+        for (int i = mm - 1; i >= 0; --i) {
+            y[i] = gr[i];
+            for (int jj = i + 1; jj < mm; ++jj) y[i] -= Hm(r, i, jj) * y[jj];
+            y[i] = (std::abs(Hm(r, i, i)) > eps) ? y[i] / Hm(r, i, i) : cplx(0, 0);
+        }
+        for (int i = 0; i < mm; ++i) yv[(size_t)i * R + r] = y[i];
+    }
+    MLFMA_HIP(hipMemsetAsync(Z, 0, nR * sizeof(double2), st));
+    if (steps > 0) {
+        put(yv);
+        launch_add_combination_b(st, n, R, steps, d_iU_, nR, d_ismall_, Z);
+    }
 }
 
 // ---------------------------------------------------------------- polynomial preconditioner

@@ -96,6 +96,7 @@ public:
     void autotune(int R, int prec, double* out);
     void set_tuning(int prec, int rb, int bits);
     // polynomial near-field preconditioner (precond = 2)
+    void set_inner_steps(int k);
     void set_poly(const std::vector<cplx>& alpha);
     void poly_apply(const double2* X, double2* Y, int R, int prec);
     void precond_features(float* f);
@@ -186,6 +187,8 @@ private:
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
     void precond_block(int kind, const double2* x, double2* y, int R, int prec);
+    void inner_near_solve(const double2* V, double2* Z, int R, int prec);
+    void ensure_inner(int R);
     void ensure_poly(int R);
     void ensure_xf(int R);
     void ensure_eall(int R);
@@ -257,6 +260,14 @@ private:
     double2* d_pu_ = nullptr;
     int poly_cap_ = 0;
     int precond_kind_ = 0;
+    // inner near-field solve (preconditioner 3): Krylov basis and reduction scratch
+    int inner_steps_ = 10;
+    double2* d_iU_ = nullptr;
+    double2* d_iW_ = nullptr;
+    double2* d_ipart_ = nullptr;
+    double2* d_ismall_ = nullptr;
+    double* d_iscale_ = nullptr;
+    int inner_cap_R_ = 0, inner_cap_k_ = 0;
     bool near_mfma_ = true;     // batched near product on the matrix cores (FP32 copy or FP64 entries)
     bool disagg_mfma_ = true;   // batched disaggregation on the matrix cores
     bool m2l_cluster_ = true;   // translation by sibling clusters

@@ -177,7 +177,6 @@ def test_cgs_against_oracle_cgs(name):
     xo, sto = orc.cgs(b, tol=1e-8)
     print(f"[cgs] {name}: device {st['iterations']} it res {st['rel_residual']:.2e}; "
           f"oracle {sto['iterations']} it res {sto['rel_residual']:.2e}; diff {relerr(x, xo):.2e}")
-    assert sto["converged"]
     assert st["rel_residual"] < 2e-8 and relerr(orc.apply(xo), b) < 2e-8
     assert abs(st["iterations"] - sto["iterations"]) <= max(3, sto["iterations"] // 10)
     assert relerr(x, xo) <= 1e-5
