@@ -211,14 +211,15 @@ int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int 
  * near field): one near-field product per degree.  Its coefficients are
  * either fitted from the solver's own Krylov pairs (mlfma_precond_statistics)
  * or predicted for an unseen problem by the coefficient network
- * (mlfma_mlp_*) from mlfma_precond_features.  A solve drops a polynomial
+ * (mlfma_mlp_*) from mlfma_precond_features.  A solve replaces a polynomial
  * preconditioner that stalls (true residual not 10% lower after a restart
- * cycle) and continues unpreconditioned. */
+ * cycle) by the block-diagonal one, and that, if it stalls too, by none. */
 #define MLFMA_PRECOND_FEATURES 20
 
 /* 20 problem features: spectral moments mu_k = z^H (D^-1 Z_near)^k z / n,
  * k = 1..8 (8 real parts, 8 imaginary parts) for a seeded unit-modulus probe
- * z, then eps_r (re, im), PMCHWT flag, log10(n_unknowns). */
+ * z, then eps_r (re, im), PMCHWT flag, log10(near-field entries per row of the full near matrix):
+ * a discretisation-density feature, so smaller objects meshed as densely cover a large one. */
 int mlfma_precond_features(mlfma_handle* h, float* features);
 
 /* Least-squares statistics of the coefficients from `steps` Arnoldi steps of

@@ -1834,7 +1834,7 @@ void Engine::poly_apply(const double2* X, double2* Y, int R, int prec) {
 // Problem features for the coefficient network (kPrecondFeatures values):
 // spectral moments mu_k = z^H B^k z / n of B = D^-1 Z_near for k = 1..8 with a
 // seeded unit-modulus probe z (re, im), then eps_r (re, im), PMCHWT flag and
-// log10 of the unknown count.  FP64 near field.
+// log10 of the near-field entries per row.  FP64 near field.
 void Engine::precond_features(float* f) {
     ensure_poly(1);
     double2* z = d_pu_;
@@ -1858,7 +1858,11 @@ void Engine::precond_features(float* f) {
     f[16] = (float)P.eps_r_re;
     f[17] = (float)P.eps_r_im;
     f[18] = ie == MLFMA_PMCHWT ? 1.0f : 0.0f;
-    f[19] = (float)std::log10((double)nunk);
+    // discretisation density, not object size: log10 of the near-field entries per row of the
+    // full near matrix (PMCHWT stores 3 of its 4 blocks), so a family of smaller objects meshed
+    // as densely covers a large member's value
+    const double full = (ie == MLFMA_PMCHWT) ? (double)near_entries / 3.0 * 4.0 : (double)near_entries;
+    f[19] = (float)std::log10(full / (double)nunk);
 }
 
 // Least-squares statistics of the polynomial coefficients from the solver's

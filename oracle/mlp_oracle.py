@@ -95,7 +95,8 @@ def features(A_near: np.ndarray, eps_r: complex, pmchwt: bool) -> np.ndarray:
         v = (A_near @ v) / d
         mu.append(np.vdot(z, v) / n)
     mu = np.array(mu)
-    return np.concatenate([mu.real, mu.imag, [eps_r.real, eps_r.imag, 1.0 if pmchwt else 0.0, np.log10(n)]])
+    density = np.count_nonzero(A_near) / n   # near-field entries per row
+    return np.concatenate([mu.real, mu.imag, [eps_r.real, eps_r.imag, 1.0 if pmchwt else 0.0, np.log10(density)]])
 
 
 def statistics(apply, A_near: np.ndarray, rhs: np.ndarray, steps: int, degree: int):
