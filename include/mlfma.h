@@ -146,7 +146,8 @@ int mlfma_set_graphs(mlfma_handle* h, int on);
  * vector units and the translation by sibling clusters or per box; keeps the
  * choice for this handle.  out4 = {rhs per pass, choice bits (bit 0 near on
  * the matrix cores, bit 1 disaggregation on the matrix cores, bit 2 cluster
- * translation), ms per batched matvec of the choice, ms of the slowest
+ * translation, bit 3 cluster translation with 12 workspace columns per thread
+ * (FP32)), ms per batched matvec of the choice, ms of the slowest
  * configuration timed}.  No reference counterpart. */
 int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
 
@@ -156,7 +157,8 @@ int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
  * choice bits of mlfma_autotune's out4[1]: bit 0 near field on the matrix
  * cores, bit 1 disaggregation on the matrix cores, bit 2 translation by
  * sibling clusters (up to 4 children of one parent share each source box,
- * read once).  No reference counterpart. */
+ * read once), bit 3 12 instead of 6 workspace columns per thread in the FP32
+ * cluster translation.  No reference counterpart. */
 int mlfma_set_tuning(mlfma_handle* h, int precision, int rhs_per_pass, int bits);
 
 /* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)

@@ -45,6 +45,22 @@ __device__ __forceinline__ void cfms(float2& acc, float2 a, float2 b) {
     acc.y -= a.x * b.y + a.y * b.x;
 }
 
+// acc += a * b as four chained fused multiply-adds (one rounding per partial
+// product instead of the mul / fma / add triple the expression form compiles
+// to): the translation kernels' inner loop.
+__device__ __forceinline__ void cmac(float2& acc, float2 a, float2 b) {
+    acc.x = __builtin_fmaf(a.x, b.x, acc.x);
+    acc.y = __builtin_fmaf(a.x, b.y, acc.y);
+    acc.x = __builtin_fmaf(-a.y, b.y, acc.x);
+    acc.y = __builtin_fmaf(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
+    acc.x = __builtin_fma(a.x, b.x, acc.x);
+    acc.y = __builtin_fma(a.x, b.y, acc.y);
+    acc.x = __builtin_fma(-a.y, b.y, acc.x);
+    acc.y = __builtin_fma(a.y, b.x, acc.y);
+}
+
 // Conversions between the FP64 solver vectors and the far-field precision.
 __device__ __forceinline__ void narrow(double2 a, double2& b) { b = a; }
 __device__ __forceinline__ void narrow(double2 a, float2& b) { b = make_float2((float)a.x, (float)a.y); }

@@ -207,7 +207,11 @@ void Engine::build_device_tree() {
             srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
             const size_t base = cl_src.size();
             cl_src.insert(cl_src.end(), srcs.begin(), srcs.end());
+            // a member without a source reads the zero row after the last
+            // translation vector; absent members keep -1
             cl_vec.resize(cl_src.size() * 4, -1);
+            for (size_t u = base; u < cl_src.size(); ++u)
+                for (int b = b0; b < b1; ++b) cl_vec[u * 4 + (b - b0)] = (int)v.drvec.size();
             for (int b = b0; b < b1; ++b)
                 for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) {
                     const size_t u = base + (std::lower_bound(srcs.begin(), srcs.end(), v.dr_idx[j]) - srcs.begin());
@@ -323,9 +327,11 @@ void Engine::build_far(int w, const WaveTables& T) {
         F.Bm[l] = dalloc<cplx_t<R>>(ws);
         // quadrature weight w_theta * dphi folded into the translation operator
         // (the reference applies it inside m2l, Code/ProcessMLFMM.h:138-142)
-        std::vector<cplx_t<R>> tfw(T.tf[l].size());
+        // plus one all-zero row after the last vector (read by the cluster
+        // kernel for members without the source)
+        std::vector<cplx_t<R>> tfw(T.tf[l].size() + D.K[l], cplx_t<R>{0, 0});
         const int K = D.K[l];
-        for (size_t e = 0; e < tfw.size(); ++e) tfw[e] = to_cx<R>(T.tf[l][e] * T.qw[l][e % K]);
+        for (size_t e = 0; e < T.tf[l].size(); ++e) tfw[e] = to_cx<R>(T.tf[l][e] * T.qw[l][e % K]);
         F.TFw[l] = upload(tfw);
     }
     for (int c = 3; c <= L; ++c) {
@@ -506,7 +512,7 @@ void Engine::translate(int w, int l, int nc) {
     const bool up = l > 2;
     const LevelDev& v = lv_[l];
     if (m2l_cluster_)
-        launch_m2l_cl<R>(st, nc, v.ncl, D.K[l], v.cl_child, v.cl_ptr, v.cl_src, v.cl_vec, F.TFw[l], F.Sm[l],
+        launch_m2l_cl<R>(st, m2l_wide_, nc, v.ncl, D.K[l], v.cl_child, v.cl_ptr, v.cl_src, v.cl_vec, F.TFw[l], F.Sm[l],
                          up ? D.K[l - 1] : 0, v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l],
                          up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
     else
@@ -817,7 +823,7 @@ void Engine::autotune(int R, int prec, double* out) {
     launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
     const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
     const bool nm = R >= 2, dm = R >= 2 && (f32 ? wd_[0].E != nullptr : wd_[0].E64 != nullptr);
-    const int valid = (nm ? 1 : 0) | (dm ? 2 : 0) | 4;
+    const int valid = (nm ? 1 : 0) | (dm ? 2 : 0) | 4 | (f32 ? 8 : 0);
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
     MLFMA_HIP(hipEventCreate(&b));
@@ -826,6 +832,7 @@ void Engine::autotune(int R, int prec, double* out) {
         near_mfma_ = (m & 1) != 0;
         disagg_mfma_ = (m & 2) != 0;
         m2l_cluster_ = (m & 4) != 0;
+        m2l_wide_ = (m & 8) != 0;
         rb_tuned_[f32 ? 1 : 0] = rb;
         apply_block(d_bx_, d_by_, R, prec);   // warm-up
         MLFMA_HIP(hipEventRecord(a, st));
@@ -841,7 +848,7 @@ void Engine::autotune(int R, int prec, double* out) {
     };
     int best_rb = rbmax, best_m = valid;
     double best = time(best_rb, best_m);
-    for (int bit = 1; bit <= 4; bit <<= 1) {
+    for (int bit = 1; bit <= 8; bit <<= 1) {
         if (!(valid & bit)) continue;
         const double t = time(best_rb, best_m ^ bit);
         if (t < best) best = t, best_m ^= bit;
@@ -857,6 +864,7 @@ void Engine::autotune(int R, int prec, double* out) {
     near_mfma_ = (best_m & 1) != 0;
     disagg_mfma_ = (best_m & 2) != 0;
     m2l_cluster_ = (best_m & 4) != 0;
+    m2l_wide_ = (best_m & 8) != 0;
     clear_graphs();   // recorded sequences follow the tuned configuration
     if (out) out[0] = best_rb, out[1] = best_m, out[2] = best, out[3] = worst;
 }
@@ -868,6 +876,7 @@ void Engine::set_tuning(int prec, int rb, int bits) {
     near_mfma_ = (bits & 1) != 0;
     disagg_mfma_ = (bits & 2) != 0;
     m2l_cluster_ = (bits & 4) != 0;
+    m2l_wide_ = (bits & 8) != 0;
     clear_graphs();
 }
 

@@ -271,6 +271,7 @@ private:
     bool near_mfma_ = true;     // batched near product on the matrix cores (FP32 copy or FP64 entries)
     bool disagg_mfma_ = true;   // batched disaggregation on the matrix cores
     bool m2l_cluster_ = true;   // translation by sibling clusters
+    bool m2l_wide_ = false;     // FP32 cluster translation with 12 workspace columns per thread
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };

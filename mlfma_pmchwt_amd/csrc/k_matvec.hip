@@ -270,14 +270,99 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
 }
 
 // Sibling-cluster form of the same product: a thread owns one sample q, CPT of
-// the nc x 3 workspace columns and all (up to 4) children of one parent.  It
+// the nc x 3 workspace columns and all NM (1..4) children of one parent.  It
 // walks the union of their distant-relative lists in ascending box order,
 // reading each source box's CPT values once and applying them to every member
-// that has the box in its list, with that member's translation vector (a
-// member's sum thus runs over its own list in ascending box order).  The
-// interpolation from the parent reads each parent value once for all members.
-// G = nc x 3 / CPT wavefronts per workgroup, one per column group; no LDS and
-// no barriers.
+// with that member's translation vector; a member without the source in its
+// list reads the all-zero row the table keeps after its last vector, so the
+// member loop has no branches (a member's sum thus runs over its own list in
+// ascending box order, plus exact zeros).  The interpolation from the parent
+// reads each parent value once for all members.  G = nc x 3 / CPT wavefronts
+// per workgroup, one per column group; no LDS and no barriers.
+template <typename R, int NC, int G, int NM>
+__device__ __forceinline__ void m2l_cl_body(int cl, int K, int q, int c0, const int* __restrict__ cl_child,
+                                            const int* __restrict__ cl_ptr, const int* __restrict__ cl_src,
+                                            const int* __restrict__ cl_vec, const cplx_t<R>* __restrict__ TFw,
+                                            const cplx_t<R>* __restrict__ Sm, int Kp, const int* __restrict__ parent,
+                                            const int* __restrict__ oct, const int* __restrict__ t_ptr,
+                                            const int* __restrict__ t_kp, const R* __restrict__ t_w,
+                                            const cplx_t<R>* __restrict__ shdn, const cplx_t<R>* __restrict__ Bmp,
+                                            cplx_t<R>* __restrict__ Bm) {
+    using C = cplx_t<R>;
+    constexpr int CPT = NC * 3 / G;
+    int child[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) child[k] = cl_child[cl * 4 + k];
+    C acc[NM][CPT];
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) acc[k][j] = C{0, 0};
+    const int u0 = cl_ptr[cl], u1 = cl_ptr[cl + 1];
+    // software pipeline over two register sets: the next source's values and
+    // its members' translation samples are loaded into one set while the
+    // other is applied
+    C sa[CPT], sb[CPT], ta[NM], tb[NM];
+    auto fetch = [&](C (&s)[CPT], C (&t)[NM], int u) {
+        const int4 v = *reinterpret_cast<const int4*>(cl_vec + (size_t)u * 4);
+        const int vk[4] = {v.x, v.y, v.z, v.w};
+        const C* S = Sm + ((size_t)cl_src[u] * NC * 3 + c0) * K + q;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) s[j] = S[(size_t)j * K];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) t[k] = TFw[(size_t)vk[k] * K + q];
+    };
+    auto apply = [&](const C (&s)[CPT], const C (&t)[NM]) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k)
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) cmac(acc[k][j], t[k], s[j]);
+    };
+    // Sources go in pairs, so that every apply has the other set's loads in
+    // flight; the prefetch after a pair is unconditional (the last source is
+    // read again past the end), so every path reaches an apply with the same
+    // loads outstanding, and the scheduling barriers keep the compiler from
+    // sinking the loads to their uses.
+    int u = u0;
+    if (u < u1) fetch(sa, ta, u);
+    for (; u + 1 < u1; u += 2) {
+        fetch(sb, tb, u + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        apply(sa, ta);
+        fetch(sa, ta, min(u + 2, u1 - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        apply(sb, tb);
+    }
+    if (u < u1) apply(sa, ta);
+    if (Bmp != nullptr) {
+        const C* P = Bmp + ((size_t)parent[child[0]] * NC * 3 + c0) * Kp;   // siblings share the parent
+        const C* sh[NM];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) sh[k] = shdn + (size_t)oct[child[k]] * Kp;
+        for (int j = t_ptr[q]; j < t_ptr[q + 1]; ++j) {
+            const int kp = t_kp[j];
+            const R w = t_w[j];
+            C pv[CPT];
+#pragma unroll
+            for (int jj = 0; jj < CPT; ++jj) pv[jj] = P[(size_t)jj * Kp + kp];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                C e = sh[k][kp];
+                e.x *= w;
+                e.y *= w;
+#pragma unroll
+                for (int jj = 0; jj < CPT; ++jj) cfma(acc[k][jj], e, pv[jj]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        C* o = Bm + ((size_t)child[k] * NC * 3 + c0) * K + q;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) o[(size_t)j * K] = acc[k][j];
+    }
+}
+
 template <typename R, int NC, int G>
 __global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, const int* __restrict__ cl_child,
                                                    const int* __restrict__ cl_ptr, const int* __restrict__ cl_src,
@@ -287,80 +372,27 @@ __global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, c
                                                    const int* __restrict__ t_ptr, const int* __restrict__ t_kp,
                                                    const R* __restrict__ t_w, const cplx_t<R>* __restrict__ shdn,
                                                    const cplx_t<R>* __restrict__ Bmp, cplx_t<R>* __restrict__ Bm) {
-    using C = cplx_t<R>;
     constexpr int CPT = NC * 3 / G;
     const int lin = xcd_remap(blockIdx.x, ncl * ktiles);
     const int cl = lin / ktiles, kt = lin % ktiles;
     const int c0 = (threadIdx.x >> 6) * CPT;
     const int q = kt * 64 + (threadIdx.x & 63);
     if (q >= K) return;
-    int child[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) child[k] = cl_child[cl * 4 + k];
-    C acc[4][CPT];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int j = 0; j < CPT; ++j) acc[k][j] = C{0, 0};
-    const int u0 = cl_ptr[cl], u1 = cl_ptr[cl + 1];
-    // software pipeline: the next source's values and vector ids are loaded
-    // while the current source is applied
-    C sn[CPT];
-    int4 vn = make_int4(-1, -1, -1, -1);
-    auto fetch = [&](int u) {
-        const C* S = Sm + ((size_t)cl_src[u] * NC * 3 + c0) * K + q;
-#pragma unroll
-        for (int j = 0; j < CPT; ++j) sn[j] = S[(size_t)j * K];
-        vn = *reinterpret_cast<const int4*>(cl_vec + (size_t)u * 4);
-    };
-    if (u0 < u1) fetch(u0);
-    for (int u = u0; u < u1; ++u) {
-        C sv[CPT];
-#pragma unroll
-        for (int j = 0; j < CPT; ++j) sv[j] = sn[j];
-        const int vk[4] = {vn.x, vn.y, vn.z, vn.w};
-        if (u + 1 < u1) fetch(u + 1);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (vk[k] >= 0) {
-                const C T = TFw[(size_t)vk[k] * K + q];
-#pragma unroll
-                for (int j = 0; j < CPT; ++j) cfma(acc[k][j], T, sv[j]);
-            }
-    }
-    if (Bmp != nullptr) {
-        const C* P = Bmp + ((size_t)parent[child[0]] * NC * 3 + c0) * Kp;   // siblings share the parent
-        const C* sh[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sh[k] = shdn + (size_t)(child[k] >= 0 ? oct[child[k]] : 0) * Kp;
-        for (int j = t_ptr[q]; j < t_ptr[q + 1]; ++j) {
-            const int kp = t_kp[j];
-            const R w = t_w[j];
-            C pv[CPT];
-#pragma unroll
-            for (int jj = 0; jj < CPT; ++jj) pv[jj] = P[(size_t)jj * Kp + kp];
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (child[k] >= 0) {
-                    C e = sh[k][kp];
-                    e.x *= w;
-                    e.y *= w;
-#pragma unroll
-                    for (int jj = 0; jj < CPT; ++jj) cfma(acc[k][jj], e, pv[jj]);
-                }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (child[k] >= 0) {
-            C* o = Bm + ((size_t)child[k] * NC * 3 + c0) * K + q;
-#pragma unroll
-            for (int j = 0; j < CPT; ++j) o[(size_t)j * K] = acc[k][j];
-        }
+    // members are packed first; the count is uniform over the workgroup
+    const int4 ch = *reinterpret_cast<const int4*>(cl_child + (size_t)cl * 4);
+    const int nm = (ch.y >= 0) + (ch.z >= 0) + (ch.w >= 0) + 1;
+#define M2LB(NM)                                                                                                      \
+    m2l_cl_body<R, NC, G, NM>(cl, K, q, c0, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp, parent, oct, t_ptr, t_kp, \
+                              t_w, shdn, Bmp, Bm)
+    if (nm == 4) M2LB(4);
+    else if (nm == 3) M2LB(3);
+    else if (nm == 2) M2LB(2);
+    else M2LB(1);
+#undef M2LB
 }
 
 template <typename R>
-void launch_m2l_cl(hipStream_t st, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+void launch_m2l_cl(hipStream_t st, bool wide, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
                    const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
                    const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
                    const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
@@ -368,11 +400,21 @@ void launch_m2l_cl(hipStream_t st, int ncur, int ncl, int K, const int* cl_child
     const int kt = (K + 63) / 64;
     // columns per thread: 3 in FP64, 6 in FP32 (3 when there are only 3)
     constexpr bool f64 = sizeof(R) == 8;
+#define M2LC_G(NC, G)                                                                                                 \
+    k_m2l_cl<R, NC, G><<<ncl * kt, 64 * G, 0, st>>>(ncl, K, kt, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp,     \
+                                                    parent, octant, t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm)
 #define M2LC(NC)                                                                                                      \
     {                                                                                                                 \
-        constexpr int G = f64 ? NC : (NC == 1 ? 1 : NC / 2);                                                          \
-        k_m2l_cl<R, NC, G><<<ncl * kt, 64 * G, 0, st>>>(ncl, K, kt, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp, \
-                                                        parent, octant, t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm); \
+        if constexpr (!f64 && NC >= 4) {                                                                              \
+            if (wide) {                                                                                               \
+                M2LC_G(NC, NC / 4);                                                                                   \
+            } else {                                                                                                  \
+                M2LC_G(NC, NC / 2);                                                                                   \
+            }                                                                                                         \
+        } else {                                                                                                      \
+            constexpr int G = f64 ? NC : (NC == 1 ? 1 : NC / 2);                                                      \
+            M2LC_G(NC, G);                                                                                            \
+        }                                                                                                             \
     }
     if (ncur == 1) M2LC(1)
     else if (ncur == 2) M2LC(2)
@@ -381,6 +423,7 @@ void launch_m2l_cl(hipStream_t st, int ncur, int ncl, int K, const int* cl_child
     else if constexpr (sizeof(R) == 4) M2LC(16)
     else throw std::invalid_argument("FP64 far field: at most 8 workspace currents per pass");
 #undef M2LC
+#undef M2LC_G
 }
 
 // ---------------------------------------------------------------- disaggregation
@@ -600,7 +643,7 @@ void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
                                          const cplx_t<R>*, const cplx_t<R>*, double2, double2*);                  \
     template void launch_disagg_pmchwt_b<R>(hipStream_t, int, int, int, const int*, const int*,                    \
                                             const PmchwtFarArgs<R>&, double2*);                                   \
-    template void launch_m2l_cl<R>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,      \
+    template void launch_m2l_cl<R>(hipStream_t, bool, int, int, int, const int*, const int*, const int*, const int*, \
                                    const cplx_t<R>*, const cplx_t<R>*, int, const int*, const int*, const int*,   \
                                    const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);
 MLFMA_FAR_INSTANTIATE(double)
