@@ -93,6 +93,115 @@ def translation_flops(op, R, ncur):
     return f
 
 
+def far_flops(op, R, group):
+    """Algorithmic FLOPs (8 per complex multiply-add) of one R-right-hand-side matvec's
+    aggregation or disaggregation: every RWG's 3K pattern samples against every current
+    (aggregation: J and M into the leaf workspaces; disaggregation: the two outputs), per
+    wavenumber."""
+    nw = 2 if op.params.integral_equation == 2 else 1
+    cur = 2 if op.params.integral_equation == 2 else 1
+    f = 0.0
+    for wn in range(1, nw + 1):
+        L = op.grid(wn, op.max_level)[0]
+        f += 8.0 * op.n_rwg * 3 * (2 * L * L) * R * cur
+    return f
+
+
+def group_roofline(op, R, dom, groups):
+    """Roofline of the dominant kernel group of the profiled matvec (per launch, from the
+    group's hipEvent time on the operator's stream)."""
+    g = groups[dom]
+    launches = max(g["launches"], 1)
+    per_launch_ms = g["ms"] / launches
+    if dom == "translate_anterpolate":
+        kern = "k_m2l_cl (sibling-cluster translation + anterpolation, FP32 vector ALUs)"
+        work, unit, peak, bound = translation_flops(op, R, 2) / launches, "TFLOP/s", FP32_PEAK_TFS, "mfma"
+        note = ("compute roofline at the FP32 dense peak; the kernel does its FP32 FMAs on the vector ALUs, where "
+                "packed FP32 instructions are disabled in this build (half that rate); counters: the vector-memory "
+                "address unit is busy about 70% of the kernel's time, VALU issue about 44%")
+    elif dom == "aggregate":
+        kern = "k_aggregate_all (FP32 matrix cores, all right-hand sides)"
+        work, unit, peak, bound = far_flops(op, R, dom) / launches, "TFLOP/s", FP32_PEAK_TFS, "mfma"
+        note = "FP32 MFMA dense peak"
+    elif dom == "disaggregate":
+        kern = "k_disagg_bf3 (split-pair BF16 matrix cores, all right-hand sides) + k_beff_all"
+        work, unit, peak, bound = far_flops(op, R, dom) / launches, "TFLOP/s", 2500.0, "mfma"
+        note = "BF16 MFMA dense peak; the split-pair product issues three BF16 products per complex multiply-add"
+    elif dom == "near":
+        kern = "k_near_bf16 (split-pair BF16 near field, all right-hand sides)"
+        work, unit, peak, bound = (op.near_entries * 8.0 + 2 * op.n * R * 8 + 2 * op.n * R * 16) / launches, \
+            "GB/s", HBM_PEAK_GBS, "hbm"
+        note = "BF16 hi/lo record per stored entry (8 B) + split X (8 B) + FP64 Y (16 B)"
+    else:
+        return {"kernel_group": dom, "ms_per_matvec": round(g["ms"], 3)}
+    scale = 1e12 if unit == "TFLOP/s" else 1e9
+    achieved = work / (per_launch_ms * 1e-3) / scale
+    return {"bound": bound, "kernel": kern, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "traffic": None, "work_per_launch": work,
+            "ms_per_matvec": round(g["ms"], 3), "launches_per_matvec": g["launches"],
+            "ms_per_launch": round(per_launch_ms, 4), "note": note}
+
+
+def matvec_steps(op, args, R):
+    """W warm-up and K timed R-right-hand-side mixed matvecs on device buffers (recorded
+    graph, the solver's form), then one profiled matvec for the stage groups (hipEvents on
+    the operator's stream) and the dominant group's roofline."""
+    import torch
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(12345)
+    Xd = torch.randn(R, op.n, dtype=torch.complex128, device="cuda", generator=gen)
+    Yd = torch.empty_like(Xd)
+    op.set_graphs(True)
+    for _ in range(args.warmup):
+        op.apply_device(Xd.data_ptr(), Yd.data_ptr(), R)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.apply_device(Xd.data_ptr(), Yd.data_ptr(), R)
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - t0) / args.steps
+    log(f"step: {R}-RHS mixed matvec {1e3 * step_s:.2f} ms ({R / step_s:.0f} RHS-matvecs/s)")
+    op.set_graphs(False)
+    op.set_profiling(True)
+    op.apply_device(Xd.data_ptr(), Yd.data_ptr(), R)
+    torch.cuda.synchronize()
+    prof = op.profile()
+    op.set_profiling(False)
+    groups = prof["groups"]
+    dom = max(groups, key=lambda g: groups[g]["ms"])
+    stage_ms = {g: round(v["ms"], 3) for g, v in groups.items()}
+    log(f"profiled matvec stage groups (ms): {stage_ms}")
+    return step_s, stage_ms, group_roofline(op, R, dom, groups)
+
+
+def c5_matvec(args):
+    """The C5 operator's 64-right-hand-side mixed matvec alone (set-up, autotune, K steps),
+    for kernel profiling; --with-precond adds two batched applications of a degree-6
+    near-field polynomial (the preconditioner's BF16 products)."""
+    import numpy as np
+    import torch
+    from mlfma_pmchwt_amd import api, configs
+    eps = configs.c5_eps(args.seed)
+    R = 64
+    op = api.Operator(configs.c5_mesh(args.frequency_c5), integral_equation=api.PMCHWT, eps_r_re=eps,
+                      far_precision=api.FP32, near_precision=api.BF16, precond_precision=api.BF16, max_rhs=R)
+    op.set_apply_precision(api.FP32)
+    tune = op.autotune(R, api.FP32)
+    step_s, stage_ms, roofline = matvec_steps(op, args, R)
+    if args.with_precond:
+        op.set_precond_poly(np.array([1.0, 0.5, 0.25, 0.1, 0.05, 0.02, 0.01], dtype=complex))
+        Xh = np.random.default_rng(2).standard_normal((op.n, R)) + 0j
+        for _ in range(2):
+            op.precond_apply(Xh)
+    print(json.dumps({"metric": "RHS-matvecs/s (64-RHS mixed matvec, C5 operator)", "value": round(R / step_s, 1),
+                      "unit": "RHS-matvec/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(1e3 * step_s, 3), "higher_is_better": True, "data": "synthetic",
+                      "dtype": "f32+bf16", "config": {"workload": f"C5 operator, {op.n} unknowns, 64 RHS",
+                                                      "tuned": tune},
+                      "stage_ms_per_64rhs_matvec_profiled": stage_ms, "roofline": roofline}))
+    op.close()
+
+
 # ---------------------------------------------------------------- CPU baseline (child process)
 def cpu_baseline_child(argv):
     """Runs in its own process (OpenMP placement set before the runtime starts): the oracle's
@@ -212,46 +321,8 @@ def c5_headline(args):
     log(f"mixed + inner near-field solve: {inner}")
     Xi = None
 
-    # ---- steps: K timed 64-RHS mixed matvecs (device buffers, recorded graph)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(12345)
-    Xd = torch.randn(R, op.n, dtype=torch.complex128, device="cuda", generator=gen)
-    Yd = torch.empty_like(Xd)
-    for _ in range(args.warmup):
-        op.apply_device(Xd.data_ptr(), Yd.data_ptr(), R)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        op.apply_device(Xd.data_ptr(), Yd.data_ptr(), R)
-    torch.cuda.synchronize()
-    step_s = (time.perf_counter() - t0) / args.steps
-    log(f"step: 64-RHS mixed matvec {1e3 * step_s:.2f} ms ({R / step_s:.0f} RHS-matvecs/s)")
-
-    # ---- profiled matvec: per-group hipEvent times on the operator's stream
-    op.set_graphs(False)
-    op.set_profiling(True)
-    op.apply_device(Xd.data_ptr(), Yd.data_ptr(), R)
-    torch.cuda.synchronize()
-    prof = op.profile()
-    op.set_profiling(False)
-    groups = prof["groups"]
-    dom = max(groups, key=lambda g: groups[g]["ms"])
-    stage_ms = {g: round(v["ms"], 3) for g, v in groups.items()}
-    roofline = None
-    if dom == "translate_anterpolate":
-        flops = translation_flops(op, R, 2)
-        per_launch_ms = groups[dom]["ms"] / max(groups[dom]["launches"], 1)
-        achieved = flops / (groups[dom]["ms"] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "kernel": "k_m2l_cl (translation + anterpolation, FP32)",
-                    "achieved": round(achieved, 2), "peak": FP32_PEAK_TFS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP32_PEAK_TFS, 4), "traffic": None,
-                    "flops_per_matvec": flops, "ms_per_matvec": round(groups[dom]["ms"], 3),
-                    "launches_per_matvec": groups[dom]["launches"], "ms_per_launch": round(per_launch_ms, 4),
-                    "note": "compute roofline at the FP32 dense peak; the kernel does its FP32 FMAs on the vector "
-                            "ALUs, where packed FP32 instructions are disabled in this build (half that rate)"}
-    else:
-        roofline = {"kernel_group": dom, "ms_per_matvec": round(groups[dom]["ms"], 3)}
-    Xd = Yd = None
+    # ---- steps: K timed 64-RHS mixed matvecs (device buffers, recorded graph) and a profiled one
+    step_s, stage_ms, roofline = matvec_steps(op, args, R)
     torch.cuda.empty_cache()
 
     n = op.n
@@ -467,7 +538,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed 64-RHS mixed matvecs (and C2 FP64 matvecs)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c5", choices=["c5", "pec_cfie", "multi_object"])
+    ap.add_argument("--workload", default="c5", choices=["c5", "c5_matvec", "pec_cfie", "multi_object"])
+    ap.add_argument("--copy-reference", action="store_true", help="a plain 1 GiB device read + write first")
+    ap.add_argument("--with-precond", action="store_true", help="c5_matvec: two batched polynomial applications")
     ap.add_argument("--seed", type=int, default=1000, help="C5: eps_r = U[2,8] from default_rng(seed)")
     ap.add_argument("--frequency-c5", type=int, default=71, help="C5 geodesic frequency (30 n^2 RWG)")
     ap.add_argument("--inner", type=int, default=10, help="GMRES steps of the inner near-field solve")
@@ -492,8 +565,18 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_reference:
+        # a plain 1 GiB device read + write (the fill before it only writes): the
+        # reference that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run
+        a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.add(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        a = b = None
     if args.workload == "multi_object":
         multi_object(args)
+    elif args.workload == "c5_matvec":
+        c5_matvec(args)
     elif args.workload == "pec_cfie":
         print(json.dumps({"metric": "MLFMA matvec/s", **pec_cfie_matvec(args)}))
     else:

@@ -61,6 +61,26 @@ __device__ __forceinline__ void cmac(double2& acc, double2 a, double2 b) {
     acc.y = __builtin_fma(a.y, b.x, acc.y);
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs of the MI355X (block b on
+// XCD b % 8).  Chunked order for a grid of units x ng groups: chunk c (units
+// c cs .. c cs + cs - 1 with all their groups) runs on XCD c % 8, its blocks
+// consecutive there, so the units of a chunk (e.g. the row ranges of one
+// leaf, or the sample groups of one leaf) and the groups of a unit share
+// their inputs through that XCD's L2, while consecutive chunks still spread
+// over all XCDs.  The grid is xcd_chunk_grid blocks; a block whose unit is
+// past the unit count has no work.
+__device__ __forceinline__ void xcd_chunk(int b, int cs, int ng, int& unit, int& group) {
+    const int x = b & 7, s = b >> 3;
+    const int per = cs * ng;
+    const int k = s / per, j = s - k * per;
+    const int u = j / ng;
+    unit = (k * 8 + x) * cs + u;
+    group = j - u * ng;
+}
+inline unsigned xcd_chunk_grid(int units, int cs, int ng) {
+    return (unsigned)((units + 8 * cs - 1) / (8 * cs)) * 8u * (unsigned)cs * (unsigned)ng;
+}
+
 // Conversions between the FP64 solver vectors and the far-field precision.
 __device__ __forceinline__ void narrow(double2 a, double2& b) { b = a; }
 __device__ __forceinline__ void narrow(double2 a, float2& b) { b = make_float2((float)a.x, (float)a.y); }

@@ -614,7 +614,7 @@ void Engine::collect_profile(int count) {
         prof_ms[0] += span(6 * w);
         prof_ms[1] += span(6 * w + 2);
         prof_ms[2] += span(6 * w + 4);
-        prof_launch[0] += 1;
+        prof_launch[0] += eall_R_ > 0 ? 0 : 1;   // aggregated for all right-hand sides beforehand
         prof_launch[1] += L - 2;
         prof_launch[2] += L - 1;
     }
@@ -678,15 +678,29 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     // aggregates all R into the passes' leaf workspaces, the passes write
     // their effective matrices into Eall, and one product disaggregates all R,
     // so each pattern row is read once per matvec in each direction
-    // (per-pass products when profiling, so the stage timings keep their
-    // meaning)
-    const bool all = f32 && mfma_near && disagg_mfma_ && !prof_on_;
+    // (with profiling on, the two all-R products are timed into the
+    // aggregation and disaggregation groups)
+    const bool all = f32 && mfma_near && disagg_mfma_;
+    auto timed = [&](int group, int launches, auto&& body) {
+        if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
+        body();
+        if (prof_on_) {
+            MLFMA_HIP(hipEventRecord(ev_[17], st));
+            MLFMA_HIP(hipEventSynchronize(ev_[17]));
+            float ms = 0;
+            MLFMA_HIP(hipEventElapsedTime(&ms, ev_[16], ev_[17]));
+            prof_ms[group] += ms;
+            prof_launch[group] += launches;
+        }
+    };
     if (all) {
         ensure_eall(R);
         const int L = tree.max_level;
-        for (int w = 0; w < nw; ++w)
-            launch_aggregate_all(st, ncur, nleaf, wd_[w].K[L], N, R, cap, d_leaf_ptr_, d_perm_, far<float>(w).Vs, X,
-                                 d_Sall_[w]);
+        timed(0, nw, [&] {
+            for (int w = 0; w < nw; ++w)
+                launch_aggregate_all(st, ncur, nleaf, wd_[w].K[L], N, R, cap, d_leaf_ptr_, d_perm_, far<float>(w).Vs,
+                                     X, d_Sall_[w]);
+        });
         eall_R_ = R;
     }
     try {
@@ -705,12 +719,14 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
     if (all) {
         const int L = tree.max_level;
         const bool pm = ie == MLFMA_PMCHWT;
-        if (d_Vb_[0])   // split-pair BF16 (near_precision = MLFMA_BF16)
-            launch_disagg_bf3(st, R, nd_, pm, wd_[0].K[L], d_Vb_[0], d_Eall_[0], pm ? wd_[1].K[L] : 0,
-                              pm ? d_Vb_[1] : nullptr, d_Eall_[1], Y);
-        else
-            launch_disagg_all(st, R, nd_, pm, wd_[0].K[L], far<float>(0).Vf, d_Eall_[0], pm ? wd_[1].K[L] : 0,
-                              pm ? far<float>(1).Vf : nullptr, d_Eall_[1], Y);
+        timed(4, 1, [&] {
+            if (d_Vb_[0])   // split-pair BF16 (near_precision = MLFMA_BF16)
+                launch_disagg_bf3(st, R, nd_, pm, wd_[0].K[L], d_Vb_[0], d_Eall_[0], pm ? wd_[1].K[L] : 0,
+                                  pm ? d_Vb_[1] : nullptr, d_Eall_[1], Y);
+            else
+                launch_disagg_all(st, R, nd_, pm, wd_[0].K[L], far<float>(0).Vf, d_Eall_[0], pm ? wd_[1].K[L] : 0,
+                                  pm ? far<float>(1).Vf : nullptr, d_Eall_[1], Y);
+        });
     }
 }
 

@@ -428,17 +428,18 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
     constexpr int kQ = 32, kP = kQ + 8;   // staged q per chunk, padded LDS row (80 B)
     constexpr int NE = NO * kQ * RT, NU = (NE + 255) / 256;
     __shared__ __align__(16) unsigned short es[2][NO][2][RT][kP];   // [hi, lo][output][Re, Im][rhs][q]
-    // the ny right-hand-side groups of one item run 8 blocks apart, on one
-    // XCD, so the later groups read the item's pattern planes from its L2
+    // chunks of 4 items (mostly the row ranges of one leaf, which read the
+    // same effective matrices) with their ny right-hand-side groups (which
+    // read the same pattern planes) run together on one XCD
     const int ny = (R + RT - 1) / RT;
-    const int slot = blockIdx.x / (8 * ny), j8 = blockIdx.x - slot * 8 * ny;
-    const int it = slot * 8 + (j8 & 7);
+    int it, gy;
+    xcd_chunk(blockIdx.x, 4, ny, it, gy);
     if (it >= nd.nitem) return;
     const int2 item = nd.items[it];
     const int t = item.x, rp = item.y;
     const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
     const int nrp = min(4, ((nt + 15) >> 4) - rp);
-    const int rb0 = (j8 >> 3) * RT;
+    const int rb0 = gy * RT;
     const int rv = min(RT, R - rb0);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
@@ -555,7 +556,7 @@ void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, in
     if (nd.nitem == 0 || R < 1) return;
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     const int ny = (R + 8 * ct - 1) / (8 * ct);
-    const unsigned grid = (unsigned)((nd.nitem + 7) / 8) * 8 * ny;   // see the block mapping in k_disagg_bf3
+    const unsigned grid = xcd_chunk_grid(nd.nitem, 4, ny);
 #define DB(NO, CT) k_disagg_bf3<NO, CT><<<grid, 256, 0, st>>>(nd, R, NO, K1, P1, E1, K2, P2, E2, y)
 #define DBN(NO)                                                                                                       \
     if (ct == 4) DB(NO, 4);                                                                                           \
@@ -592,15 +593,15 @@ __global__ __launch_bounds__(256) void k_aggregate_all(int nleaf, int K, int N, 
     constexpr int NE = kJ * CC, NU = (NE + 255) / 256;
     __shared__ float2 xs[kJ][CC];
     const int Q = 3 * K, QG = (Q + 63) / 64;
-    // the column groups of one (leaf, sample group) run 8 blocks apart, on one
-    // XCD, so the later ones read its pattern rows from that XCD's L2
+    // the sample groups of one leaf (which read the same X rows) with their
+    // column groups (which read the same pattern rows) run together on one XCD
     const int ncol = NCUR * R, ncg = (ncol + CC - 1) / CC;
-    const int slot = blockIdx.x / (8 * ncg), j8 = blockIdx.x - slot * 8 * ncg;
-    const int lq = slot * 8 + (j8 & 7);
+    int lq, cg;
+    xcd_chunk(blockIdx.x, QG, ncg, lq, cg);
     if (lq >= nleaf * QG) return;
     const int leaf = lq / QG, qg = lq - leaf * QG;
     const int i0 = leaf_ptr[leaf], nt = leaf_ptr[leaf + 1] - i0;
-    const int cb0 = (j8 >> 3) * CC;
+    const int cb0 = cg * CC;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int part = li >> 3;
@@ -690,8 +691,8 @@ void launch_aggregate_all(hipStream_t st, int ncur, int nleaf, int K, int N, int
     if (nleaf == 0 || R < 1) return;
     const int ncol = ncur * R;
     const int ct = ncol > 32 ? 8 : (ncol > 16 ? 4 : (ncol > 8 ? 2 : 1));
-    const int nlq = nleaf * ((3 * K + 63) / 64), ncg = (ncol + 8 * ct - 1) / (8 * ct);
-    const unsigned grid = (unsigned)((nlq + 7) / 8) * 8 * ncg;   // see the block mapping in k_aggregate_all
+    const int QG = (3 * K + 63) / 64, ncg = (ncol + 8 * ct - 1) / (8 * ct);
+    const unsigned grid = xcd_chunk_grid(nleaf * QG, QG, ncg);
 #define AA(NC, CT) k_aggregate_all<NC, CT><<<grid, 256, 0, st>>>(nleaf, K, N, R, cap, leaf_ptr, perm, Vs, x, Sall)
 #define AAN(NC)                                                                                                       \
     if (ct == 8) AA(NC, 8);                                                                                           \

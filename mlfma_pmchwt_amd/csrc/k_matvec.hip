@@ -24,6 +24,18 @@ __device__ __forceinline__ int xcd_remap(int lin, int total) {
     return x * per + min(x, rem) + s;
 }
 
+// Window order of n items x nt tiles: items go in windows of wc, and inside
+// a window the tile index is the slow one, so the workgroups that run at
+// once on one XCD work on the same tile of wc neighbouring items (which
+// share most of their inputs in that XCD's L2); wc = 1 is item-major order.
+__device__ __forceinline__ void window_order(int lin, int n, int nt, int wc, int& item, int& tile) {
+    const int span = wc * nt;
+    const int win = lin / span, r = lin - win * span;
+    const int w = min(wc, n - win * wc);
+    tile = r / w;
+    item = win * wc + (r - tile * w);
+}
+
 // ---------------------------------------------------------------- aggregation
 // Code/ProcessMLFMM.h:33-56 (mexp), J and M currents of PMCHWT in one pass.
 // One workgroup per leaf, threads over the K samples (blockDim = K rounded up
@@ -364,7 +376,7 @@ __device__ __forceinline__ void m2l_cl_body(int cl, int K, int q, int c0, const 
 }
 
 template <typename R, int NC, int G>
-__global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, const int* __restrict__ cl_child,
+__global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, int wc, const int* __restrict__ cl_child,
                                                    const int* __restrict__ cl_ptr, const int* __restrict__ cl_src,
                                                    const int* __restrict__ cl_vec, const cplx_t<R>* __restrict__ TFw,
                                                    const cplx_t<R>* __restrict__ Sm, int Kp,
@@ -373,8 +385,8 @@ __global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, c
                                                    const R* __restrict__ t_w, const cplx_t<R>* __restrict__ shdn,
                                                    const cplx_t<R>* __restrict__ Bmp, cplx_t<R>* __restrict__ Bm) {
     constexpr int CPT = NC * 3 / G;
-    const int lin = xcd_remap(blockIdx.x, ncl * ktiles);
-    const int cl = lin / ktiles, kt = lin % ktiles;
+    int cl, kt;
+    window_order(xcd_remap(blockIdx.x, ncl * ktiles), ncl, ktiles, wc, cl, kt);
     const int c0 = (threadIdx.x >> 6) * CPT;
     const int q = kt * 64 + (threadIdx.x & 63);
     if (q >= K) return;
@@ -398,10 +410,13 @@ void launch_m2l_cl(hipStream_t st, bool wide, int ncur, int ncl, int K, const in
                    const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
     if (ncl == 0) return;
     const int kt = (K + 63) / 64;
+    // windows of 128 clusters: the fetch from memory drops from about 7.6 to
+    // 4.5 times the level's workspaces (the kernel's time does not change)
+    constexpr int wc = 128;
     // columns per thread: 3 in FP64, 6 in FP32 (3 when there are only 3)
     constexpr bool f64 = sizeof(R) == 8;
 #define M2LC_G(NC, G)                                                                                                 \
-    k_m2l_cl<R, NC, G><<<ncl * kt, 64 * G, 0, st>>>(ncl, K, kt, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp,     \
+    k_m2l_cl<R, NC, G><<<ncl * kt, 64 * G, 0, st>>>(ncl, K, kt, wc, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp, \
                                                     parent, octant, t_ptr, t_kp, t_w, shift_down, Bm_parent, Bm)
 #define M2LC(NC)                                                                                                      \
     {                                                                                                                 \

@@ -151,19 +151,19 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     using xb_t = typename std::conditional<S, uint2, unsigned>::type;
     const xb_t* __restrict__ xb = static_cast<const xb_t*>(xbv);
     __shared__ __align__(16) unsigned short xs[NS][NX][2][RT][kKP];   // [part][current][Re, Im][rhs][column]
-    // Blocks go round-robin to the 8 XCDs: the ny right-hand-side groups of one
-    // item are 8 blocks apart, so they run on one XCD at about the same time
-    // and the later groups read the item's records from that XCD's L2.
+    // Chunks of 4 items (mostly the row ranges of one leaf, which gather the
+    // same X columns) with their ny right-hand-side groups (which read the
+    // same records) run together on one XCD, so the later reads hit its L2.
     const int ny = (R + RT - 1) / RT;
-    const int slot = blockIdx.x / (8 * ny), j8 = blockIdx.x - slot * 8 * ny;
-    const int it = slot * 8 + (j8 & 7);
+    int it, gy;
+    xcd_chunk(blockIdx.x, 4, ny, it, gy);
     if (it >= nd.nitem) return;
     const int2 item = nd.items[it];
     const int t = item.x, rp = item.y;
     const int i0 = nd.leaf_ptr[t], nt = nd.leaf_ptr[t + 1] - i0;
     const int nrt = (nt + 15) >> 4;
     const int nrp = min(kRTC, nrt - rp);
-    const int rb0 = (j8 >> 3) * RT;
+    const int rb0 = gy * RT;
     const int rv = min(RT, R - rb0);
     const int C = nd.tgt_cols[t];
     const int nkb = (C + 31) >> 5;
@@ -356,7 +356,7 @@ static void near_bf16_t(hipStream_t st, int R, int ldr, int r0, const NearDev& n
     else k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, (unsigned*)xb);
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     const int ny = (R + 8 * ct - 1) / (8 * ct);
-    const unsigned grid = (unsigned)((nd.nitem + 7) / 8) * 8 * ny;   // see the block mapping in k_near_bf16
+    const unsigned grid = xcd_chunk_grid(nd.nitem, 4, ny);
 #define NB(P, CT) k_near_bf16<P, CT, S><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
 #define NBP(P)                                                                                                        \
     if (ct == 4) NB(P, 4);                                                                                            \
