@@ -205,6 +205,23 @@ int mlfma_solve(mlfma_handle* h, const mlfma_solve_params* sp, int n, const doub
 int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int nrhs, const double* rhs, double* x,
                       mlfma_solve_stats* stats);
 
+/* Several objects solved together (no reference counterpart; replaces the
+ * one-operator-per-task, one-task-after-another loop of
+ * Code/BatchRunner.h:470-480).  Object k is handle hs[k] with nrhs[k] (1..256)
+ * right-hand sides rhs[k] (n_k x nrhs[k] complex, column-major) and initial
+ * guesses / solutions x[k]; stats[k] (may be NULL, as may stats) receives
+ * nrhs[k] entries.  The objects' batched GMRES runs (as mlfma_solve_batch,
+ * same parameters for all) advance in lockstep, and every Arnoldi step
+ * launches the preconditioned matvecs of all objects still iterating as one
+ * recorded graph with one branch per object, so objects of uneven size run
+ * side by side from one launch.  Each object's iterates are those of
+ * mlfma_solve_batch on it alone, except that a block whose columns have
+ * partly converged keeps its width.  Every handle needs mlfma_set_graphs(h, 1)
+ * and profiling off, all on one device, each handle at most once.  Errors are
+ * reported through hs[0]. */
+int mlfma_solve_multi(int count, mlfma_handle* const* hs, const mlfma_solve_params* sp, const int* nrhs,
+                      const double* const* rhs, double* const* x, mlfma_solve_stats* const* stats);
+
 /* ---- Learned near-field preconditioner (no reference counterpart; the
  * reference solves unpreconditioned, Code/MLFMM.cpp:370-397).
  *

@@ -92,6 +92,7 @@ def library() -> C.CDLL:
         lib.mlfma_mlp_train.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_float, vp]
         lib.mlfma_solve.argtypes = [vp, C.POINTER(SolveParams), C.c_int, vp, vp, C.POINTER(SolveStats)]
         lib.mlfma_solve_batch.argtypes = [vp, C.POINTER(SolveParams), C.c_int, C.c_int, vp, vp, vp]
+        lib.mlfma_solve_multi.argtypes = [C.c_int, vp, C.POINTER(SolveParams), vp, vp, vp, vp]
         lib.mlfma_excitation.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         lib.mlfma_rcs.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, vp]
         lib.mlfma_excitation_batch.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -497,6 +498,40 @@ def scatter_grid(theta_start, theta_end, phi_start, phi_end, step):
     th, ph = np.empty(n.value), np.empty(n.value)
     library().mlfma_scatter_grid(theta_start, theta_end, phi_start, phi_end, step, C.byref(n), _p(th), _p(ph))
     return th, ph
+
+
+def _stats_dicts(st):
+    return [{"iterations": s.iterations, "matvecs": s.matvecs, "converged": bool(s.converged),
+             "rel_residual": s.rel_residual, "seconds": s.seconds, "fp64_matvecs": s.fp64_matvecs,
+             "precond_fallbacks": s.precond_fallbacks} for s in st]
+
+
+def solve_multi(ops, rhs_list, x0_list=None, itr_max=None, restart=30, tol_rel=1e-3, tol_abs=0.0, precond=0):
+    """Batched GMRES of several operators in lockstep (mlfma_solve_multi): every Arnoldi step
+    runs all still-iterating objects' matvecs as one recorded graph.  rhs_list[k] is n_k x R_k;
+    returns ([X_k], [[stats per right-hand side] per object]).  Graphs are switched on."""
+    K = len(ops)
+    if K < 1 or len(rhs_list) != K:
+        raise ValueError("one right-hand-side block per operator")
+    rhs = [np.asfortranarray(b, dtype=np.complex128) for b in rhs_list]
+    for op, b in zip(ops, rhs):
+        if b.ndim != 2 or b.shape[0] != op.n:
+            raise ValueError("rhs must be n x R for its operator")
+        op.set_graphs(True)
+    xs = [np.zeros_like(b, order="F") if x0_list is None or x0_list[k] is None
+          else np.asfortranarray(x0_list[k], dtype=np.complex128).copy(order="F") for k, b in enumerate(rhs)]
+    sp = SolveParams(GMRES, itr_max if itr_max is not None else 4 * max(op.n for op in ops), restart, tol_rel,
+                     tol_abs, precond)
+    sts = [(SolveStats * b.shape[1])() for b in rhs]
+    hs = (C.c_void_p * K)(*[op._h for op in ops])
+    nr = (C.c_int * K)(*[b.shape[1] for b in rhs])
+    bp = (C.c_void_p * K)(*[b.ctypes.data for b in rhs])
+    xp = (C.c_void_p * K)(*[x.ctypes.data for x in xs])
+    stp = (C.c_void_p * K)(*[C.cast(s, C.c_void_p) for s in sts])
+    rc = library().mlfma_solve_multi(K, hs, C.byref(sp), nr, bp, xp, stp)
+    if rc not in (0, 5):
+        ops[0]._check(rc)
+    return xs, [_stats_dicts(s) for s in sts]
 
 
 class Batch:

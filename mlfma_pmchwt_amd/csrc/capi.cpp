@@ -257,6 +257,53 @@ int mlfma_solve_batch(mlfma_handle* h, const mlfma_solve_params* sp, int n, int 
     return conv ? MLFMA_OK : MLFMA_ERR_NOT_CONVERGED;
 }
 
+int mlfma_solve_multi(int count, mlfma_handle* const* hs, const mlfma_solve_params* sp, const int* nrhs,
+                      const double* const* rhs, double* const* x, mlfma_solve_stats* const* stats) {
+    if (count < 1 || !hs || !sp || !nrhs || !rhs || !x) return MLFMA_ERR_ARG;
+    for (int k = 0; k < count; ++k)
+        if (!hs[k]) return MLFMA_ERR_ARG;
+    int conv = 1;
+    int rc = guarded(hs[0], [&] {
+        if (sp->solver != MLFMA_SOLVER_GMRES) throw mlfma::Error(MLFMA_ERR_ARG, "lockstep solve supports GMRES only");
+        if (sp->itr_max < 0 || sp->restart < 1 || sp->tol_rel < 0) throw mlfma::Error(MLFMA_ERR_ARG, "bad solver parameters");
+        std::vector<mlfma::Engine*> es(count);
+        std::vector<const double2*> B(count);
+        std::vector<double2*> X(count);
+        std::vector<int> R(count);
+        std::vector<std::vector<mlfma_solve_stats>> st(count);
+        std::vector<mlfma_solve_stats*> sp_out(count);
+        for (int k = 0; k < count; ++k) {
+            mlfma::Engine& e = *hs[k]->eng;
+            if (nrhs[k] < 1 || nrhs[k] > 256 || !rhs[k] || !x[k])
+                throw mlfma::Error(MLFMA_ERR_ARG, "lockstep solve: 1..256 right-hand sides and buffers per object");
+            const size_t cnt = (size_t)e.nunk * nrhs[k];
+            hs[k]->io.ensure(cnt);
+            MLFMA_HIP(hipMemcpy(hs[k]->io.x, rhs[k], cnt * sizeof(double2), hipMemcpyHostToDevice));
+            MLFMA_HIP(hipMemcpy(hs[k]->io.y, x[k], cnt * sizeof(double2), hipMemcpyHostToDevice));
+            e.block_load(hs[k]->io.x, hs[k]->io.y, nrhs[k]);
+            es[k] = &e;
+            B[k] = e.block_b();
+            X[k] = e.block_x();
+            R[k] = nrhs[k];
+            st[k].resize(nrhs[k]);
+            sp_out[k] = st[k].data();
+        }
+        mlfma::solve_multi(es, *sp, B, X, R, sp_out);
+        for (int k = 0; k < count; ++k) {
+            mlfma::Engine& e = *hs[k]->eng;
+            const size_t cnt = (size_t)e.nunk * nrhs[k];
+            e.block_store(hs[k]->io.y, nrhs[k]);
+            MLFMA_HIP(hipMemcpy(x[k], hs[k]->io.y, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+            for (int r = 0; r < nrhs[k]; ++r) {
+                if (stats && stats[k]) stats[k][r] = st[k][r];
+                conv = conv && st[k][r].converged;
+            }
+        }
+    });
+    if (rc != MLFMA_OK) return rc;
+    return conv ? MLFMA_OK : MLFMA_ERR_NOT_CONVERGED;
+}
+
 int mlfma_precond_features(mlfma_handle* h, float* features) {
     if (!h || !features) return MLFMA_ERR_ARG;
     return guarded(h, [&] { h->eng->precond_features(features); });

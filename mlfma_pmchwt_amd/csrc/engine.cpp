@@ -68,6 +68,8 @@ Engine::Engine(const mlfma_params& p) : P(p) {
 
 Engine::~Engine() {
     if (st) hipStreamSynchronize(st);
+    if (h_get_) hipHostFree(h_get_);
+    if (h_put_) hipHostFree(h_put_);
     for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : ev_)
         if (e) hipEventDestroy(e);
@@ -911,10 +913,18 @@ void Engine::dfree(void* p) {
 // Column-major device blocks (vector j at b + j * nunk), as the C ABI passes them.
 void Engine::solve_device(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, int R,
                           mlfma_solve_stats* stats) {
+    block_load(d_b, d_x, R);
+    solve_block(sp, d_bx_, d_by_, R, stats);
+    block_store(d_x, R);
+}
+
+void Engine::block_load(const double2* d_b, const double2* d_x, int R) {
     ensure_block(R);
     launch_transpose(st, R, nunk, d_b, d_bx_);
     launch_transpose(st, R, nunk, d_x, d_by_);
-    solve_block(sp, d_bx_, d_by_, R, stats);
+}
+
+void Engine::block_store(double2* d_x, int R) {
     launch_transpose(st, nunk, R, d_by_, d_x);
     MLFMA_HIP(hipStreamSynchronize(st));
 }
@@ -1216,25 +1226,102 @@ void Engine::solve(const mlfma_solve_params& sp, const double2* d_b, double2* d_
 // the polynomial preconditioner is applied in flexible form as in solve().
 // Once a quarter of the block has converged, the rest continues as a narrower
 // block (a recursive call), so converged columns cost no further matvecs.
-void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R,
-                         mlfma_solve_stats* stats) {
-    const double t0 = wall();
-    const int n = nunk;
-    const int m = std::max(1, sp.restart);
-    const double eps = 1e-16;
+// ---------------------------------------------------------------- batched GMRES
+// Lockstep flexible GMRES(m) over the R right-hand sides of one operator, as
+// phases on a BlockState: start, per restart cycle begin / steps / end, and
+// per step the (preconditioned) matvec (pre, body between fixed buffers, post)
+// and four orthogonalisation phases.  The host reads the device between the
+// orthogonalisation phases only after synchronising st, so several operators
+// can advance their phases together with one synchronisation per phase
+// (solve_multi), and their matvec bodies can be recorded into one graph.
+// Reductions go through pinned host buffers (h_get_ device -> host, h_put_
+// host -> device), rewritten only after such a synchronisation.
+void Engine::ensure_pinned(size_t count) {
+    if (count <= pin_cap_) return;
+    MLFMA_HIP(hipStreamSynchronize(st));
+    put_pending_ = false;
+    if (h_get_) hipHostFree(h_get_);
+    if (h_put_) hipHostFree(h_put_);
+    h_get_ = h_put_ = nullptr;
+    pin_cap_ = 0;
+    MLFMA_HIP(hipHostMalloc((void**)&h_get_, count * sizeof(double2), hipHostMallocDefault));
+    MLFMA_HIP(hipHostMalloc((void**)&h_put_, count * sizeof(double2), hipHostMallocDefault));
+    pin_cap_ = count;
+}
+
+void Engine::bs_issue_norms(BlockState& s, const double2* v) {
+    launch_norm2_partial_b(st, s.n, s.R, v, d_bpart_, kRedBlocks);
+    launch_reduce_partials_b(st, 1, s.R, kRedBlocks, d_bpart_, d_bsmall_);
+    MLFMA_HIP(hipMemcpyAsync(h_get_, d_bsmall_, s.R * sizeof(double2), hipMemcpyDeviceToHost, st));
+}
+
+void Engine::bs_read_norms(const BlockState& s, std::vector<double>& out) const {
+    out.resize(s.R);
+    for (int r = 0; r < s.R; ++r) out[r] = std::sqrt(h_get_[r].x);
+}
+
+void Engine::bs_issue_dots(BlockState& s) {   // <V_j, W>_r, j <= k
+    launch_multidot_b(st, s.n, s.R, s.k + 1, d_bV_, s.nR, d_bW_, d_bpart_, kRedBlocks);
+    launch_reduce_partials_b(st, s.k + 1, s.R, kRedBlocks, d_bpart_, d_bsmall_);
+    MLFMA_HIP(hipMemcpyAsync(h_get_, d_bsmall_, (size_t)(s.k + 1) * s.R * sizeof(double2), hipMemcpyDeviceToHost, st));
+}
+
+// the host -> device staging buffer, once the copy last issued from it has run
+double2* Engine::put_buffer() {
+    if (put_pending_) MLFMA_HIP(hipStreamSynchronize(st));
+    put_pending_ = false;
+    return h_put_;
+}
+
+void Engine::bs_put(const std::vector<cplx>& v, double2* dst) {
+    double2* hp = put_buffer();
+    for (size_t e = 0; e < v.size(); ++e) hp[e] = d2(v[e]);
+    MLFMA_HIP(hipMemcpyAsync(dst, hp, v.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    put_pending_ = true;
+}
+
+void Engine::bs_put_scale(const std::vector<double>& a) {
+    double* hp = reinterpret_cast<double*>(put_buffer());
+    for (size_t r = 0; r < a.size(); ++r) hp[r] = a[r];
+    MLFMA_HIP(hipMemcpyAsync(d_bscale_, hp, a.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    put_pending_ = true;
+}
+
+void Engine::bs_true_residual(BlockState& s, std::vector<double>& rn) {
+    apply_block(s.X, d_bRr_, s.R, MLFMA_FP64);
+    s.matvecs += s.R;
+    s.fp64_matvecs += s.R;
+    launch_residual(st, (int)s.nR, s.B, d_bRr_);
+    bs_issue_norms(s, d_bRr_);
+    MLFMA_HIP(hipStreamSynchronize(st));
+    bs_read_norms(s, rn);
+}
+
+void Engine::bs_start(BlockState& s, const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R) {
+    s = BlockState();
+    s.t0 = wall();
+    s.sp = sp;
+    s.B = d_B;
+    s.X = d_X;
+    s.R = R;
+    s.n = nunk;
+    s.m = std::max(1, sp.restart);
+    const int m = s.m;
     if (sp.solver != MLFMA_SOLVER_GMRES) throw Error(MLFMA_ERR_ARG, "batched solve supports GMRES only");
     if (sp.precond < 0 || sp.precond > 3) throw Error(MLFMA_ERR_ARG, "batched solve: precond must be 0, 1, 2 or 3");
     if (sp.precond == 2 && poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
-    int pkind = sp.precond;   // 0 none, 1 near-field block-diagonal, 2 polynomial, 3 inner near-field solve
-    if (pkind == 1) precond_setup();
-    bool pc = pkind != 0;
+    s.pkind = sp.precond;   // 0 none, 1 near-field block-diagonal, 2 polynomial, 3 inner near-field solve
+    if (s.pkind == 1) precond_setup();
+    s.pc = s.pkind != 0;
     if (R < 1 || R > 256) throw Error(MLFMA_ERR_ARG, "batched solve: 1..256 right-hand sides");
+    s.nR = (size_t)s.n * R;
+    if (s.nR > 0x7fffffff) throw Error(MLFMA_ERR_ARG, "batched solve: n * R exceeds 2^31");
     // every workspace a batched matvec of R may use is sized here, before any
     // recorded matvec could reach it (a resize inside a capture is an error)
     ensure_xf(R);
     ensure_eall(R);
-    const size_t nR = (size_t)n * R;
-    stats_fallbacks_ = 0;
+    ensure_pinned((size_t)(m + 1) * R);
+    const size_t nR = s.nR;
     if (bV_cols_ < m + 1 || bV_R_ != R) {
         for (void* p : {(void*)d_bV_, (void*)d_bW_, (void*)d_bRr_, (void*)d_bpart_, (void*)d_bsmall_, (void*)d_bscale_,
                         (void*)d_bZ_, (void*)d_bG_})
@@ -1253,271 +1340,311 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
     }
     // the flexible basis z_k = M^-1 v_k is sized whenever a preconditioner may run
     // (a stalling polynomial falls back to the block-diagonal one mid-solve)
-    if (pc && (bZf_cols_ < m || bZf_nR_ != nR)) {
+    if (s.pc && (bZf_cols_ < m || bZf_nR_ != nR)) {
         dfree(d_bZf_);
         d_bZf_ = dalloc<double2>(nR * m);
         bZf_cols_ = m;
         bZf_nR_ = nR;
     }
-    int matvecs = 0, fp64_matvecs = 0;
-    std::vector<int> iters(R, 0);
-    auto norms = [&](const double2* v, std::vector<double>& out) {
-        launch_norm2_partial_b(st, n, R, v, d_bpart_, kRedBlocks);
-        launch_reduce_partials_b(st, 1, R, kRedBlocks, d_bpart_, d_bsmall_);
-        std::vector<double2> h(R);
-        MLFMA_HIP(hipMemcpyAsync(h.data(), d_bsmall_, R * sizeof(double2), hipMemcpyDeviceToHost, st));
-        MLFMA_HIP(hipStreamSynchronize(st));
-        out.resize(R);
-        for (int r = 0; r < R; ++r) out[r] = std::sqrt(h[r].x);
-    };
-    auto dots = [&](int cnt, std::vector<cplx>& h) {   // h[j * R + r] = <V_j, W>_r
-        launch_multidot_b(st, n, R, cnt, d_bV_, nR, d_bW_, d_bpart_, kRedBlocks);
-        launch_reduce_partials_b(st, cnt, R, kRedBlocks, d_bpart_, d_bsmall_);
-        std::vector<double2> tmp((size_t)cnt * R);
-        MLFMA_HIP(hipMemcpyAsync(tmp.data(), d_bsmall_, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
-        MLFMA_HIP(hipStreamSynchronize(st));
-        h.resize(tmp.size());
-        for (size_t e = 0; e < tmp.size(); ++e) h[e] = cx(tmp[e]);
-    };
-    auto put = [&](const std::vector<cplx>& v) {
-        std::vector<double2> tmp(v.size());
-        for (size_t e = 0; e < v.size(); ++e) tmp[e] = d2(v[e]);
-        MLFMA_HIP(hipMemcpyAsync(d_bsmall_, tmp.data(), tmp.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-        MLFMA_HIP(hipStreamSynchronize(st));
-    };
-    auto put_scale = [&](const std::vector<double>& a) {
-        MLFMA_HIP(hipMemcpyAsync(d_bscale_, a.data(), R * sizeof(double), hipMemcpyHostToDevice, st));
-        MLFMA_HIP(hipStreamSynchronize(st));
-    };
-    auto true_residual = [&](std::vector<double>& rn) {
-        apply_block(d_X, d_bRr_, R, MLFMA_FP64);
-        matvecs += R;
-        fp64_matvecs += R;
-        launch_residual(st, (int)std::min<size_t>(nR, 0x7fffffff), d_B, d_bRr_);
-        norms(d_bRr_, rn);
-    };
-    if (nR > 0x7fffffff) throw Error(MLFMA_ERR_ARG, "batched solve: n * R exceeds 2^31");
-
-    std::vector<double> bnorm, rnorm, err(R);
-    norms(d_B, bnorm);
-    for (double& b : bnorm)
+    if (s.pkind == 2) ensure_poly(R);
+    const double eps = 1e-16;
+    s.iters.assign(R, 0);
+    s.kused.assign(R, 0);
+    s.err.assign(R, 0.0);
+    bs_issue_norms(s, d_B);
+    MLFMA_HIP(hipStreamSynchronize(st));
+    bs_read_norms(s, s.bnorm);
+    for (double& b : s.bnorm)
         if (b < eps) b = 1.0;
-    true_residual(rnorm);
-    std::vector<char> done(R);
+    bs_true_residual(s, s.rnorm);
+    s.done.assign(R, 0);
     for (int r = 0; r < R; ++r) {
-        err[r] = rnorm[r] / bnorm[r];
-        if (!std::isfinite(err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite initial residual");
-        done[r] = err[r] < sp.tol_rel || err[r] < sp.tol_abs;
+        s.err[r] = s.rnorm[r] / s.bnorm[r];
+        if (!std::isfinite(s.err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite initial residual");
+        s.done[r] = s.err[r] < sp.tol_rel || s.err[r] < sp.tol_abs;
     }
-    auto all_done = [&] { return std::all_of(done.begin(), done.end(), [](char c) { return c != 0; }); };
-    std::vector<double> err_cycle = err;
-    std::vector<cplx> H((size_t)R * (m + 1) * m), cs((size_t)R * m), sn((size_t)R * m), sv((size_t)R * (m + 1));
-    auto Hm = [&](int r, int i, int j) -> cplx& { return H[((size_t)r * m + j) * (m + 1) + i]; };
-    std::vector<cplx> h1, h2, yv;
-    std::vector<int> kused(R);
-    int total = 0;
-    for (int cyc = 1; !all_done() && total < sp.itr_max; ++cyc) {
-        std::vector<double> sc(R);
-        for (int r = 0; r < R; ++r) sc[r] = (done[r] || rnorm[r] == 0.0) ? 0.0 : 1.0 / rnorm[r];
-        put_scale(sc);
-        launch_scale_copy_b(st, n, R, d_bscale_, d_bRr_, d_bV_);
-        std::fill(H.begin(), H.end(), cplx(0, 0));
-        std::fill(sv.begin(), sv.end(), cplx(0, 0));
-        std::vector<char> live(R);
-        for (int r = 0; r < R; ++r) {
-            sv[(size_t)r * (m + 1)] = rnorm[r];
-            live[r] = !done[r];
-            kused[r] = 0;
+    s.err_cycle = s.err;
+    s.H.assign((size_t)R * (m + 1) * m, cplx(0, 0));
+    s.cs.assign((size_t)R * m, cplx(0, 0));
+    s.sn.assign((size_t)R * m, cplx(0, 0));
+    s.sv.assign((size_t)R * (m + 1), cplx(0, 0));
+}
+
+bool Engine::bs_active(const BlockState& s) const {
+    return !s.stopped && s.total < s.sp.itr_max &&
+           std::any_of(s.done.begin(), s.done.end(), [](char c) { return c == 0; });
+}
+
+void Engine::bs_cycle_begin(BlockState& s) {
+    ++s.cyc;
+    const int R = s.R;
+    std::vector<double> sc(R);
+    for (int r = 0; r < R; ++r) sc[r] = (s.done[r] || s.rnorm[r] == 0.0) ? 0.0 : 1.0 / s.rnorm[r];
+    bs_put_scale(sc);
+    launch_scale_copy_b(st, s.n, R, d_bscale_, d_bRr_, d_bV_);
+    std::fill(s.H.begin(), s.H.end(), cplx(0, 0));
+    std::fill(s.sv.begin(), s.sv.end(), cplx(0, 0));
+    s.live.assign(R, 0);
+    for (int r = 0; r < R; ++r) {
+        s.sv[(size_t)r * (s.m + 1)] = s.rnorm[r];
+        s.live[r] = !s.done[r];
+        s.kused[r] = 0;
+    }
+    s.k = 0;
+}
+
+bool Engine::bs_step_open(const BlockState& s) const {
+    return s.k < s.m && s.total < s.sp.itr_max &&
+           std::any_of(s.live.begin(), s.live.end(), [](char c) { return c != 0; });
+}
+
+int Engine::bs_graph_kind(const BlockState& s) const { return s.pc && s.pkind != 3 ? (s.pkind == 2 ? 1 : 3) : 2; }
+
+// The matvec of step k.  recorded: the input goes to the fixed buffer G
+// (through the inner near-field solve first with precond 3, which synchronises
+// with the host and so is never recorded), and bs_body runs between fixed
+// buffers.  Otherwise the whole step runs here.
+void Engine::bs_pre(BlockState& s, bool recorded) {
+    ++s.total;
+    const size_t nR = s.nR;
+    double2* Vk = d_bV_ + (size_t)s.k * nR;
+    double2* Zk = s.pc ? d_bZf_ + (size_t)s.k * nR : nullptr;
+    if (recorded) {
+        if (s.pc && s.pkind == 3) {
+            precond_block(3, Vk, Zk, s.R, apply_prec);
+            MLFMA_HIP(hipMemcpyAsync(d_bG_, Zk, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        } else {
+            if (s.pc && s.pkind == 2) ensure_poly(s.R);
+            MLFMA_HIP(hipMemcpyAsync(d_bG_, Vk, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
         }
-        for (int k = 0; k < m && total < sp.itr_max; ++k) {
-            if (std::none_of(live.begin(), live.end(), [](char c) { return c != 0; })) break;
-            ++total;
-            if (pc && pkind == 3) {
-                // the inner solve synchronises with the host, so only the matvec is recorded
-                precond_block(3, d_bV_ + (size_t)k * nR, d_bZf_ + (size_t)k * nR, R, apply_prec);
-                if (use_graphs_ && !prof_on_) {
-                    MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bZf_ + (size_t)k * nR, nR * sizeof(double2),
-                                             hipMemcpyDeviceToDevice, st));
-                    const int prec = apply_prec;
-                    run_graph(GraphKey{2, R, prec}, [&] { apply_block(d_bG_, d_bW_, R, prec); });
-                } else {
-                    apply_block(d_bZf_ + (size_t)k * nR, d_bW_, R, apply_prec);
-                }
-            } else if (use_graphs_ && !prof_on_) {
-                // fixed input buffer, then one replay of the recorded (preconditioned) matvec
-                MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bV_ + (size_t)k * nR, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
-                const int prec = apply_prec;
-                if (pc) {
-                    if (pkind == 2) ensure_poly(R);
-                    run_graph(GraphKey{pkind == 2 ? 1 : 3, R, prec}, [&] {
-                        precond_block(pkind, d_bG_, d_bZ_, R, prec);
-                        apply_block(d_bZ_, d_bW_, R, prec);
-                    });
-                    MLFMA_HIP(hipMemcpyAsync(d_bZf_ + (size_t)k * nR, d_bZ_, nR * sizeof(double2),
-                                             hipMemcpyDeviceToDevice, st));
-                } else {
-                    run_graph(GraphKey{2, R, prec}, [&] { apply_block(d_bG_, d_bW_, R, prec); });
-                }
-            } else if (pc) {   // flexible: keep z_k = M^-1 v_k
-                precond_block(pkind, d_bV_ + (size_t)k * nR, d_bZf_ + (size_t)k * nR, R, apply_prec);
-                apply_block(d_bZf_ + (size_t)k * nR, d_bW_, R, apply_prec);
+    } else if (s.pc) {   // flexible: keep z_k = M^-1 v_k
+        precond_block(s.pkind, Vk, Zk, s.R, apply_prec);
+        apply_block(Zk, d_bW_, s.R, apply_prec);
+    } else {
+        apply_block(Vk, d_bW_, s.R, apply_prec);
+    }
+}
+
+void Engine::bs_body(BlockState& s) {
+    if (s.pc && s.pkind != 3) {
+        precond_block(s.pkind, d_bG_, d_bZ_, s.R, apply_prec);
+        apply_block(d_bZ_, d_bW_, s.R, apply_prec);
+    } else {
+        apply_block(d_bG_, d_bW_, s.R, apply_prec);
+    }
+}
+
+void Engine::bs_post(BlockState& s, bool recorded) {
+    if (recorded && s.pc && s.pkind != 3)
+        MLFMA_HIP(hipMemcpyAsync(d_bZf_ + (size_t)s.k * s.nR, d_bZ_, s.nR * sizeof(double2), hipMemcpyDeviceToDevice,
+                                 st));
+    s.matvecs += s.R;
+}
+
+// Classical Gram-Schmidt twice (CGS2) and the Givens update of step k:
+// phase 0 issues <V_j, W>; 1 subtracts it and issues the second pass; 2
+// subtracts that and issues ||W||; 3 updates every live right-hand side's
+// Hessenberg column and scales W into V_{k+1}.  The caller synchronises st
+// between phases.
+void Engine::bs_orth(BlockState& s, int phase) {
+    const int R = s.R, m = s.m, k = s.k;
+    const double eps = 1e-16;
+    auto Hm = [&](int r, int i, int j) -> cplx& { return s.H[((size_t)r * m + j) * (m + 1) + i]; };
+    auto read_dots = [&](std::vector<cplx>& h) {
+        h.resize((size_t)(k + 1) * R);
+        for (size_t e = 0; e < h.size(); ++e) h[e] = cx(h_get_[e]);
+    };
+    if (phase == 0) {
+        bs_issue_dots(s);
+    } else if (phase == 1 || phase == 2) {
+        std::vector<cplx>& h = phase == 1 ? s.h1 : s.h2;
+        read_dots(h);
+        bs_put(h, d_bsmall_);
+        launch_sub_combination_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bsmall_, d_bW_);
+        if (phase == 1) bs_issue_dots(s);
+        else bs_issue_norms(s, d_bW_);
+    } else {
+        std::vector<double> hn;
+        bs_read_norms(s, hn);
+        std::vector<double> inv(R, 0.0);
+        for (int r = 0; r < R; ++r) {
+            if (!s.live[r]) continue;
+            s.kused[r] = k + 1;
+            ++s.iters[r];
+            for (int j = 0; j <= k; ++j) Hm(r, j, k) = s.h1[(size_t)j * R + r] + s.h2[(size_t)j * R + r];
+            if (!std::isfinite(hn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite Arnoldi vector");
+            Hm(r, k + 1, k) = hn[r];
+            const bool breakdown = hn[r] < eps * s.rnorm[r];
+            if (!breakdown) inv[r] = 1.0 / hn[r];
+            cplx* c_ = &s.cs[(size_t)r * m];
+            cplx* s_ = &s.sn[(size_t)r * m];
+            cplx* g = &s.sv[(size_t)r * (m + 1)];
+            // Code/GMRES.h:22-48, 144-152
+            for (int j = 0; j < k; ++j) {
+                cplx tmp = std::conj(c_[j]) * Hm(r, j, k) + std::conj(s_[j]) * Hm(r, j + 1, k);
+                Hm(r, j + 1, k) = -s_[j] * Hm(r, j, k) + c_[j] * Hm(r, j + 1, k);
+                Hm(r, j, k) = tmp;
+            }
+            cplx a = Hm(r, k, k), b = Hm(r, k + 1, k);
+            if (std::abs(b) < eps) {
+                c_[k] = 1.0, s_[k] = 0.0;
+            } else if (std::abs(a) < eps) {
+                c_[k] = 0.0, s_[k] = 1.0;
             } else {
-                apply_block(d_bV_ + (size_t)k * nR, d_bW_, R, apply_prec);
+                double rr = std::sqrt(std::real(a * std::conj(a) + b * std::conj(b)));
+                c_[k] = a / rr, s_[k] = b / rr;
             }
-            matvecs += R;
-            dots(k + 1, h1);
-            put(h1);
-            launch_sub_combination_b(st, n, R, k + 1, d_bV_, nR, d_bsmall_, d_bW_);
-            dots(k + 1, h2);
-            put(h2);
-            launch_sub_combination_b(st, n, R, k + 1, d_bV_, nR, d_bsmall_, d_bW_);
-            std::vector<double> hn;
-            norms(d_bW_, hn);
-            std::vector<double> inv(R, 0.0);
-            for (int r = 0; r < R; ++r) {
-                if (!live[r]) continue;
-                kused[r] = k + 1;
-                ++iters[r];
-                for (int j = 0; j <= k; ++j) Hm(r, j, k) = h1[(size_t)j * R + r] + h2[(size_t)j * R + r];
-                if (!std::isfinite(hn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite Arnoldi vector");
-                Hm(r, k + 1, k) = hn[r];
-                const bool breakdown = hn[r] < eps * rnorm[r];
-                if (!breakdown) inv[r] = 1.0 / hn[r];
-                cplx* c_ = &cs[(size_t)r * m];
-                cplx* s_ = &sn[(size_t)r * m];
-                cplx* g = &sv[(size_t)r * (m + 1)];
-                // Code/GMRES.h:22-48, 144-152
-                for (int j = 0; j < k; ++j) {
-                    cplx tmp = std::conj(c_[j]) * Hm(r, j, k) + std::conj(s_[j]) * Hm(r, j + 1, k);
-                    Hm(r, j + 1, k) = -s_[j] * Hm(r, j, k) + c_[j] * Hm(r, j + 1, k);
-                    Hm(r, j, k) = tmp;
-                }
-                cplx a = Hm(r, k, k), b = Hm(r, k + 1, k);
-                if (std::abs(b) < eps) {
-                    c_[k] = 1.0, s_[k] = 0.0;
-                } else if (std::abs(a) < eps) {
-                    c_[k] = 0.0, s_[k] = 1.0;
-                } else {
-                    double rr = std::sqrt(std::real(a * std::conj(a) + b * std::conj(b)));
-                    c_[k] = a / rr, s_[k] = b / rr;
-                }
-                Hm(r, k, k) = std::conj(c_[k]) * a + std::conj(s_[k]) * b;
-                Hm(r, k + 1, k) = 0.0;
-                cplx tmp = std::conj(c_[k]) * g[k] + std::conj(s_[k]) * g[k + 1];
-                g[k + 1] = -s_[k] * g[k] + c_[k] * g[k + 1];
-                g[k] = tmp;
-                err[r] = std::abs(g[k + 1]) / bnorm[r];
-                if (breakdown || err[r] < sp.tol_rel || err[r] < sp.tol_abs) live[r] = 0;
-            }
-            if (k + 1 < m + 1) {
-                put_scale(inv);
-                launch_scale_copy_b(st, n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * nR);
-            }
+            Hm(r, k, k) = std::conj(c_[k]) * a + std::conj(s_[k]) * b;
+            Hm(r, k + 1, k) = 0.0;
+            cplx tmp = std::conj(c_[k]) * g[k] + std::conj(s_[k]) * g[k + 1];
+            g[k + 1] = -s_[k] * g[k] + c_[k] * g[k + 1];
+            g[k] = tmp;
+            s.err[r] = std::abs(g[k + 1]) / s.bnorm[r];
+            if (breakdown || s.err[r] < s.sp.tol_rel || s.err[r] < s.sp.tol_abs) s.live[r] = 0;
         }
-        // x_r += V y_r with y_r from the triangular Hessenberg of each right-hand side
-        int kmax = 0;
-        for (int r = 0; r < R; ++r) kmax = std::max(kmax, kused[r]);
-        yv.assign((size_t)std::max(kmax, 1) * R, cplx(0, 0));
-        for (int r = 0; r < R; ++r) {
-            const int mm = kused[r];
-            const cplx* g = &sv[(size_t)r * (m + 1)];
-            std::vector<cplx> y(mm);
-            for (int i = mm - 1; i >= 0; --i) {
-                y[i] = g[i];
-                for (int j = i + 1; j < mm; ++j) y[i] -= Hm(r, i, j) * y[j];
-                y[i] = (std::abs(Hm(r, i, i)) > eps) ? y[i] / Hm(r, i, i) : cplx(0, 0);
-            }
-            for (int i = 0; i < mm; ++i) yv[(size_t)i * R + r] = y[i];
+        bs_put_scale(inv);
+        launch_scale_copy_b(st, s.n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * s.nR);
+        ++s.k;
+    }
+}
+
+// End of a restart cycle: x_r += Z y_r (V y_r without preconditioner), the
+// true FP64 residuals, the preconditioner fallback, the reference's
+// divergence guard and (single operator only) the compaction of the block.
+void Engine::bs_cycle_end(BlockState& s, bool compaction) {
+    const int R = s.R, m = s.m, n = s.n;
+    const double eps = 1e-16;
+    const mlfma_solve_params& sp = s.sp;
+    auto Hm = [&](int r, int i, int j) -> cplx& { return s.H[((size_t)r * m + j) * (m + 1) + i]; };
+    // x_r += V y_r with y_r from the triangular Hessenberg of each right-hand side
+    int kmax = 0;
+    for (int r = 0; r < R; ++r) kmax = std::max(kmax, s.kused[r]);
+    std::vector<cplx> yv((size_t)std::max(kmax, 1) * R, cplx(0, 0));
+    for (int r = 0; r < R; ++r) {
+        const int mm = s.kused[r];
+        const cplx* g = &s.sv[(size_t)r * (m + 1)];
+        std::vector<cplx> y(mm);
+        for (int i = mm - 1; i >= 0; --i) {
+            y[i] = g[i];
+            for (int j = i + 1; j < mm; ++j) y[i] -= Hm(r, i, j) * y[j];
+            y[i] = (std::abs(Hm(r, i, i)) > eps) ? y[i] / Hm(r, i, i) : cplx(0, 0);
         }
-        if (kmax > 0) {
-            put(yv);
-            if (pc) {   // x += Z y with the kept z_k = M^-1 v_k
-                launch_add_combination_b(st, n, R, kmax, d_bZf_, nR, d_bsmall_, d_X);
-            } else {
-                launch_add_combination_b(st, n, R, kmax, d_bV_, nR, d_bsmall_, d_X);
-            }
-        }
-        const std::vector<double> err_prev = err_cycle;   // true residuals at the start of this cycle
-        true_residual(rnorm);
-        bool progress = false;
-        for (int r = 0; r < R; ++r) {
-            err[r] = rnorm[r] / bnorm[r];
-            if (!std::isfinite(err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
-            if (err[r] < sp.tol_rel || err[r] < sp.tol_abs) done[r] = 1;
-            if (!done[r] && err[r] <= 0.9 * err_prev[r]) progress = true;
-        }
-        if (P.verbose) {
-            int nd = 0;
-            double emax = 0;
-            for (int r = 0; r < R; ++r) nd += done[r] ? 1 : 0, emax = std::max(emax, err[r]);
-            std::fprintf(stderr, "[mlfma] batched gmres cycle: %d of %d converged, max true residual %.3e\n", nd, R,
-                         emax);
-        }
-        // fallback as in solve(): a stalling preconditioner is replaced by the
-        // next plainer one (polynomial -> block-diagonal -> none)
-        if (pc && !all_done() && !progress) {
-            pkind = pkind >= 2 ? 1 : 0;
-            if (pkind == 1) precond_setup();
-            pc = pkind != 0;
-            ++stats_fallbacks_;
-            if (P.verbose)
-                std::fprintf(stderr, "[mlfma] preconditioner stalled: continuing with %s\n",
-                             pc ? "the block-diagonal one" : "none");
-        }
-        err_cycle = err;
-        bool stalled = true;   // Code/GMRES.h:208-214, all right-hand sides
-        for (int r = 0; r < R; ++r)
-            if (!done[r] && !(err[r] > 1000.0 * sp.tol_rel)) stalled = false;
-        if (stalled && cyc > 8) break;
-        // compaction: once a quarter of the block has converged, the remaining
-        // right-hand sides continue as a narrower block (same recurrences; each
-        // restarts from its own current iterate), so no matvec work is spent on
-        // converged columns
-        std::vector<int> idx;
-        for (int r = 0; r < R; ++r)
-            if (!done[r]) idx.push_back(r);
-        const int Rc = (int)idx.size();
-        if (Rc > 0 && 4 * Rc <= 3 * R && total < sp.itr_max) {
-            int* d_idx = upload(idx);
-            double2* Bc = dalloc<double2>((size_t)n * Rc);
-            double2* Xc = dalloc<double2>((size_t)n * Rc);
-            launch_gather_cols(st, n, R, Rc, d_idx, d_B, Bc);
-            launch_gather_cols(st, n, R, Rc, d_idx, d_X, Xc);
-            mlfma_solve_params sp2 = sp;
-            sp2.itr_max = sp.itr_max - total;
-            sp2.precond = pkind;   // a preconditioner replaced above stays replaced
-            const int fb = stats_fallbacks_;
-            std::vector<mlfma_solve_stats> st2(Rc);
-            if (P.verbose) std::fprintf(stderr, "[mlfma] batched gmres: continuing with %d of %d right-hand sides\n", Rc, R);
-            solve_block(sp2, Bc, Xc, Rc, st2.data());
-            launch_scatter_cols(st, n, R, Rc, d_idx, Xc, d_X);
-            MLFMA_HIP(hipStreamSynchronize(st));
-            dfree(Bc);
-            dfree(Xc);
-            dfree(d_idx);
-            stats_fallbacks_ = fb + st2[0].precond_fallbacks;
-            for (int j = 0; j < Rc; ++j) {
-                const int r = idx[j];
-                iters[r] += st2[j].iterations;
-                done[r] = st2[j].converged ? 1 : 0;
-                err[r] = st2[j].rel_residual;
-            }
-            matvecs += st2[0].matvecs * Rc;
-            fp64_matvecs += st2[0].fp64_matvecs * Rc;
-            break;
+        for (int i = 0; i < mm; ++i) yv[(size_t)i * R + r] = y[i];
+    }
+    if (kmax > 0) {
+        bs_put(yv, d_bsmall_);
+        if (s.pc) {   // x += Z y with the kept z_k = M^-1 v_k
+            launch_add_combination_b(st, n, R, kmax, d_bZf_, s.nR, d_bsmall_, s.X);
+        } else {
+            launch_add_combination_b(st, n, R, kmax, d_bV_, s.nR, d_bsmall_, s.X);
         }
     }
-    const double secs = wall() - t0;
+    const std::vector<double> err_prev = s.err_cycle;   // true residuals at the start of this cycle
+    bs_true_residual(s, s.rnorm);
+    bool progress = false;
     for (int r = 0; r < R; ++r) {
-        stats[r].iterations = iters[r];
-        stats[r].matvecs = matvecs / R;
-        stats[r].fp64_matvecs = fp64_matvecs / R;
-        stats[r].converged = done[r] ? 1 : 0;
-        stats[r].rel_residual = err[r];
+        s.err[r] = s.rnorm[r] / s.bnorm[r];
+        if (!std::isfinite(s.err[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite residual");
+        if (s.err[r] < sp.tol_rel || s.err[r] < sp.tol_abs) s.done[r] = 1;
+        if (!s.done[r] && s.err[r] <= 0.9 * err_prev[r]) progress = true;
+    }
+    const bool all_done = std::all_of(s.done.begin(), s.done.end(), [](char c) { return c != 0; });
+    if (P.verbose) {
+        int nd = 0;
+        double emax = 0;
+        for (int r = 0; r < R; ++r) nd += s.done[r] ? 1 : 0, emax = std::max(emax, s.err[r]);
+        std::fprintf(stderr, "[mlfma] batched gmres cycle: %d of %d converged, max true residual %.3e\n", nd, R, emax);
+    }
+    // fallback as in solve(): a stalling preconditioner is replaced by the
+    // next plainer one (polynomial -> block-diagonal -> none)
+    if (s.pc && !all_done && !progress) {
+        s.pkind = s.pkind >= 2 ? 1 : 0;
+        if (s.pkind == 1) precond_setup();
+        s.pc = s.pkind != 0;
+        ++s.fallbacks;
+        if (P.verbose)
+            std::fprintf(stderr, "[mlfma] preconditioner stalled: continuing with %s\n",
+                         s.pc ? "the block-diagonal one" : "none");
+    }
+    s.err_cycle = s.err;
+    bool stalled = true;   // Code/GMRES.h:208-214, all right-hand sides
+    for (int r = 0; r < R; ++r)
+        if (!s.done[r] && !(s.err[r] > 1000.0 * sp.tol_rel)) stalled = false;
+    if (stalled && s.cyc > 8) {
+        s.stopped = true;
+        return;
+    }
+    if (!compaction) return;
+    // compaction: once a quarter of the block has converged, the remaining
+    // right-hand sides continue as a narrower block (same recurrences; each
+    // restarts from its own current iterate), so no matvec work is spent on
+    // converged columns
+    std::vector<int> idx;
+    for (int r = 0; r < R; ++r)
+        if (!s.done[r]) idx.push_back(r);
+    const int Rc = (int)idx.size();
+    if (Rc > 0 && 4 * Rc <= 3 * R && s.total < sp.itr_max) {
+        int* d_idx = upload(idx);
+        double2* Bc = dalloc<double2>((size_t)n * Rc);
+        double2* Xc = dalloc<double2>((size_t)n * Rc);
+        launch_gather_cols(st, n, R, Rc, d_idx, s.B, Bc);
+        launch_gather_cols(st, n, R, Rc, d_idx, s.X, Xc);
+        mlfma_solve_params sp2 = sp;
+        sp2.itr_max = sp.itr_max - s.total;
+        sp2.precond = s.pkind;   // a preconditioner replaced above stays replaced
+        std::vector<mlfma_solve_stats> st2(Rc);
+        if (P.verbose) std::fprintf(stderr, "[mlfma] batched gmres: continuing with %d of %d right-hand sides\n", Rc, R);
+        solve_block(sp2, Bc, Xc, Rc, st2.data());
+        launch_scatter_cols(st, n, R, Rc, d_idx, Xc, s.X);
+        MLFMA_HIP(hipStreamSynchronize(st));
+        dfree(Bc);
+        dfree(Xc);
+        dfree(d_idx);
+        s.fallbacks += st2[0].precond_fallbacks;
+        for (int j = 0; j < Rc; ++j) {
+            const int r = idx[j];
+            s.iters[r] += st2[j].iterations;
+            s.done[r] = st2[j].converged ? 1 : 0;
+            s.err[r] = st2[j].rel_residual;
+        }
+        s.matvecs += st2[0].matvecs * Rc;
+        s.fp64_matvecs += st2[0].fp64_matvecs * Rc;
+        s.stopped = true;
+    }
+}
+
+void Engine::bs_stats(const BlockState& s, mlfma_solve_stats* stats) const {
+    const double secs = wall() - s.t0;
+    for (int r = 0; r < s.R; ++r) {
+        stats[r].iterations = s.iters[r];
+        stats[r].matvecs = s.matvecs / s.R;
+        stats[r].fp64_matvecs = s.fp64_matvecs / s.R;
+        stats[r].converged = s.done[r] ? 1 : 0;
+        stats[r].rel_residual = s.err[r];
         stats[r].seconds = secs;
-        stats[r].precond_fallbacks = stats_fallbacks_;
+        stats[r].precond_fallbacks = s.fallbacks;
     }
-    stats_fallbacks_ = 0;
+}
+
+void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R,
+                         mlfma_solve_stats* stats) {
+    BlockState s;
+    bs_start(s, sp, d_B, d_X, R);
+    while (bs_active(s)) {
+        bs_cycle_begin(s);
+        while (bs_step_open(s)) {
+            const bool recorded = use_graphs_ && !prof_on_;
+            bs_pre(s, recorded);
+            if (recorded) run_graph(GraphKey{bs_graph_kind(s), R, apply_prec}, [&] { bs_body(s); });
+            bs_post(s, recorded);
+            for (int phase = 0; phase < 4; ++phase) {
+                bs_orth(s, phase);
+                if (phase < 3) MLFMA_HIP(hipStreamSynchronize(st));
+            }
+        }
+        bs_cycle_end(s, true);
+    }
+    bs_stats(s, stats);
 }
 
 // Near-field block-diagonal preconditioner: exact inverse of every leaf's self

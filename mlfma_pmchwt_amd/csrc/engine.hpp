@@ -69,6 +69,26 @@ struct WaveDev {
     double2* E64 = nullptr;              // FP32 and FP64 far field
 };
 
+// Host state of one batched GMRES (lockstep over the R right-hand sides of
+// one operator), advanced phase by phase by Engine::solve_block and by the
+// lockstep multi-object solve (solve_multi).
+struct BlockState {
+    mlfma_solve_params sp{};
+    const double2* B = nullptr;
+    double2* X = nullptr;
+    int R = 0, n = 0, m = 1;
+    size_t nR = 0;
+    int pkind = 0;        // 0 none, 1 block-diagonal, 2 polynomial, 3 inner near-field solve
+    bool pc = false;
+    int matvecs = 0, fp64_matvecs = 0, total = 0, cyc = 0, k = 0, fallbacks = 0;
+    bool stopped = false;   // the divergence guard fired, or the block was handed on narrower
+    double t0 = 0;
+    std::vector<int> iters, kused;
+    std::vector<double> bnorm, rnorm, err, err_cycle;
+    std::vector<char> done, live;
+    std::vector<cplx> H, cs, sn, sv, h1, h2;
+};
+
 class Engine {
 public:
     explicit Engine(const mlfma_params& p);
@@ -91,6 +111,28 @@ public:
     void solve_device(const mlfma_solve_params& sp, const double2* d_b, double2* d_x, int R,
                       mlfma_solve_stats* stats);
     void ensure_block(int R);
+    // phases of the batched GMRES (the caller synchronises st between the
+    // orthogonalisation phases 0..3)
+    void bs_start(BlockState& s, const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R);
+    bool bs_active(const BlockState& s) const;
+    void bs_cycle_begin(BlockState& s);
+    bool bs_step_open(const BlockState& s) const;
+    int bs_graph_kind(const BlockState& s) const;
+    void bs_pre(BlockState& s, bool recorded);
+    void bs_body(BlockState& s);
+    void bs_post(BlockState& s, bool recorded);
+    void bs_orth(BlockState& s, int phase);
+    void bs_cycle_end(BlockState& s, bool compaction);
+    void bs_stats(const BlockState& s, mlfma_solve_stats* stats) const;
+    // column-major device blocks (vector j at b + j * n_unknowns) to and from
+    // the RHS-fastest blocks the batched solver works on
+    void block_load(const double2* d_b, const double2* d_x, int R);
+    void block_store(double2* d_x, int R);
+    const double2* block_b() const { return d_bx_; }
+    double2* block_x() { return d_by_; }
+    // a capture of several operators' matvecs into one graph (solve_multi)
+    void set_capturing(bool on) { capturing_ = on; }
+    bool graphs_on() const { return use_graphs_ && !prof_on_; }
     // hipGraph replay of batched matvecs; launch-configuration autotuner
     void set_graphs(bool on);
     void autotune(int R, int prec, double* out);
@@ -245,6 +287,19 @@ private:
     double2* d_bG_ = nullptr;   // fixed input of recorded Arnoldi matvecs
     int bV_cols_ = 0, bV_R_ = 0;
     int stats_fallbacks_ = 0;
+    // pinned host staging of the batched solver's reductions and coefficients
+    double2* h_get_ = nullptr;
+    double2* h_put_ = nullptr;
+    size_t pin_cap_ = 0;
+    bool put_pending_ = false;   // a host -> device copy from h_put_ may not have run yet
+    void ensure_pinned(size_t count);
+    double2* put_buffer();
+    void bs_put(const std::vector<cplx>& v, double2* dst);
+    void bs_put_scale(const std::vector<double>& a);
+    void bs_issue_norms(BlockState& s, const double2* v);
+    void bs_read_norms(const BlockState& s, std::vector<double>& out) const;
+    void bs_issue_dots(BlockState& s);
+    void bs_true_residual(BlockState& s, std::vector<double>& rn);
     // profiling events: per wave 3 groups x 2, near 2, far 2
     bool prof_on_ = false;
     hipEvent_t ev_[18] = {};   // 16..17: the all-RHS near product of a batched matvec
@@ -275,5 +330,13 @@ private:
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };
+
+// Batched GMRES of several operators in lockstep, one recorded graph per
+// Arnoldi step over all their matvecs (multi.cpp).  B[k], X[k]: RHS-fastest
+// device blocks of R[k] columns (block_b / block_x after block_load); stats[k]
+// has R[k] entries.
+void solve_multi(const std::vector<Engine*>& es, const mlfma_solve_params& sp, const std::vector<const double2*>& B,
+                 const std::vector<double2*>& X, const std::vector<int>& R,
+                 const std::vector<mlfma_solve_stats*>& stats);
 
 }  // namespace mlfma
