@@ -488,46 +488,87 @@ def pec_cfie_matvec(args):
 
 
 def multi_object(args):
-    """Several objects of uneven size in one batch run (the reference's batch
-    configuration format): eight PMCHWT spheres of 3k-35k RWG, 6 incidences x 2
-    polarisations each, GMRES(50) to 1e-4 (the reference drivers' settings) in
-    FP32 mode, first one object after another, then `--concurrency` objects at
-    once on that many host threads.  The results must not depend on it."""
-    import tempfile
+    """C4: the 16 seeded objects of uneven size (ellipsoids, cubes, cylinders, eps_r U[1.5, 6],
+    mesh edge lambda/10, 2.6k-29k RWG), 8 incidences each, PMCHWT, FP32 far field with
+    split-pair BF16 near field, GMRES(50) to a true FP64 residual of 1e-3.  Three ways over the
+    same operators: one object after another (batched solve per object), `--concurrency` host
+    threads taking objects from a queue (each object on its own stream), and all 16 in one
+    lockstep solve whose every Arnoldi step is ONE recorded graph with a branch per object
+    (mlfma_solve_multi).  The solutions must agree."""
+    import queue
+    import threading
     import numpy as np
-    from mlfma_pmchwt_amd import api, geometry
-    objs = [(10, 0.3), (12, 0.35), (14, 0.4), (16, 0.45), (18, 0.5), (20, 0.55), (22, 0.6), (24, 0.65)]
-    tmp = tempfile.mkdtemp(prefix="mlfma_objects_")
-    paths = []
-    for k, (n, r) in enumerate(objs):
-        p = os.path.join(tmp, f"object{k}.nas")
-        geometry.write_nas(p, geometry.geodesic_sphere(n, r))
-        paths.append(p)
-    cfg = (f"nas_files = {'; '.join(paths)}\nfrequencies = 299792458\nincident_theta_range = 0:150:30\n"
-           "material_list = 4.0,-0.1\nsca_theta = 0:180\nsca_phi = 0:0\nstep = 2\nintegral_equation = 2\n")
-    sp = api.SolveParams(api.GMRES, 2000, 50, 1e-4, 0.0, 0)
-    runs = {}
-    for conc in (1, args.concurrency):
-        b = api.Batch(text=cfg)
-        t0 = time.perf_counter()
-        rc = b.run(sp, concurrency=conc, far_precision=api.FP32, near_precision=api.FP32)
-        dt = time.perf_counter() - t0
-        res = [[b.result(i, pol) for pol in (0, 1)] for i in range(b.n_tasks)]
-        b.close()
-        its = [r[2]["iterations"] for t in res for r in t]
-        log(f"concurrency {conc}: {dt:.2f} s, status {rc}, iterations {min(its)}..{max(its)}")
-        runs[conc] = (dt, res)
-    (t1, r1), (tk, rk) = runs[1], runs[args.concurrency]
-    same_its = all(a[2]["iterations"] == c[2]["iterations"] for ta, tc in zip(r1, rk) for a, c in zip(ta, tc))
-    max_db = max(float(np.max(np.abs(a[1] - c[1]))) for ta, tc in zip(r1, rk) for a, c in zip(ta, tc))
+    from mlfma_pmchwt_amd import api, configs
+    objs = configs.c4_objects()
+    inc = configs.c4_incidences()
+    t0 = time.perf_counter()
+    ops, Bs = [], []
+    for o in objs:
+        op = api.Operator(o["mesh"], integral_equation=api.PMCHWT, eps_r_re=o["eps_r"], far_precision=api.FP32,
+                          near_precision=api.BF16, max_rhs=len(inc))
+        op.set_apply_precision(api.FP32)
+        op.set_graphs(True)
+        ops.append(op)
+        Bs.append(op.excitation_batch([a[0] for a in inc], [a[1] for a in inc], [a[2] for a in inc]))
+    setup_s = time.perf_counter() - t0
+    log(f"C4: {len(ops)} operators ({min(op.n for op in ops)}-{max(op.n for op in ops)} unknowns, "
+        f"{min(op.max_level for op in ops) + 1}-{max(op.max_level for op in ops) + 1} levels) in {setup_s:.1f} s")
+    kw = dict(restart=50, tol_rel=1e-3, itr_max=5000)
+    # warm-up: record every object's graphs once
+    for op, B in zip(ops, Bs):
+        op.solve_batch(B, itr_max=2, restart=50, tol_rel=1e-3)
+    t0 = time.perf_counter()
+    seq = [op.solve_batch(B, **kw) for op, B in zip(ops, Bs)]
+    seq_s = time.perf_counter() - t0
+    log(f"one after another: {seq_s:.2f} s")
+    q = queue.Queue()
+    for k in range(len(ops)):
+        q.put(k)
+    thr = [None] * len(ops)
+
+    def worker():
+        while True:
+            try:
+                k = q.get_nowait()
+            except queue.Empty:
+                return
+            thr[k] = ops[k].solve_batch(Bs[k], **kw)
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=worker) for _ in range(args.concurrency)]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    thr_s = time.perf_counter() - t0
+    log(f"{args.concurrency} host threads: {thr_s:.2f} s")
+    api.solve_multi(ops, Bs, itr_max=2, **{k: v for k, v in kw.items() if k != "itr_max"})
+    t0 = time.perf_counter()
+    Xm, stm = api.solve_multi(ops, Bs, **kw)
+    multi_s = time.perf_counter() - t0
+    log(f"one lockstep solve (a graph per Arnoldi step over all objects): {multi_s:.2f} s")
+    same_its = all([s["iterations"] for s in a[1]] == [s["iterations"] for s in b]
+                   for a, b in zip(seq, stm))
+    max_diff = max(float(np.linalg.norm(xm - a[0]) / np.linalg.norm(a[0])) for xm, a in zip(Xm, seq))
+    thr_diff = max(float(np.linalg.norm(t[0] - a[0]) / np.linalg.norm(a[0])) for t, a in zip(thr, seq))
+    conv = all(s["converged"] for st in stm for s in st)
+    its = [max(s["iterations"] for s in st) for st in stm]
     print(json.dumps({
-        "metric": "multi-object batch wall s (8 PMCHWT spheres, 12 runs each)", "value": round(tk, 3), "unit": "s",
-        "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "f32",
-        "config": {"workload": "8 PMCHWT spheres eps_r=4-0.1j, geodesic n=10..24 (3,000-17,280 RWG), 6 theta x 2 "
-                               "polarisations each, GMRES(50) to 1e-4, FP32 mode",
-                   "objects": [{"geodesic_n": n, "radius_lambda": r, "n_rwg": 30 * n * n} for n, r in objs]},
-        "sequential_s": round(t1, 3), "concurrency": args.concurrency, "concurrent_s": round(tk, 3),
-        "speedup": round(t1 / tk, 3), "same_iterations": same_its, "max_rcs_diff_db": max_db}))
+        "metric": "C4 solve wall s (16 uneven objects x 8 incidences, to 1e-3)", "value": round(multi_s, 3),
+        "unit": "s", "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "f32+bf16",
+        "config": {"workload": "C4: 16 seeded objects (ellipsoids, cubes, cylinders; default_rng(4)), eps_r U[1.5, 6], "
+                               "edge lambda/10, 8 incidences each, PMCHWT, GMRES(50) to 1e-3, FP32 far field, split-pair "
+                               "BF16 near field",
+                   "objects": [{"kind": o["kind"], "eps_r": round(o["eps_r"], 3), "unknowns": op.n,
+                                "levels": op.max_level + 1, "max_iterations": it,
+                                "sequential_solve_s": round(sq[1][0]["seconds"], 3)}
+                               for o, op, it, sq in zip(objs, ops, its, seq)]},
+        "setup_s": round(setup_s, 2), "sequential_s": round(seq_s, 3), "threads": args.concurrency,
+        "threads_s": round(thr_s, 3), "lockstep_graph_s": round(multi_s, 3),
+        "speedup_vs_sequential": round(seq_s / multi_s, 2), "speedup_vs_threads": round(thr_s / multi_s, 2),
+        "all_converged": conv, "same_iterations_as_sequential": same_its,
+        "max_solution_diff_lockstep_vs_sequential": max_diff, "max_solution_diff_threads_vs_sequential": thr_diff}))
+    for op in ops:
+        op.close()
 
 
 def main():

@@ -147,7 +147,9 @@ int mlfma_set_graphs(mlfma_handle* h, int on);
  * choice for this handle.  out4 = {rhs per pass, choice bits (bit 0 near on
  * the matrix cores, bit 1 disaggregation on the matrix cores, bit 2 cluster
  * translation, bit 3 cluster translation with 12 workspace columns per thread
- * (FP32)), ms per batched matvec of the choice, ms of the slowest
+ * (FP32), bit 4 cluster translation over every child of a parent (up to 8)
+ * with the members' translation samples staged in LDS (FP32; takes precedence
+ * over bit 3)), ms per batched matvec of the choice, ms of the slowest
  * configuration timed}.  No reference counterpart. */
 int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
 
@@ -158,7 +160,9 @@ int mlfma_autotune(mlfma_handle* h, int nrhs, int precision, double* out4);
  * cores, bit 1 disaggregation on the matrix cores, bit 2 translation by
  * sibling clusters (up to 4 children of one parent share each source box,
  * read once), bit 3 12 instead of 6 workspace columns per thread in the FP32
- * cluster translation.  No reference counterpart. */
+ * cluster translation, bit 4 FP32 cluster translation over every child of a
+ * parent with LDS-staged translation samples (takes precedence over bit 3).
+ * No reference counterpart. */
 int mlfma_set_tuning(mlfma_handle* h, int precision, int rhs_per_pass, int bits);
 
 /* Krylov solve on the device.  Replaces MGMRES::mgmres (Code/GMRES.h:65-217)

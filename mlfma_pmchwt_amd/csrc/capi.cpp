@@ -181,8 +181,8 @@ int mlfma_set_tuning(mlfma_handle* h, int precision, int rhs_per_pass, int bits)
     return guarded(h, [&] {
         if (precision != MLFMA_FP64 && precision != MLFMA_FP32) throw mlfma::Error(MLFMA_ERR_ARG, "bad precision");
         if (!(rhs_per_pass == 0 || rhs_per_pass == 1 || rhs_per_pass == 2 || rhs_per_pass == 4 || rhs_per_pass == 8) ||
-            bits < 0 || bits > 15)
-            throw mlfma::Error(MLFMA_ERR_ARG, "rhs_per_pass must be 0, 1, 2, 4 or 8 and bits in 0..15");
+            bits < 0 || bits > 31)
+            throw mlfma::Error(MLFMA_ERR_ARG, "rhs_per_pass must be 0, 1, 2, 4 or 8 and bits in 0..31");
         h->eng->set_tuning(precision, rhs_per_pass, bits);
     });
 }

@@ -197,32 +197,47 @@ void Engine::build_device_tree() {
         for (int i = 0; i < v.n; ++i) oct[i] = (int)(v.code[i] & 7);
         d.oct = upload(oct);
         // sibling clusters: boxes are Morton-sorted, so the children of one
-        // parent are consecutive
-        std::vector<int> cl_child, cl_ptr(1, 0), cl_src, cl_vec;
-        for (int b0 = 0; b0 < v.n;) {
-            int b1 = b0;
-            while (b1 < v.n && v.parent[b1] == v.parent[b0] && b1 - b0 < 4) ++b1;
-            std::vector<int> srcs;
-            for (int b = b0; b < b1; ++b)
-                for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) srcs.push_back(v.dr_idx[j]);
-            std::sort(srcs.begin(), srcs.end());
-            srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
-            const size_t base = cl_src.size();
-            cl_src.insert(cl_src.end(), srcs.begin(), srcs.end());
-            // a member without a source reads the zero row after the last
-            // translation vector; absent members keep -1
-            cl_vec.resize(cl_src.size() * 4, -1);
-            for (size_t u = base; u < cl_src.size(); ++u)
-                for (int b = b0; b < b1; ++b) cl_vec[u * 4 + (b - b0)] = (int)v.drvec.size();
-            for (int b = b0; b < b1; ++b)
-                for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) {
-                    const size_t u = base + (std::lower_bound(srcs.begin(), srcs.end(), v.dr_idx[j]) - srcs.begin());
-                    cl_vec[u * 4 + (b - b0)] = v.dr_vec[j];
-                }
-            for (int k = 0; k < 4; ++k) cl_child.push_back(b0 + k < b1 ? b0 + k : -1);
-            cl_ptr.push_back((int)cl_src.size());
-            d.cl_pairs += v.dr_ptr[b1] - v.dr_ptr[b0];
-            b0 = b1;
+        // parent are consecutive; slots clusters of up to `slots` children
+        auto clusters = [&](int slots, std::vector<int>& cl_child, std::vector<int>& cl_ptr, std::vector<int>& cl_src,
+                            std::vector<int>& cl_vec, long long* pairs) {
+            cl_ptr.assign(1, 0);
+            for (int b0 = 0; b0 < v.n;) {
+                int b1 = b0;
+                while (b1 < v.n && v.parent[b1] == v.parent[b0] && b1 - b0 < slots) ++b1;
+                std::vector<int> srcs;
+                for (int b = b0; b < b1; ++b)
+                    for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) srcs.push_back(v.dr_idx[j]);
+                std::sort(srcs.begin(), srcs.end());
+                srcs.erase(std::unique(srcs.begin(), srcs.end()), srcs.end());
+                const size_t base = cl_src.size();
+                cl_src.insert(cl_src.end(), srcs.begin(), srcs.end());
+                // a member without a source reads the zero row after the last
+                // translation vector; absent members keep -1
+                cl_vec.resize(cl_src.size() * slots, -1);
+                for (size_t u = base; u < cl_src.size(); ++u)
+                    for (int b = b0; b < b1; ++b) cl_vec[u * slots + (b - b0)] = (int)v.drvec.size();
+                for (int b = b0; b < b1; ++b)
+                    for (int j = v.dr_ptr[b]; j < v.dr_ptr[b + 1]; ++j) {
+                        const size_t u =
+                            base + (std::lower_bound(srcs.begin(), srcs.end(), v.dr_idx[j]) - srcs.begin());
+                        cl_vec[u * slots + (b - b0)] = v.dr_vec[j];
+                    }
+                for (int k = 0; k < slots; ++k) cl_child.push_back(b0 + k < b1 ? b0 + k : -1);
+                cl_ptr.push_back((int)cl_src.size());
+                if (pairs) *pairs += v.dr_ptr[b1] - v.dr_ptr[b0];
+                b0 = b1;
+            }
+        };
+        std::vector<int> cl_child, cl_ptr, cl_src, cl_vec;
+        clusters(4, cl_child, cl_ptr, cl_src, cl_vec, &d.cl_pairs);
+        {
+            std::vector<int> c8, p8, s8, v8;
+            clusters(8, c8, p8, s8, v8, nullptr);
+            d.ncl8 = (int)p8.size() - 1;
+            d.cl8_child = upload(c8);
+            d.cl8_ptr = upload(p8);
+            d.cl8_src = upload(s8);
+            d.cl8_vec = upload(v8);
         }
         d.ncl = (int)cl_ptr.size() - 1;
         d.cl_entries = (long long)cl_src.size();
@@ -513,8 +528,12 @@ void Engine::translate(int w, int l, int nc) {
     FarDev<R>& F = far<R>(w);
     const bool up = l > 2;
     const LevelDev& v = lv_[l];
-    if (m2l_cluster_)
-        launch_m2l_cl<R>(st, m2l_wide_, nc, v.ncl, D.K[l], v.cl_child, v.cl_ptr, v.cl_src, v.cl_vec, F.TFw[l], F.Sm[l],
+    if (m2l_cluster_ && m2l_variant_ == 2 && std::is_same<R, float>::value && (nc == 4 || nc == 8 || nc == 16))
+        launch_m2l_cls<R>(st, nc, v.ncl8, D.K[l], v.cl8_child, v.cl8_ptr, v.cl8_src, v.cl8_vec, F.TFw[l], F.Sm[l],
+                          up ? D.K[l - 1] : 0, v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l],
+                          up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
+    else if (m2l_cluster_)
+        launch_m2l_cl<R>(st, m2l_variant_, nc, v.ncl, D.K[l], v.cl_child, v.cl_ptr, v.cl_src, v.cl_vec, F.TFw[l], F.Sm[l],
                          up ? D.K[l - 1] : 0, v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l],
                          up ? F.Bm[l - 1] : nullptr, F.Bm[l]);
     else
@@ -841,7 +860,7 @@ void Engine::autotune(int R, int prec, double* out) {
     launch_probe(st, (int)std::min<size_t>((size_t)nunk * R, 0x7fffffff), 99u, d_bx_);
     const int rbmax = f32 ? rb_max<float>() : rb_max<double>();
     const bool nm = R >= 2, dm = R >= 2 && (f32 ? wd_[0].E != nullptr : wd_[0].E64 != nullptr);
-    const int valid = (nm ? 1 : 0) | (dm ? 2 : 0) | 4 | (f32 ? 8 : 0);
+    const int valid = (nm ? 1 : 0) | (dm ? 2 : 0) | 4 | (f32 ? 16 : 0);
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
     MLFMA_HIP(hipEventCreate(&b));
@@ -850,7 +869,7 @@ void Engine::autotune(int R, int prec, double* out) {
         near_mfma_ = (m & 1) != 0;
         disagg_mfma_ = (m & 2) != 0;
         m2l_cluster_ = (m & 4) != 0;
-        m2l_wide_ = (m & 8) != 0;
+        m2l_variant_ = (m & 16) ? 2 : ((m & 8) ? 1 : 0);
         rb_tuned_[f32 ? 1 : 0] = rb;
         apply_block(d_bx_, d_by_, R, prec);   // warm-up
         MLFMA_HIP(hipEventRecord(a, st));
@@ -866,11 +885,17 @@ void Engine::autotune(int R, int prec, double* out) {
     };
     int best_rb = rbmax, best_m = valid;
     double best = time(best_rb, best_m);
-    for (int bit = 1; bit <= 8; bit <<= 1) {
+    for (int bit = 1; bit <= 4; bit <<= 1) {
         if (!(valid & bit)) continue;
         const double t = time(best_rb, best_m ^ bit);
         if (t < best) best = t, best_m ^= bit;
     }
+    if (f32 && (best_m & 4))   // the cluster translation's forms: two samples (start), twelve columns, six
+        for (int v : {8, 0}) {
+            const int m = (best_m & ~24) | v;
+            const double t = time(best_rb, m);
+            if (t < best) best = t, best_m = m;
+        }
     for (int rb = rbmax / 2; rb >= std::max(1, rbmax / 4); rb /= 2) {
         const double t = time(rb, best_m);
         if (t < best) best = t, best_rb = rb;
@@ -882,7 +907,7 @@ void Engine::autotune(int R, int prec, double* out) {
     near_mfma_ = (best_m & 1) != 0;
     disagg_mfma_ = (best_m & 2) != 0;
     m2l_cluster_ = (best_m & 4) != 0;
-    m2l_wide_ = (best_m & 8) != 0;
+    m2l_variant_ = (best_m & 16) ? 2 : ((best_m & 8) ? 1 : 0);
     clear_graphs();   // recorded sequences follow the tuned configuration
     if (out) out[0] = best_rb, out[1] = best_m, out[2] = best, out[3] = worst;
 }
@@ -894,7 +919,7 @@ void Engine::set_tuning(int prec, int rb, int bits) {
     near_mfma_ = (bits & 1) != 0;
     disagg_mfma_ = (bits & 2) != 0;
     m2l_cluster_ = (bits & 4) != 0;
-    m2l_wide_ = (bits & 8) != 0;
+    m2l_variant_ = (bits & 16) ? 2 : ((bits & 8) ? 1 : 0);
     clear_graphs();
 }
 
@@ -1260,12 +1285,6 @@ void Engine::bs_read_norms(const BlockState& s, std::vector<double>& out) const 
     for (int r = 0; r < s.R; ++r) out[r] = std::sqrt(h_get_[r].x);
 }
 
-void Engine::bs_issue_dots(BlockState& s) {   // <V_j, W>_r, j <= k
-    launch_multidot_b(st, s.n, s.R, s.k + 1, d_bV_, s.nR, d_bW_, d_bpart_, kRedBlocks);
-    launch_reduce_partials_b(st, s.k + 1, s.R, kRedBlocks, d_bpart_, d_bsmall_);
-    MLFMA_HIP(hipMemcpyAsync(h_get_, d_bsmall_, (size_t)(s.k + 1) * s.R * sizeof(double2), hipMemcpyDeviceToHost, st));
-}
-
 // the host -> device staging buffer, once the copy last issued from it has run
 double2* Engine::put_buffer() {
     if (put_pending_) MLFMA_HIP(hipStreamSynchronize(st));
@@ -1320,7 +1339,7 @@ void Engine::bs_start(BlockState& s, const mlfma_solve_params& sp, const double2
     // recorded matvec could reach it (a resize inside a capture is an error)
     ensure_xf(R);
     ensure_eall(R);
-    ensure_pinned((size_t)(m + 1) * R);
+    ensure_pinned((size_t)(2 * m + 3) * R);
     const size_t nR = s.nR;
     if (bV_cols_ < m + 1 || bV_R_ != R) {
         for (void* p : {(void*)d_bV_, (void*)d_bW_, (void*)d_bRr_, (void*)d_bpart_, (void*)d_bsmall_, (void*)d_bscale_,
@@ -1441,71 +1460,73 @@ void Engine::bs_post(BlockState& s, bool recorded) {
     s.matvecs += s.R;
 }
 
-// Classical Gram-Schmidt twice (CGS2) and the Givens update of step k:
-// phase 0 issues <V_j, W>; 1 subtracts it and issues the second pass; 2
-// subtracts that and issues ||W||; 3 updates every live right-hand side's
-// Hessenberg column and scales W into V_{k+1}.  The caller synchronises st
-// between phases.
+// Classical Gram-Schmidt twice (CGS2) and the Givens update of step k.
+// Phase 0 runs both passes on the device (each pass's coefficients <V_j, W>
+// stay in d_bsmall_ for the subtraction) and copies both passes'
+// coefficients and ||W|| to the host; after the caller has synchronised st,
+// phase 1 updates every live right-hand side's Hessenberg column and scales
+// W into V_{k+1}.
 void Engine::bs_orth(BlockState& s, int phase) {
     const int R = s.R, m = s.m, k = s.k;
     const double eps = 1e-16;
-    auto Hm = [&](int r, int i, int j) -> cplx& { return s.H[((size_t)r * m + j) * (m + 1) + i]; };
-    auto read_dots = [&](std::vector<cplx>& h) {
-        h.resize((size_t)(k + 1) * R);
-        for (size_t e = 0; e < h.size(); ++e) h[e] = cx(h_get_[e]);
-    };
+    const size_t nh = (size_t)(k + 1) * R;
     if (phase == 0) {
-        bs_issue_dots(s);
-    } else if (phase == 1 || phase == 2) {
-        std::vector<cplx>& h = phase == 1 ? s.h1 : s.h2;
-        read_dots(h);
-        bs_put(h, d_bsmall_);
-        launch_sub_combination_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bsmall_, d_bW_);
-        if (phase == 1) bs_issue_dots(s);
-        else bs_issue_norms(s, d_bW_);
-    } else {
-        std::vector<double> hn;
-        bs_read_norms(s, hn);
-        std::vector<double> inv(R, 0.0);
-        for (int r = 0; r < R; ++r) {
-            if (!s.live[r]) continue;
-            s.kused[r] = k + 1;
-            ++s.iters[r];
-            for (int j = 0; j <= k; ++j) Hm(r, j, k) = s.h1[(size_t)j * R + r] + s.h2[(size_t)j * R + r];
-            if (!std::isfinite(hn[r])) throw Error(MLFMA_ERR_NONFINITE, "non-finite Arnoldi vector");
-            Hm(r, k + 1, k) = hn[r];
-            const bool breakdown = hn[r] < eps * s.rnorm[r];
-            if (!breakdown) inv[r] = 1.0 / hn[r];
-            cplx* c_ = &s.cs[(size_t)r * m];
-            cplx* s_ = &s.sn[(size_t)r * m];
-            cplx* g = &s.sv[(size_t)r * (m + 1)];
-            // Code/GMRES.h:22-48, 144-152
-            for (int j = 0; j < k; ++j) {
-                cplx tmp = std::conj(c_[j]) * Hm(r, j, k) + std::conj(s_[j]) * Hm(r, j + 1, k);
-                Hm(r, j + 1, k) = -s_[j] * Hm(r, j, k) + c_[j] * Hm(r, j + 1, k);
-                Hm(r, j, k) = tmp;
-            }
-            cplx a = Hm(r, k, k), b = Hm(r, k + 1, k);
-            if (std::abs(b) < eps) {
-                c_[k] = 1.0, s_[k] = 0.0;
-            } else if (std::abs(a) < eps) {
-                c_[k] = 0.0, s_[k] = 1.0;
-            } else {
-                double rr = std::sqrt(std::real(a * std::conj(a) + b * std::conj(b)));
-                c_[k] = a / rr, s_[k] = b / rr;
-            }
-            Hm(r, k, k) = std::conj(c_[k]) * a + std::conj(s_[k]) * b;
-            Hm(r, k + 1, k) = 0.0;
-            cplx tmp = std::conj(c_[k]) * g[k] + std::conj(s_[k]) * g[k + 1];
-            g[k + 1] = -s_[k] * g[k] + c_[k] * g[k + 1];
-            g[k] = tmp;
-            s.err[r] = std::abs(g[k + 1]) / s.bnorm[r];
-            if (breakdown || s.err[r] < s.sp.tol_rel || s.err[r] < s.sp.tol_abs) s.live[r] = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            launch_multidot_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bW_, d_bpart_, kRedBlocks);
+            launch_reduce_partials_b(st, k + 1, R, kRedBlocks, d_bpart_, d_bsmall_);
+            MLFMA_HIP(hipMemcpyAsync(h_get_ + pass * nh, d_bsmall_, nh * sizeof(double2), hipMemcpyDeviceToHost, st));
+            launch_sub_combination_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bsmall_, d_bW_);
         }
-        bs_put_scale(inv);
-        launch_scale_copy_b(st, s.n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * s.nR);
-        ++s.k;
+        launch_norm2_partial_b(st, s.n, R, d_bW_, d_bpart_, kRedBlocks);
+        launch_reduce_partials_b(st, 1, R, kRedBlocks, d_bpart_, d_bsmall_);
+        MLFMA_HIP(hipMemcpyAsync(h_get_ + 2 * nh, d_bsmall_, R * sizeof(double2), hipMemcpyDeviceToHost, st));
+        return;
     }
+    put_pending_ = false;   // the caller synchronised st
+    auto Hm = [&](int r, int i, int j) -> cplx& { return s.H[((size_t)r * m + j) * (m + 1) + i]; };
+    const double2* h1 = h_get_;
+    const double2* h2 = h_get_ + nh;
+    const double2* hn2 = h_get_ + 2 * nh;
+    std::vector<double> inv(R, 0.0);
+    for (int r = 0; r < R; ++r) {
+        if (!s.live[r]) continue;
+        s.kused[r] = k + 1;
+        ++s.iters[r];
+        for (int j = 0; j <= k; ++j) Hm(r, j, k) = cx(h1[(size_t)j * R + r]) + cx(h2[(size_t)j * R + r]);
+        const double hn = std::sqrt(hn2[r].x);
+        if (!std::isfinite(hn)) throw Error(MLFMA_ERR_NONFINITE, "non-finite Arnoldi vector");
+        Hm(r, k + 1, k) = hn;
+        const bool breakdown = hn < eps * s.rnorm[r];
+        if (!breakdown) inv[r] = 1.0 / hn;
+        cplx* c_ = &s.cs[(size_t)r * m];
+        cplx* s_ = &s.sn[(size_t)r * m];
+        cplx* g = &s.sv[(size_t)r * (m + 1)];
+        // Code/GMRES.h:22-48, 144-152
+        for (int j = 0; j < k; ++j) {
+            cplx tmp = std::conj(c_[j]) * Hm(r, j, k) + std::conj(s_[j]) * Hm(r, j + 1, k);
+            Hm(r, j + 1, k) = -s_[j] * Hm(r, j, k) + c_[j] * Hm(r, j + 1, k);
+            Hm(r, j, k) = tmp;
+        }
+        cplx a = Hm(r, k, k), b = Hm(r, k + 1, k);
+        if (std::abs(b) < eps) {
+            c_[k] = 1.0, s_[k] = 0.0;
+        } else if (std::abs(a) < eps) {
+            c_[k] = 0.0, s_[k] = 1.0;
+        } else {
+            double rr = std::sqrt(std::real(a * std::conj(a) + b * std::conj(b)));
+            c_[k] = a / rr, s_[k] = b / rr;
+        }
+        Hm(r, k, k) = std::conj(c_[k]) * a + std::conj(s_[k]) * b;
+        Hm(r, k + 1, k) = 0.0;
+        cplx tmp = std::conj(c_[k]) * g[k] + std::conj(s_[k]) * g[k + 1];
+        g[k + 1] = -s_[k] * g[k] + c_[k] * g[k + 1];
+        g[k] = tmp;
+        s.err[r] = std::abs(g[k + 1]) / s.bnorm[r];
+        if (breakdown || s.err[r] < s.sp.tol_rel || s.err[r] < s.sp.tol_abs) s.live[r] = 0;
+    }
+    bs_put_scale(inv);
+    launch_scale_copy_b(st, s.n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * s.nR);
+    ++s.k;
 }
 
 // End of a restart cycle: x_r += Z y_r (V y_r without preconditioner), the
@@ -1613,6 +1634,50 @@ void Engine::bs_cycle_end(BlockState& s, bool compaction) {
     }
 }
 
+bool Engine::bs_narrow(BlockState& s, BlockState& child, NarrowRecord& rec) {
+    if (s.stopped) return false;
+    std::vector<int> idx;
+    for (int r = 0; r < s.R; ++r)
+        if (!s.done[r]) idx.push_back(r);
+    const int Rc = (int)idx.size();
+    if (!(Rc > 0 && 4 * Rc <= 3 * s.R && s.total < s.sp.itr_max)) return false;
+    const int n = s.n;
+    rec.idx = idx;
+    rec.d_idx = upload(idx);
+    rec.Bc = dalloc<double2>((size_t)n * Rc);
+    rec.Xc = dalloc<double2>((size_t)n * Rc);
+    launch_gather_cols(st, n, s.R, Rc, rec.d_idx, s.B, rec.Bc);
+    launch_gather_cols(st, n, s.R, Rc, rec.d_idx, s.X, rec.Xc);
+    mlfma_solve_params sp2 = s.sp;
+    sp2.itr_max = s.sp.itr_max - s.total;
+    sp2.precond = s.pkind;   // a preconditioner replaced before stays replaced
+    if (P.verbose) std::fprintf(stderr, "[mlfma] batched gmres: continuing with %d of %d right-hand sides\n", Rc, s.R);
+    bs_start(child, sp2, rec.Bc, rec.Xc, Rc);
+    s.stopped = true;
+    return true;
+}
+
+void Engine::bs_unnarrow(BlockState& s, const BlockState& child, NarrowRecord& rec) {
+    const int Rc = child.R;
+    std::vector<mlfma_solve_stats> st2(Rc);
+    bs_stats(child, st2.data());
+    launch_scatter_cols(st, s.n, s.R, Rc, rec.d_idx, rec.Xc, s.X);
+    MLFMA_HIP(hipStreamSynchronize(st));
+    s.fallbacks += st2[0].precond_fallbacks;
+    for (int j = 0; j < Rc; ++j) {
+        const int r = rec.idx[j];
+        s.iters[r] += st2[j].iterations;
+        s.done[r] = st2[j].converged ? 1 : 0;
+        s.err[r] = st2[j].rel_residual;
+    }
+    s.matvecs += st2[0].matvecs * Rc;
+    s.fp64_matvecs += st2[0].fp64_matvecs * Rc;
+    dfree(rec.Bc);
+    dfree(rec.Xc);
+    dfree(rec.d_idx);
+    rec = NarrowRecord();
+}
+
 void Engine::bs_stats(const BlockState& s, mlfma_solve_stats* stats) const {
     const double secs = wall() - s.t0;
     for (int r = 0; r < s.R; ++r) {
@@ -1637,10 +1702,9 @@ void Engine::solve_block(const mlfma_solve_params& sp, const double2* d_B, doubl
             bs_pre(s, recorded);
             if (recorded) run_graph(GraphKey{bs_graph_kind(s), R, apply_prec}, [&] { bs_body(s); });
             bs_post(s, recorded);
-            for (int phase = 0; phase < 4; ++phase) {
-                bs_orth(s, phase);
-                if (phase < 3) MLFMA_HIP(hipStreamSynchronize(st));
-            }
+            bs_orth(s, 0);
+            MLFMA_HIP(hipStreamSynchronize(st));
+            bs_orth(s, 1);
         }
         bs_cycle_end(s, true);
     }

@@ -47,6 +47,12 @@ struct LevelDev {
     int* cl_src = nullptr;
     int* cl_vec = nullptr;     // [entries][4]
     long long cl_pairs = 0, cl_entries = 0;   // sum of DR list lengths, sum of union sizes
+    // the same with every child of a parent in one cluster (up to 8 slots)
+    int ncl8 = 0;
+    int* cl8_child = nullptr;   // [ncl8][8]
+    int* cl8_ptr = nullptr;
+    int* cl8_src = nullptr;
+    int* cl8_vec = nullptr;     // [entries][8]
 };
 
 // Far-field tables and workspaces of one wavenumber in real type R.
@@ -86,7 +92,17 @@ struct BlockState {
     std::vector<int> iters, kused;
     std::vector<double> bnorm, rnorm, err, err_cycle;
     std::vector<char> done, live;
-    std::vector<cplx> H, cs, sn, sv, h1, h2;
+    std::vector<cplx> H, cs, sn, sv;
+};
+
+// A block narrowed to its unconverged columns (the compaction of the batched
+// solve, as a child state instead of a nested call): the columns' indices and
+// their own right-hand-side and solution blocks.
+struct NarrowRecord {
+    std::vector<int> idx;
+    int* d_idx = nullptr;
+    double2* Bc = nullptr;
+    double2* Xc = nullptr;
 };
 
 class Engine {
@@ -112,7 +128,7 @@ public:
                       mlfma_solve_stats* stats);
     void ensure_block(int R);
     // phases of the batched GMRES (the caller synchronises st between the
-    // orthogonalisation phases 0..3)
+    // orthogonalisation phases 0 and 1)
     void bs_start(BlockState& s, const mlfma_solve_params& sp, const double2* d_B, double2* d_X, int R);
     bool bs_active(const BlockState& s) const;
     void bs_cycle_begin(BlockState& s);
@@ -124,6 +140,11 @@ public:
     void bs_orth(BlockState& s, int phase);
     void bs_cycle_end(BlockState& s, bool compaction);
     void bs_stats(const BlockState& s, mlfma_solve_stats* stats) const;
+    // compaction without nesting: start `child` on the unconverged columns of a
+    // block whose cycle ended with a quarter or more converged (s stops), and
+    // later fold the child's solutions and counts back into s
+    bool bs_narrow(BlockState& s, BlockState& child, NarrowRecord& rec);
+    void bs_unnarrow(BlockState& s, const BlockState& child, NarrowRecord& rec);
     // column-major device blocks (vector j at b + j * n_unknowns) to and from
     // the RHS-fastest blocks the batched solver works on
     void block_load(const double2* d_b, const double2* d_x, int R);
@@ -298,7 +319,6 @@ private:
     void bs_put_scale(const std::vector<double>& a);
     void bs_issue_norms(BlockState& s, const double2* v);
     void bs_read_norms(const BlockState& s, std::vector<double>& out) const;
-    void bs_issue_dots(BlockState& s);
     void bs_true_residual(BlockState& s, std::vector<double>& rn);
     // profiling events: per wave 3 groups x 2, near 2, far 2
     bool prof_on_ = false;
@@ -326,7 +346,7 @@ private:
     bool near_mfma_ = true;     // batched near product on the matrix cores (FP32 copy or FP64 entries)
     bool disagg_mfma_ = true;   // batched disaggregation on the matrix cores
     bool m2l_cluster_ = true;   // translation by sibling clusters
-    bool m2l_wide_ = false;     // FP32 cluster translation with 12 workspace columns per thread
+    int m2l_variant_ = 0;       // FP32 cluster translation: 1 twelve columns per thread, 2 all-sibling clusters, LDS-staged samples
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;
 };

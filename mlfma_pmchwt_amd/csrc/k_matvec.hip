@@ -375,6 +375,194 @@ __device__ __forceinline__ void m2l_cl_body(int cl, int K, int q, int c0, const 
     }
 }
 
+// All-sibling clusters with LDS-staged translation samples (FP32): a cluster
+// is every child of one parent (NM <= 8 members; with up to 4 a parent of 5-8
+// children needs two clusters, which read the union's sources twice), a
+// thread owns one sample q and CPT = 6 columns.  The translation samples of
+// the members for kS sources at a time are staged in LDS by the whole
+// workgroup, so each is read from memory once per workgroup instead of once
+// per column-group wavefront, and a source costs a wavefront CPT vector loads
+// (its workspace values) and NM LDS reads.  The staging is double-buffered
+// (the next chunk's samples are loaded into registers during the current
+// chunk's products, one barrier per chunk); the last chunk is padded with
+// zero samples, and members without a source read the table's zero row, so
+// the product loop has no branches.  Per sample and member the sums run over
+// the same sources in the same order as the other cluster forms (plus exact
+// zeros), so the results are identical.
+constexpr int kClsS = 4;   // sources per staged chunk (8 spills registers)
+
+template <int NC, int G, int NM>
+__device__ __forceinline__ void m2l_cls_body(int cl, int K, int kt, int c0, float2 (*tfs)[kClsS][8][64],
+                                             const int* __restrict__ cl_child, const int* __restrict__ cl_ptr,
+                                             const int* __restrict__ cl_src, const int* __restrict__ cl_vec,
+                                             const float2* __restrict__ TFw, const float2* __restrict__ Sm, int Kp,
+                                             const int* __restrict__ parent, const int* __restrict__ oct,
+                                             const int* __restrict__ t_ptr, const int* __restrict__ t_kp,
+                                             const float* __restrict__ t_w, const float2* __restrict__ shdn,
+                                             const float2* __restrict__ Bmp, float2* __restrict__ Bm) {
+    using C = float2;
+    constexpr int CPT = NC * 3 / G;
+    constexpr int NTH = 64 * G;
+    constexpr int NE = kClsS * NM * 64, EPT = (NE + NTH - 1) / NTH;   // staged samples per chunk, per thread
+    const int lane = threadIdx.x & 63;
+    const int q = kt * 64 + lane, qc = min(q, K - 1);   // lanes past K compute on clamped data, store nothing
+    int child[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) child[k] = cl_child[cl * 8 + k];
+    C acc[NM][CPT];
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) acc[k][j] = C{0, 0};
+    const int u0 = cl_ptr[cl], u1 = cl_ptr[cl + 1];
+    const int nchunk = (u1 - u0 + kClsS - 1) / kClsS;
+    // staging: entry e = (s, k, l) of chunk c -> tfs[b][s][k][l]
+    C tv[EPT];
+    auto stage_load = [&](int c) {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int e = min((int)threadIdx.x + NTH * i, NE - 1);
+            const int s = e / (NM * 64), k = (e / 64) % NM, l = e % 64;
+            const int u = u0 + c * kClsS + s;
+            const int vid = cl_vec[(size_t)min(u, u1 - 1) * 8 + k];
+            const C v = TFw[(size_t)vid * K + min(kt * 64 + l, K - 1)];
+            tv[i] = u < u1 ? v : C{0, 0};   // the last chunk's padding applies exact zeros
+        }
+    };
+    auto stage_store = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int e = (int)threadIdx.x + NTH * i;
+            if (NE % NTH == 0 || e < NE) {
+                const int s = e / (NM * 64), k = (e / 64) % NM, l = e % 64;
+                tfs[b][s][k][l] = tv[i];
+            }
+        }
+    };
+    C sa[CPT], sb[CPT];
+    auto fetch = [&](C (&sv)[CPT], int u) {
+        const C* S = Sm + ((size_t)cl_src[min(u, u1 - 1)] * NC * 3 + c0) * K + qc;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) sv[j] = S[(size_t)j * K];
+    };
+    auto apply = [&](const C (&sv)[CPT], int b, int s) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            const C t = tfs[b][s][k][lane];
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) cmac(acc[k][j], t, sv[j]);
+        }
+    };
+    if (nchunk > 0) {
+        stage_load(0);
+        stage_store(0);
+        fetch(sa, u0);
+        __syncthreads();
+        for (int c = 0; c < nchunk; ++c) {
+            const int b = c & 1, ub = u0 + c * kClsS;
+            stage_load(c + 1 < nchunk ? c + 1 : c);   // the last chunk reloads itself (never stored)
+            // kClsS is even: the chunk's first source is in set A
+#pragma unroll
+            for (int s = 0; s < kClsS; s += 2) {
+                fetch(sb, ub + s + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                apply(sa, b, s);
+                fetch(sa, ub + s + 2);
+                __builtin_amdgcn_sched_barrier(0);
+                apply(sb, b, s + 1);
+            }
+            if (c + 1 < nchunk) stage_store(b ^ 1);
+            __syncthreads();
+        }
+    }
+    if (Bmp != nullptr) {
+        const C* P = Bmp + ((size_t)parent[child[0]] * NC * 3 + c0) * Kp;   // siblings share the parent
+        const C* sh[NM];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) sh[k] = shdn + (size_t)oct[child[k]] * Kp;
+        for (int j = t_ptr[qc]; j < t_ptr[qc + 1]; ++j) {
+            const int kp = t_kp[j];
+            const float w = t_w[j];
+            C pv[CPT];
+#pragma unroll
+            for (int jj = 0; jj < CPT; ++jj) pv[jj] = P[(size_t)jj * Kp + kp];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                C e = sh[k][kp];
+                e.x *= w;
+                e.y *= w;
+#pragma unroll
+                for (int jj = 0; jj < CPT; ++jj) cfma(acc[k][jj], e, pv[jj]);
+            }
+        }
+    }
+    if (q >= K) return;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        C* o = Bm + ((size_t)child[k] * NC * 3 + c0) * K + q;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) o[(size_t)j * K] = acc[k][j];
+    }
+}
+
+template <int NC, int G>
+__global__ __launch_bounds__(64 * G) void k_m2l_cls(int ncl, int K, int ktiles, int wc,
+                                                    const int* __restrict__ cl_child, const int* __restrict__ cl_ptr,
+                                                    const int* __restrict__ cl_src, const int* __restrict__ cl_vec,
+                                                    const float2* __restrict__ TFw, const float2* __restrict__ Sm,
+                                                    int Kp, const int* __restrict__ parent,
+                                                    const int* __restrict__ oct, const int* __restrict__ t_ptr,
+                                                    const int* __restrict__ t_kp, const float* __restrict__ t_w,
+                                                    const float2* __restrict__ shdn, const float2* __restrict__ Bmp,
+                                                    float2* __restrict__ Bm) {
+    __shared__ float2 tfs[2][kClsS][8][64];
+    constexpr int CPT = NC * 3 / G;
+    int cl, kt;
+    window_order(xcd_remap(blockIdx.x, ncl * ktiles), ncl, ktiles, wc, cl, kt);
+    const int c0 = (threadIdx.x >> 6) * CPT;
+    // members are packed first; the count is uniform over the workgroup
+    int nm = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nm += cl_child[cl * 8 + k] >= 0 ? 1 : 0;
+#define M2LS(NM)                                                                                                      \
+    m2l_cls_body<NC, G, NM>(cl, K, kt, c0, tfs, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm, Kp, parent, oct, t_ptr,   \
+                            t_kp, t_w, shdn, Bmp, Bm)
+    switch (nm) {
+        case 8: M2LS(8); break;
+        case 7: M2LS(7); break;
+        case 6: M2LS(6); break;
+        case 5: M2LS(5); break;
+        case 4: M2LS(4); break;
+        case 3: M2LS(3); break;
+        case 2: M2LS(2); break;
+        default: M2LS(1); break;
+    }
+#undef M2LS
+}
+
+template <typename R>
+void launch_m2l_cls(hipStream_t st, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+                    const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
+                    const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
+                    const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
+    if constexpr (sizeof(R) == 8) {
+        throw std::invalid_argument("LDS-staged cluster translation: FP32 far field only");
+    } else {
+        if (ncl == 0) return;
+        const int kt = (K + 63) / 64;
+        constexpr int wc = 128;
+#define M2LS_L(NC)                                                                                                    \
+    k_m2l_cls<NC, NC / 2><<<ncl * kt, 32 * NC, 0, st>>>(ncl, K, kt, wc, cl_child, cl_ptr, cl_src, cl_vec, TFw, Sm,   \
+                                                         Kp, parent, octant, t_ptr, t_kp, t_w, shift_down,            \
+                                                         Bm_parent, Bm)
+        if (ncur == 4) M2LS_L(4);
+        else if (ncur == 8) M2LS_L(8);
+        else if (ncur == 16) M2LS_L(16);
+        else throw std::invalid_argument("LDS-staged cluster translation: 4, 8 or 16 workspace currents per pass");
+#undef M2LS_L
+    }
+}
+
 template <typename R, int NC, int G>
 __global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, int wc, const int* __restrict__ cl_child,
                                                    const int* __restrict__ cl_ptr, const int* __restrict__ cl_src,
@@ -404,7 +592,7 @@ __global__ __launch_bounds__(64 * G) void k_m2l_cl(int ncl, int K, int ktiles, i
 }
 
 template <typename R>
-void launch_m2l_cl(hipStream_t st, bool wide, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+void launch_m2l_cl(hipStream_t st, int variant, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
                    const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
                    const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
                    const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
@@ -421,7 +609,7 @@ void launch_m2l_cl(hipStream_t st, bool wide, int ncur, int ncl, int K, const in
 #define M2LC(NC)                                                                                                      \
     {                                                                                                                 \
         if constexpr (!f64 && NC >= 4) {                                                                              \
-            if (wide) {                                                                                               \
+            if (variant == 1) {                                                                                       \
                 M2LC_G(NC, NC / 4);                                                                                   \
             } else {                                                                                                  \
                 M2LC_G(NC, NC / 2);                                                                                   \
@@ -658,7 +846,10 @@ void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
                                          const cplx_t<R>*, const cplx_t<R>*, double2, double2*);                  \
     template void launch_disagg_pmchwt_b<R>(hipStream_t, int, int, int, const int*, const int*,                    \
                                             const PmchwtFarArgs<R>&, double2*);                                   \
-    template void launch_m2l_cl<R>(hipStream_t, bool, int, int, int, const int*, const int*, const int*, const int*, \
+    template void launch_m2l_cls<R>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,      \
+                                    const cplx_t<R>*, const cplx_t<R>*, int, const int*, const int*, const int*,  \
+                                    const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);        \
+    template void launch_m2l_cl<R>(hipStream_t, int, int, int, int, const int*, const int*, const int*, const int*,  \
                                    const cplx_t<R>*, const cplx_t<R>*, int, const int*, const int*, const int*,   \
                                    const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);
 MLFMA_FAR_INSTANTIATE(double)

@@ -48,13 +48,22 @@ void launch_m2l_l2l(hipStream_t st, int ncur, int nbox, int K, const int* dr_ptr
 
 // Same product by sibling clusters: up to 4 children of one parent share every
 // source box of the union of their distant-relative lists, read once per column
-// group.  wide: 12 workspace columns per thread instead of 6 (FP32, 4 or more
-// currents), so each translation value serves twice the columns.
+// group.  variant 1 (FP32, 4 or more currents): 12 workspace columns per
+// thread instead of 6, so each translation value serves twice the columns.
 template <typename R>
-void launch_m2l_cl(hipStream_t st, bool wide, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+void launch_m2l_cl(hipStream_t st, int variant, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
                    const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
                    const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
                    const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm);
+
+// Same product by all-sibling clusters (up to 8 children of one parent;
+// cl_child and cl_vec have 8 slots per cluster) with the members' translation
+// samples staged in LDS once per workgroup (FP32, 4, 8 or 16 currents).
+template <typename R>
+void launch_m2l_cls(hipStream_t st, int ncur, int ncl, int K, const int* cl_child, const int* cl_ptr,
+                    const int* cl_src, const int* cl_vec, const cplx_t<R>* TFw, const cplx_t<R>* Sm, int Kp,
+                    const int* parent, const int* octant, const int* t_ptr, const int* t_kp, const R* t_w,
+                    const cplx_t<R>* shift_down, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm);
 
 // PEC: y[perm[i]] += c * sum_k Vf[i].B_leaf[k], leaf = leaf_of[i]
 template <typename R>

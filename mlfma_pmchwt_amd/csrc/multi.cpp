@@ -8,15 +8,15 @@
 // (Code/BatchRunner.h:470-480).
 //
 // Per object the recurrences are those of Engine::solve_block (same phases,
-// same kernels, same order of operations), so each object's iterates equal a
-// solve of that object alone; only the narrowing of a block whose columns
-// have partly converged is left out (an object keeps its width until it is
-// done).  The orthogonalisation phases of all objects are issued together and
-// share one synchronisation per phase.  Graphs are recorded per set of
+// same kernels, same order of operations, the same narrowing of a block once a
+// quarter of its columns has converged), so each object's iterates equal a
+// solve of that object alone.  The orthogonalisation of all objects is issued
+// together and shares one synchronisation per step.  Graphs are recorded per set of
 // (object, graph kind, right-hand sides, precision) taking part in a step, so
 // an object that finishes or falls back to a plainer preconditioner leads to
 // a new recording; the recordings live for the call.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <vector>
 
@@ -69,16 +69,28 @@ void solve_multi(const std::vector<Engine*>& es, const mlfma_solve_params& sp, c
         MLFMA_HIP(hipEventCreateWithFlags(&rec.join[k], hipEventDisableTiming));
     }
 
-    std::vector<BlockState> s(K);
-    for (int k = 0; k < K; ++k) es[k]->bs_start(s[k], sp, B[k], X[k], R[k]);
+    // per object a stack of states: a block narrowed to its unconverged
+    // columns continues as a child state on top of its parent
+    std::vector<std::vector<BlockState>> stk(K);
+    std::vector<std::vector<NarrowRecord>> recs(K);
+    for (int k = 0; k < K; ++k) {
+        stk[k].emplace_back();
+        es[k]->bs_start(stk[k].back(), sp, B[k], X[k], R[k]);
+    }
+    auto top = [&](int k) -> BlockState& { return stk[k].back(); };
 
     // one graph over the matvec bodies of the objects in `open`
+    static const bool eager_exp = getenv("MLFMA_MULTI_EAGER") != nullptr;
     auto launch = [&](const std::vector<int>& open) {
+        if (eager_exp) {
+            for (int k : open) es[k]->bs_body(top(k));
+            return;
+        }
         std::vector<int> key;
         for (int k : open) {
             key.push_back(k);
-            key.push_back(es[k]->bs_graph_kind(s[k]));
-            key.push_back(s[k].R);
+            key.push_back(es[k]->bs_graph_kind(top(k)));
+            key.push_back(top(k).R);
             key.push_back(es[k]->apply_prec);
         }
         // the inputs each object staged on its own stream come first
@@ -97,7 +109,7 @@ void solve_multi(const std::vector<Engine*>& es, const mlfma_solve_params& sp, c
                     captured = k;
                     MLFMA_HIP(hipStreamWaitEvent(es[k]->st, rec.fork, 0));
                     es[k]->set_capturing(true);
-                    es[k]->bs_body(s[k]);
+                    es[k]->bs_body(top(k));
                     es[k]->set_capturing(false);
                     MLFMA_HIP(hipEventRecord(rec.join[k], es[k]->st));
                     MLFMA_HIP(hipStreamWaitEvent(rec.ms, rec.join[k], 0));
@@ -123,28 +135,43 @@ void solve_multi(const std::vector<Engine*>& es, const mlfma_solve_params& sp, c
     for (;;) {
         std::vector<int> active;
         for (int k = 0; k < K; ++k)
-            if (es[k]->bs_active(s[k])) active.push_back(k);
+            if (es[k]->bs_active(top(k))) active.push_back(k);
         if (active.empty()) break;
-        for (int k : active) es[k]->bs_cycle_begin(s[k]);
+        for (int k : active) es[k]->bs_cycle_begin(top(k));
         for (;;) {
             std::vector<int> open;
             for (int k : active)
-                if (es[k]->bs_step_open(s[k])) open.push_back(k);
+                if (es[k]->bs_step_open(top(k))) open.push_back(k);
             if (open.empty()) break;
-            for (int k : open) es[k]->bs_pre(s[k], true);
+            for (int k : open) es[k]->bs_pre(top(k), true);
             launch(open);
-            for (int k : open) es[k]->bs_post(s[k], true);
-            for (int phase = 0; phase < 4; ++phase) {
-                for (int k : open) es[k]->bs_orth(s[k], phase);
-                if (phase < 3)
-                    for (int k : open) MLFMA_HIP(hipStreamSynchronize(es[k]->st));
+            for (int k : open) es[k]->bs_post(top(k), true);
+            for (int k : open) es[k]->bs_orth(top(k), 0);
+            for (int k : open) MLFMA_HIP(hipStreamSynchronize(es[k]->st));
+            for (int k : open) es[k]->bs_orth(top(k), 1);
+        }
+        for (int k : active) {
+            es[k]->bs_cycle_end(top(k), false);
+            // once a quarter of the block has converged, its other columns go on
+            // as a narrower block (the graphs of the new width are recorded anew)
+            if (!es[k]->bs_active(top(k))) continue;
+            BlockState child;
+            NarrowRecord nr;
+            if (es[k]->bs_narrow(top(k), child, nr)) {
+                stk[k].push_back(std::move(child));
+                recs[k].push_back(std::move(nr));
             }
         }
-        for (int k : active) es[k]->bs_cycle_end(s[k], false);
     }
     for (int k = 0; k < K; ++k) {
+        while (stk[k].size() > 1) {
+            BlockState child = std::move(stk[k].back());
+            stk[k].pop_back();
+            es[k]->bs_unnarrow(stk[k].back(), child, recs[k].back());
+            recs[k].pop_back();
+        }
         MLFMA_HIP(hipStreamSynchronize(es[k]->st));
-        es[k]->bs_stats(s[k], stats[k]);
+        es[k]->bs_stats(stk[k][0], stats[k]);
     }
 }
 

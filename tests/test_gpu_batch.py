@@ -175,7 +175,7 @@ def test_cluster_translation_matches_per_box(name):
     mesh, op, orc = pair(name)
     X = np.stack([seeded_x(op.n, 900 + j) for j in range(9)], axis=1)   # passes of 8 and 1
     out = {}
-    for bits in (0, 4, 12):   # per box, sibling clusters, sibling clusters with 12 columns per thread (FP32)
+    for bits in (0, 4, 12, 20):   # per box, sibling clusters, 12 columns per thread, all-sibling LDS-staged (FP32)
         for prec in (api.FP64, api.FP32):
             op.set_tuning(prec, 0, bits)
             op.set_apply_precision(prec)
@@ -187,9 +187,11 @@ def test_cluster_translation_matches_per_box(name):
         e64 = relerr(out[(4, api.FP64)][k], out[(0, api.FP64)][k])
         e32 = relerr(out[(4, api.FP32)][k], out[(0, api.FP32)][k])
         w32 = relerr(out[(12, api.FP32)][k], out[(4, api.FP32)][k])
-        print(f"[cluster] {name} {'batched' if k else 'single'} fp64 {e64:.2e} fp32 {e32:.2e} wide fp32 {w32:.2e}")
+        s32 = relerr(out[(20, api.FP32)][k], out[(4, api.FP32)][k])
+        print(f"[cluster] {name} {'batched' if k else 'single'} fp64 {e64:.2e} fp32 {e32:.2e} wide fp32 {w32:.2e} "
+              f"all-sibling fp32 {s32:.2e}")
         assert e64 <= 1e-13 and e32 <= 2e-6
-        assert w32 <= 1e-12   # the same sums in the same order, other column grouping
+        assert w32 <= 1e-12 and s32 <= 1e-12   # the same sums in the same order (plus exact zeros)
     assert relerr(out[(4, api.FP64)][0], orc.apply(X[:, 0])) <= 1e-10
 
 
