@@ -16,7 +16,6 @@
 // an object that finishes or falls back to a plainer preconditioner leads to
 // a new recording; the recordings live for the call.
 #include <algorithm>
-#include <cstdlib>
 #include <map>
 #include <vector>
 
@@ -80,12 +79,7 @@ void solve_multi(const std::vector<Engine*>& es, const mlfma_solve_params& sp, c
     auto top = [&](int k) -> BlockState& { return stk[k].back(); };
 
     // one graph over the matvec bodies of the objects in `open`
-    static const bool eager_exp = getenv("MLFMA_MULTI_EAGER") != nullptr;
     auto launch = [&](const std::vector<int>& open) {
-        if (eager_exp) {
-            for (int k : open) es[k]->bs_body(top(k));
-            return;
-        }
         std::vector<int> key;
         for (int k : open) {
             key.push_back(k);
