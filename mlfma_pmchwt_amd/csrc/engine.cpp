@@ -256,10 +256,18 @@ void Engine::build_wave(int w) {
     D.tap.assign(L + 1, nullptr);
     D.t_ptr.assign(L + 1, nullptr);
     D.t_kp.assign(L + 1, nullptr);
+    D.th_tap.assign(L + 1, nullptr);
+    D.ph_tap.assign(L + 1, nullptr);
+    D.Lp.assign(L + 1, 0);
+    D.Lc.assign(L + 1, 0);
     for (int l = 2; l <= L; ++l) D.K[l] = T.grid[l].K();
     for (int c = 3; c <= L; ++c) {
         const InterpTable& it = T.interp[c];
         D.tap[c] = upload(it.tap);
+        D.th_tap[c] = upload(it.th_tap);
+        D.ph_tap[c] = upload(it.ph_tap);
+        D.Lp[c] = it.Lp;
+        D.Lc[c] = it.Lc;
         D.t_ptr[c] = upload(it.t_ptr);
         D.t_kp[c] = upload(it.t_kp);
     }
@@ -316,6 +324,8 @@ void Engine::build_far(int w, const WaveTables& T) {
     F.Bm.assign(L + 1, nullptr);
     F.TFw.assign(L + 1, nullptr);
     F.tw.assign(L + 1, nullptr);
+    F.th_w.assign(L + 1, nullptr);
+    F.ph_w.assign(L + 1, nullptr);
     F.t_w.assign(L + 1, nullptr);
     F.sh_up.assign(L + 1, nullptr);
     F.sh_dn.assign(L + 1, nullptr);
@@ -354,6 +364,8 @@ void Engine::build_far(int w, const WaveTables& T) {
     for (int c = 3; c <= L; ++c) {
         const InterpTable& it = T.interp[c];
         F.tw[c] = upload(to_real<R>(it.w));
+        F.th_w[c] = upload(to_real<R>(it.th_w));
+        F.ph_w[c] = upload(to_real<R>(it.ph_w));
         F.t_w[c] = upload(to_real<R>(it.t_w));
         std::vector<cplx_t<R>> su(T.shift_up[c].size()), sd(T.shift_down[c].size());
         for (size_t e = 0; e < su.size(); ++e) su[e] = to_cx<R>(T.shift_up[c][e]);
@@ -510,14 +522,27 @@ void Engine::far_pass(int w, const double2* x, int ldr, int r0, int rb) {
                               F.Sm[L]);
     mark(6 * w + 1);
     mark(6 * w + 2);
-    for (int l = L - 1; l >= 2; --l)
-        launch_m2m<R>(st, nc, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx, lv_[l + 1].oct,
-                      D.tap[l + 1], F.tw[l + 1], F.sh_up[l + 1], F.Sm[l + 1], F.Sm[l]);
+    for (int l = L - 1; l >= 2; --l) interpolate<R>(w, l, nc);
     mark(6 * w + 3);
     mark(6 * w + 4);
     for (int l = 2; l <= L; ++l) translate<R>(w, l, nc);
     F.Sm[L] = leaf_ws;
     mark(6 * w + 5);
+}
+
+// m2m of parent level l from level l + 1: the separable form where it
+// applies, else the 9-tap gather (m2m_separable_ off: always the 9-tap form).
+template <typename R>
+void Engine::interpolate(int w, int l, int nc) {
+    WaveDev& D = wd_[w];
+    FarDev<R>& F = far<R>(w);
+    const int c = l + 1;
+    if (m2m_separable_ && launch_m2m_sep<R>(st, nc, lv_[l].n, D.Lp[c], D.Lc[c], lv_[l].son_ptr, lv_[l].son_idx,
+                                            lv_[c].oct, D.th_tap[c], F.th_w[c], D.ph_tap[c], F.ph_w[c], F.sh_up[c],
+                                            F.Sm[c], F.Sm[l]))
+        return;
+    launch_m2m<R>(st, nc, lv_[l].n, D.K[l], D.K[c], lv_[l].son_ptr, lv_[l].son_idx, lv_[c].oct, D.tap[c], F.tw[c],
+                  F.sh_up[c], F.Sm[c], F.Sm[l]);
 }
 
 // m2l + l2l of one level: sibling-cluster kernel (sources staged once per
@@ -974,10 +999,9 @@ void Engine::stage(int which, const double2* dx, double2* dy) {
             for (int w = 0; w < nw; ++w) {
                 WaveDev& D = wd_[w];
                 const FarDev<double>& F = D.f64;
-                for (int l = L - 1; l >= 2; --l)
-                    launch_m2m<double>(st, ncur, lv_[l].n, D.K[l], D.K[l + 1], lv_[l].son_ptr, lv_[l].son_idx,
-                                       lv_[l + 1].oct, D.tap[l + 1], F.tw[l + 1], F.sh_up[l + 1], F.Sm[l + 1],
-                                       F.Sm[l]);
+                (void)D;
+                (void)F;
+                for (int l = L - 1; l >= 2; --l) interpolate<double>(w, l, ncur);
             }
             break;
         case MLFMA_STAGE_M2L_L2L:

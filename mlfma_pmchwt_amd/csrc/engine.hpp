@@ -63,12 +63,15 @@ struct FarDev {
     R* kp0 = nullptr;                    // [3][K0] finest-level directions
     std::vector<cplx_t<R>*> Sm, Bm, TFw;     // per tree level
     std::vector<R*> tw, t_w;             // per child tree level
+    std::vector<R*> th_w, ph_w;          // per child tree level: the stencil's weights by direction
     std::vector<cplx_t<R>*> sh_up, sh_dn;
 };
 
 struct WaveDev {
     std::vector<int> K;                  // per tree level
     std::vector<int*> tap, t_ptr, t_kp;  // per child tree level
+    std::vector<int*> th_tap, ph_tap;    // per child tree level: the stencil's taps by direction
+    std::vector<int> Lp, Lc;             // per child tree level: theta rows of the parent and child grids
     FarDev<double> f64;                  // always built (true residuals, stage API)
     FarDev<float> f32;                   // built when far_precision = MLFMA_FP32
     float2* E = nullptr;                 // batched matrix-core disaggregation: effective receive-side matrices,
@@ -240,6 +243,8 @@ private:
     template <typename R>
     void translate(int w, int l, int nc);
     template <typename R>
+    void interpolate(int w, int l, int nc);
+    template <typename R>
     cplx_t<R>* beff_buf(int w);
     void apply_pass(const double2* x, double2* y, int ldr, int r0, int rb, int prec, bool with_near);
     void mark(int slot);
@@ -346,6 +351,7 @@ private:
     bool near_mfma_ = true;     // batched near product on the matrix cores (FP32 copy or FP64 entries)
     bool disagg_mfma_ = true;   // batched disaggregation on the matrix cores
     bool m2l_cluster_ = true;   // translation by sibling clusters
+    bool m2m_separable_ = true;  // interpolation: separable (theta, then phi) form where it applies
     int m2l_variant_ = 0;       // FP32 cluster translation: 1 twelve columns per thread, 2 all-sibling clusters, LDS-staged samples
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;

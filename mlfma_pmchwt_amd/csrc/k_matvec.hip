@@ -211,6 +211,139 @@ void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int
 #undef M2M
 }
 
+// Separable form of the same interpolation: the 3 x 3 stencil of a parent
+// sample (i, j) is child rows th_tap[i][a] x child columns ph_tap[j][b] with
+// weights th_w[i][a] ph_w[j][b], so a workgroup owning parent p's row i first
+// interpolates in theta, U(m) = sum_a th_w[i][a] S_c(th_tap[i][a], m), for all
+// 2 Lc child columns m of each of its CPT columns into LDS (contiguous rows of
+// the child workspace, each read once per parent row), then in phi,
+// r(j) = sum_b ph_w[j][b] U(ph_tap[j][b]), and adds shift x r, child after
+// child.  About half the loads and multiply-adds of the 9-tap form; the sums
+// are regrouped (theta inside), which changes only rounding.  A thread owns
+// parent columns j = lane + 64 jj (jj < JJ) of the row; G = nc x 3 / CPT
+// wavefronts, one per column group, each with its own LDS rows.
+template <typename R, int NC, int CPT, int JJ>
+__global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, const int* __restrict__ son_ptr,
+                                                        const int* __restrict__ son_idx,
+                                                        const int* __restrict__ oct_child,
+                                                        const int* __restrict__ th_tap, const R* __restrict__ th_w,
+                                                        const int* __restrict__ ph_tap, const R* __restrict__ ph_w,
+                                                        const cplx_t<R>* __restrict__ shup,
+                                                        const cplx_t<R>* __restrict__ Smc,
+                                                        cplx_t<R>* __restrict__ Smp) {
+    using C = cplx_t<R>;
+    extern __shared__ __align__(16) unsigned char m2m_lds[];
+    const int pfn = 2 * Lp, psn = 2 * Lc, Kp = Lp * pfn, Kc = Lc * psn;
+    const int lin = xcd_remap(blockIdx.x, (int)gridDim.x);
+    const int p = lin / Lp, i = lin % Lp;   // the rows of one parent are consecutive
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c0 = wave * CPT;
+    C* U = reinterpret_cast<C*>(m2m_lds) + (size_t)wave * CPT * psn;   // this wavefront's [CPT][2 Lc]
+    int ti[3];
+    R lt[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ti[a] = th_tap[3 * i + a] * psn;
+        lt[a] = th_w[3 * i + a];
+    }
+    int pj[JJ][3];
+    R wj[JJ][3];
+#pragma unroll
+    for (int jj = 0; jj < JJ; ++jj) {
+        const int j = min(lane + 64 * jj, pfn - 1);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            pj[jj][b] = ph_tap[3 * j + b];
+            wj[jj][b] = ph_w[3 * j + b];
+        }
+    }
+    C acc[JJ][CPT];
+#pragma unroll
+    for (int jj = 0; jj < JJ; ++jj)
+#pragma unroll
+        for (int d = 0; d < CPT; ++d) acc[jj][d] = C{0, 0};
+    for (int s = son_ptr[p]; s < son_ptr[p + 1]; ++s) {
+        const int c = son_idx[s];
+        const C* S = Smc + ((size_t)c * NC * 3 + c0) * Kc;
+        for (int m = lane; m < psn; m += 64) {
+#pragma unroll
+            for (int d = 0; d < CPT; ++d) {
+                const C* sd = S + (size_t)d * Kc + m;
+                const C f0 = sd[ti[0]], f1 = sd[ti[1]], f2 = sd[ti[2]];
+                C u;
+                u.x = f0.x * lt[0] + f1.x * lt[1] + f2.x * lt[2];
+                u.y = f0.y * lt[0] + f1.y * lt[1] + f2.y * lt[2];
+                U[d * psn + m] = u;
+            }
+        }
+        __syncthreads();
+        const C* sh = shup + (size_t)oct_child[c] * Kp + (size_t)i * pfn;
+#pragma unroll
+        for (int jj = 0; jj < JJ; ++jj) {
+            const int j = lane + 64 * jj;
+            if (j < pfn) {
+                const C e = sh[j];
+#pragma unroll
+                for (int d = 0; d < CPT; ++d) {
+                    const C u0 = U[d * psn + pj[jj][0]], u1 = U[d * psn + pj[jj][1]], u2 = U[d * psn + pj[jj][2]];
+                    C r;
+                    r.x = u0.x * wj[jj][0] + u1.x * wj[jj][1] + u2.x * wj[jj][2];
+                    r.y = u0.y * wj[jj][0] + u1.y * wj[jj][1] + u2.y * wj[jj][2];
+                    cfma(acc[jj][d], e, r);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    C* o = Smp + ((size_t)p * NC * 3 + c0) * Kp + (size_t)i * pfn;
+#pragma unroll
+    for (int jj = 0; jj < JJ; ++jj) {
+        const int j = lane + 64 * jj;
+        if (j < pfn)
+#pragma unroll
+            for (int d = 0; d < CPT; ++d) o[(size_t)d * Kp + j] = acc[jj][d];
+    }
+}
+
+// The separable interpolation when its LDS rows fit (64 KB per workgroup) and
+// a parent row has at most 256 samples; false otherwise (the caller takes the
+// 9-tap kernel).
+template <typename R>
+bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const int* son_ptr, const int* son_idx,
+                    const int* octant_child, const int* th_tap, const R* th_w, const int* ph_tap, const R* ph_w,
+                    const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child, cplx_t<R>* Sm_parent) {
+    if (nparent == 0) return true;
+    const int JJ = (2 * Lp + 63) / 64;
+    if (JJ > 4) return false;
+    const size_t lds = (size_t)ncur * 3 * 2 * Lc * sizeof(cplx_t<R>);   // all column groups' rows
+    if (lds > 64 * 1024) return false;
+    const unsigned grid = (unsigned)nparent * Lp;
+#define M2MS(NC, CPT, J)                                                                                              \
+    k_m2m_sep<R, NC, CPT, J><<<grid, 64 * NC * 3 / CPT, lds, st>>>(Lp, Lc, son_ptr, son_idx, octant_child, th_tap,   \
+                                                                    th_w, ph_tap, ph_w, shift_up, Sm_child, Sm_parent)
+#define M2MS_J(NC, CPT)                                                                                               \
+    if (JJ == 1) M2MS(NC, CPT, 1);                                                                                    \
+    else if (JJ == 2) M2MS(NC, CPT, 2);                                                                               \
+    else if (JJ == 3) M2MS(NC, CPT, 3);                                                                               \
+    else M2MS(NC, CPT, 4);
+    if (ncur == 1) {
+        M2MS_J(1, 3)
+    } else if (ncur == 2) {
+        M2MS_J(2, 3)
+    } else if (ncur == 4) {
+        M2MS_J(4, 3)
+    } else if (ncur == 8) {
+        M2MS_J(8, 3)
+    } else if constexpr (sizeof(R) == 4) {
+        M2MS_J(16, 6)
+    } else {
+        throw std::invalid_argument("FP64 far field: at most 8 workspace currents per pass");
+    }
+#undef M2MS_J
+#undef M2MS
+    return true;
+}
+
 // ---------------------------------------------------------------- translation + anterpolation
 // Code/ProcessMLFMM.h:112-147 (m2l, quadrature weight folded into TFw) and
 // :152-202 (l2l) in gather form: each child sample sums the parent samples whose
@@ -846,6 +979,9 @@ void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
                                          const cplx_t<R>*, const cplx_t<R>*, double2, double2*);                  \
     template void launch_disagg_pmchwt_b<R>(hipStream_t, int, int, int, const int*, const int*,                    \
                                             const PmchwtFarArgs<R>&, double2*);                                   \
+    template bool launch_m2m_sep<R>(hipStream_t, int, int, int, int, const int*, const int*, const int*, const int*, \
+                                    const R*, const int*, const R*, const cplx_t<R>*, const cplx_t<R>*,           \
+                                    cplx_t<R>*);                                                                  \
     template void launch_m2l_cls<R>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,      \
                                     const cplx_t<R>*, const cplx_t<R>*, int, const int*, const int*, const int*,  \
                                     const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);        \

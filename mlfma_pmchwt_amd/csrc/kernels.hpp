@@ -38,6 +38,14 @@ void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int
                 const int* octant_child, const int* tap, const R* w, const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child,
                 cplx_t<R>* Sm_parent);
 
+// Separable (theta, then phi) form of launch_m2m from the direction tables of
+// the stencil; returns false when it does not apply (the caller then runs
+// launch_m2m).
+template <typename R>
+bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const int* son_ptr, const int* son_idx,
+                    const int* octant_child, const int* th_tap, const R* th_w, const int* ph_tap, const R* ph_w,
+                    const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child, cplx_t<R>* Sm_parent);
+
 // Bm[b][..][q] = sum_dr TFw[vec][q] * Sm[dr][..][q]
 //              + (parent present) sum_{(kp,w) in T[q]} w * shift_down[oct][kp] * Bm_parent[par][..][kp]
 template <typename R>

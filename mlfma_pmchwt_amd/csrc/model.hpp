@@ -75,6 +75,12 @@ struct InterpTable {
     int Kp = 0, Kc = 0;
     std::vector<int> tap;        // [9][Kp] child sample index (plane per tap)
     std::vector<double> w;       // [9][Kp]
+    // the same stencil by direction: tap (3 a + b) is child sample
+    // th_tap[i][a] * 2 Lc + ph_tap[j][b] with weight th_w[i][a] * ph_w[j][b]
+    // for parent sample i * 2 Lp + j
+    int Lp = 0, Lc = 0;
+    std::vector<int> th_tap, ph_tap;    // [Lp][3], [2 Lp][3]
+    std::vector<double> th_w, ph_w;     // [Lp][3], [2 Lp][3]
     // transposed (gather form of the anterpolation): for each child sample the
     // parent samples that use it, ascending
     std::vector<int> t_ptr;      // Kc + 1

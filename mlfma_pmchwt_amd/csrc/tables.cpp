@@ -229,6 +229,12 @@ void build_wave_tables(const Octree& t, cplx k, double kabs, int tf_mode, WaveTa
         it.tap.assign(9 * (size_t)it.Kp, 0);
         it.w.assign(9 * (size_t)it.Kp, 0.0);
         const int pfn = (int)P.phi.size(), psn = (int)S.phi.size();
+        it.Lp = (int)P.theta.size();
+        it.Lc = (int)S.theta.size();
+        it.th_tap.assign(3 * (size_t)it.Lp, 0);
+        it.th_w.assign(3 * (size_t)it.Lp, 0.0);
+        it.ph_tap.assign(3 * (size_t)pfn, 0);
+        it.ph_w.assign(3 * (size_t)pfn, 0.0);
         for (int q = 0; q < it.Kp; ++q) {
             int ti[3], pi[3];
             double t0 = P.theta[q / pfn], p0 = P.phi[q % pfn];
@@ -245,6 +251,13 @@ void build_wave_tables(const Octree& t, cplx k, double kabs, int tf_mode, WaveTa
                     it.tap[(size_t)(3 * a + b) * it.Kp + q] = ti[a] * psn + pi[b];
                     it.w[(size_t)(3 * a + b) * it.Kp + q] = lt[a] * lp[b];
                 }
+            const int i = q / pfn, j = q % pfn;
+            for (int a = 0; a < 3; ++a) {
+                it.th_tap[3 * (size_t)i + a] = ti[a];
+                it.th_w[3 * (size_t)i + a] = lt[a];
+                it.ph_tap[3 * (size_t)j + a] = pi[a];
+                it.ph_w[3 * (size_t)j + a] = lp[a];
+            }
         }
         it.t_ptr.assign(it.Kc + 1, 0);
         for (int q = 0; q < it.Kp; ++q)
