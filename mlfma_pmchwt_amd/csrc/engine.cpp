@@ -281,6 +281,24 @@ void Engine::build_wave(int w) {
             const int Q = 3 * D.K[L], Qp = (Q + 31) & ~31;
             d_Vb_[w] = dalloc<unsigned short>((size_t)4 * N * Qp);
             launch_split_rows(st, N, Q, Qp, far<float>(w).Vf, d_Vb_[w]);
+            // and transposed planes of the radiation patterns (all-RHS aggregation):
+            // the RWGs of a leaf from a 32-aligned column on
+            if (!d_jmap_) {
+                std::vector<int> joff(nleaf + 1, 0), jmap;
+                for (int t = 0; t < nleaf; ++t) {
+                    const int nt = tree.leaf_ptr[t + 1] - tree.leaf_ptr[t];
+                    joff[t + 1] = joff[t] + (nt + 31) / 32 * 32;
+                }
+                jmap.assign(joff[nleaf], -1);
+                for (int t = 0; t < nleaf; ++t)
+                    for (int i = tree.leaf_ptr[t]; i < tree.leaf_ptr[t + 1]; ++i) jmap[joff[t] + i - tree.leaf_ptr[t]] = i;
+                Np_ = joff[nleaf];
+                d_joff_ = upload(joff);
+                d_jmap_ = upload(jmap);
+            }
+            Qa_[w] = (Q + 63) / 64 * 64;
+            d_Va_[w] = dalloc<unsigned short>((size_t)4 * Qa_[w] * Np_);
+            launch_split_cols(st, Q, Qa_[w], Np_, d_jmap_, far<float>(w).Vs, d_Va_[w]);
             MLFMA_HIP(hipGetLastError());
         }
     }
@@ -743,9 +761,14 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
         ensure_eall(R);
         const int L = tree.max_level;
         timed(0, nw, [&] {
-            for (int w = 0; w < nw; ++w)
-                launch_aggregate_all(st, ncur, nleaf, wd_[w].K[L], N, R, cap, d_leaf_ptr_, d_perm_, far<float>(w).Vs,
-                                     X, d_Sall_[w]);
+            for (int w = 0; w < nw; ++w) {
+                if (d_Va_[w])   // split-pair BF16 (near_precision = MLFMA_BF16)
+                    launch_aggregate_bf3(st, ncur, nleaf, wd_[w].K[L], N, R, cap, Qa_[w], Np_, d_leaf_ptr_, d_joff_,
+                                         d_perm_, d_Va_[w], X, d_Sall_[w]);
+                else
+                    launch_aggregate_all(st, ncur, nleaf, wd_[w].K[L], N, R, cap, d_leaf_ptr_, d_perm_,
+                                         far<float>(w).Vs, X, d_Sall_[w]);
+            }
         });
         eall_R_ = R;
     }

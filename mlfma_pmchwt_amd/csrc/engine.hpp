@@ -283,6 +283,11 @@ private:
     long long* d_bfoff_ = nullptr;
     float2* d_Eall_[2] = {nullptr, nullptr};   // all-RHS effective matrices (FP32), nleaf x no x 3K x eall_cap_
     unsigned short* d_Vb_[2] = {nullptr, nullptr};   // split-pair BF16 receive-pattern planes (near_precision = MLFMA_BF16)
+    unsigned short* d_Va_[2] = {nullptr, nullptr};   // split-pair BF16 radiation-pattern planes, transposed [4][Qa][Np]
+    int Qa_[2] = {0, 0};                             // their sample rows (3 K rounded up to 64)
+    int Np_ = 0;                                     // their RWG columns (every leaf rounded up to 32)
+    int* d_joff_ = nullptr;                          // first column of each leaf
+    int* d_jmap_ = nullptr;                          // RWG of each column, -1 for padding
     float2* d_Sall_[2] = {nullptr, nullptr};   // all-RHS leaf radiation workspaces of the passes, nleaf x ncur x 3K x eall_cap_
     int eall_cap_ = 0;
     int eall_R_ = 0;   // > 0 while a batched FP32 matvec defers its disaggregation to one all-RHS product

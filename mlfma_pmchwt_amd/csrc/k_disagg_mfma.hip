@@ -708,6 +708,193 @@ void launch_aggregate_all(hipStream_t st, int ncur, int nleaf, int K, int N, int
 #undef AA
 }
 
+// ---------------------------------------------------------------- aggregation, split-pair BF16
+// The same all-RHS product S (3K x ncur R) = V^T (3K x nt) X (nt x ncur R) per
+// leaf on the BF16 matrix cores from split pairs (near_precision = MLFMA_BF16,
+// FP32 far field): V = Vh + Vl from transposed planes split once at set-up,
+// X = Xh + Xl split while it is staged, S = Vh Xh + Vh Xl + Vl Xh (about 2^-17
+// of |V||X| from the FP32 product), FP32 accumulation.  The planes are
+// [hi Re, hi Im, lo Re, lo Im][Qp][Np]: row q of every pattern, the RWGs of a
+// leaf from a 32-aligned column joff[leaf] on, zero past the leaf and past Q, so
+// a lane's A fragment (8 consecutive RWGs of one sample) is one 16-byte load.
+// A workgroup owns one leaf, 64 samples (a 16-row tile per wavefront) and CT
+// column tiles of 8 complex columns, columns current-major (c = cur R + r, so
+// a staged column group reads consecutive right-hand sides); X is staged 32
+// RWGs at a time, the next block's X and patterns loaded during the current
+// block's products.
+__global__ void k_split_cols(int Q, int Qp, int Np, const int* __restrict__ jmap, const float2* __restrict__ V,
+                             unsigned short* __restrict__ planes) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)Qp * Np) return;
+    const int q = (int)(e / Np), jp = (int)(e - (size_t)q * Np);
+    const int i = jmap[jp];
+    const float2 v = (i >= 0 && q < Q) ? V[(size_t)i * Q + q] : make_float2(0.f, 0.f);
+    const unsigned hr = bf16b(v.x), hi = bf16b(v.y);
+    const size_t plane = (size_t)Qp * Np;
+    planes[e] = (unsigned short)hr;
+    planes[plane + e] = (unsigned short)hi;
+    planes[2 * plane + e] = (unsigned short)bf16b(v.x - bf16f(hr));
+    planes[3 * plane + e] = (unsigned short)bf16b(v.y - bf16f(hi));
+}
+
+void launch_split_cols(hipStream_t st, int Q, int Qp, int Np, const int* jmap, const float2* V,
+                       unsigned short* planes) {
+    const size_t total = (size_t)Qp * Np;
+    if (total == 0) return;
+    k_split_cols<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(Q, Qp, Np, jmap, V, planes);
+}
+
+template <int NCUR, int CT>
+__global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                                                       const int* __restrict__ leaf_ptr,
+                                                       const int* __restrict__ joff, const int* __restrict__ perm,
+                                                       const unsigned short* __restrict__ planes,
+                                                       const double2* __restrict__ x, float2* __restrict__ Sall) {
+    constexpr int CC = 8 * CT;            // complex columns per workgroup
+    constexpr int kJ = 32, kP = kJ + 8;   // staged RWGs per block, padded LDS row (80 B)
+    constexpr int NE = kJ * CC, NU = (NE + 255) / 256;
+    __shared__ __align__(16) unsigned short xs[2][2][CC][kP];   // [hi, lo][Re, Im][column][RWG]
+    const int Q = 3 * K, QG = Qp / 64;
+    const int ncol = NCUR * R, ncg = (ncol + CC - 1) / CC;
+    int lq, cg;
+    xcd_chunk(blockIdx.x, QG, ncg, lq, cg);
+    if (lq >= nleaf * QG) return;
+    const int leaf = lq / QG, qg = lq - leaf * QG;
+    const int i0 = leaf_ptr[leaf], nt = leaf_ptr[leaf + 1] - i0;
+    const int nkb = (nt + kJ - 1) / kJ;
+    const int cb0 = cg * CC;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int part = li >> 3;
+    const size_t plane = (size_t)Qp * Np;
+    const unsigned short* arow = planes + (size_t)((qg * 4 + wave) * 16 + li) * Np + joff[leaf] + 8 * lk;
+    floatx4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (nt > 0) {
+        // patterns of block kb, clamped to the last block (a block past nkb
+        // meets zero-staged X), without branches
+        auto load_a = [&](int kb, uint4* a) {
+            const int kc = min(kb, nkb - 1) * kJ;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) a[p] = *reinterpret_cast<const uint4*>(arow + p * plane + kc);
+        };
+        double2 px[NU];
+        unsigned pmask = 0;
+        auto load_x = [&](int kb) {
+            pmask = 0;
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = min((int)threadIdx.x + 256 * u, NE - 1);
+                const int jj = e / CC, c = e - jj * CC;
+                const int col = min(cb0 + c, ncol - 1);
+                const int cur = col / R, r = col - cur * R;
+                const int jg = kb * kJ + jj;
+                px[u] = x[((size_t)cur * N + perm[i0 + min(jg, nt - 1)]) * R + r];
+                pmask |= (unsigned)((NE % 256 == 0 || (int)threadIdx.x + 256 * u < NE) & (jg < nt) &
+                                    (cb0 + c < ncol)) << u;
+            }
+        };
+        auto store_x = [&] {
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int e = (int)threadIdx.x + 256 * u;
+                if (NE % 256 == 0 || e < NE) {
+                    const int jj = e / CC, c = e - jj * CC;
+                    const double2 v = ((pmask >> u) & 1u) ? px[u] : make_double2(0.0, 0.0);
+                    const float fr = (float)v.x, fi = (float)v.y;
+                    const unsigned hr = bf16b(fr), hi = bf16b(fi);
+                    xs[0][0][c][jj] = (unsigned short)hr;
+                    xs[0][1][c][jj] = (unsigned short)hi;
+                    xs[1][0][c][jj] = (unsigned short)bf16b(fr - bf16f(hr));
+                    xs[1][1][c][jj] = (unsigned short)bf16b(fi - bf16f(hi));
+                }
+            }
+        };
+        // one block: stage its X, then the next block's loads, then the
+        // products from the current patterns (cur); blocks alternate between
+        // two register sets, so nothing is copied
+        auto block = [&](int kb, const uint4* cur, uint4* nxt) {
+            __syncthreads();
+            store_x();
+            __syncthreads();
+            load_x(kb + 1);
+            load_a(kb + 1, nxt);
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const int rc = ct * 8 + (li & 7);
+                floatx4 d = acc[ct];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {   // X part: hi, then lo (lo meets only the hi patterns)
+                    const uint4 xa = *reinterpret_cast<const uint4*>(&xs[h][part][rc][8 * lk]);       // Re X | Im X
+                    const uint4 xo = *reinterpret_cast<const uint4*>(&xs[h][1 - part][rc][8 * lk]);   // Im X | Re X
+                    const uint4 b1 = xa, b2 = part ? xo : neg8b(xo);
+                    d = mfb(cur[0], b1, d);
+                    d = mfb(cur[1], b2, d);
+                    if (h == 0) {
+                        d = mfb(cur[2], b1, d);
+                        d = mfb(cur[3], b2, d);
+                    }
+                }
+                acc[ct] = d;
+            }
+        };
+        uint4 aa[4], ab[4];
+        load_x(0);
+        load_a(0, aa);
+        for (int kb = 0; kb < nkb; kb += 2) {
+            block(kb, aa, ab);
+            block(kb + 1, ab, aa);
+        }
+    }
+    // D[4 lk + v][li]: lanes li < 8 hold Re of column ct * 8 + li, lane li + 8 its Im
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int c = cb0 + ct * 8 + li;
+        const int cur = c / R, r = c - cur * R;
+        int p0 = 0, rb = 1;   // the pass holding right-hand side r
+        for (;;) {
+            rb = 1;
+            while (rb * 2 <= cap && p0 + rb * 2 <= R) rb *= 2;
+            if (r < p0 + rb || p0 + rb >= R) break;
+            p0 += rb;
+        }
+        float2* S = Sall + (size_t)nleaf * NCUR * p0 * Q + ((size_t)leaf * rb + (r - p0)) * NCUR * Q +
+                    (size_t)cur * Q;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float re = acc[ct][v];
+            const float im = __shfl_xor(re, 8, 64);
+            const int qq = (qg * 4 + wave) * 16 + 4 * lk + v;
+            if (part == 0 && c < ncol && qq < Q) S[qq] = make_float2(re, im);
+        }
+    }
+}
+
+void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                          const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
+                          const double2* x, float2* Sall) {
+    if (nleaf == 0 || R < 1) return;
+    const int ncol = ncur * R;
+    const int ct = ncol > 32 ? 8 : (ncol > 16 ? 4 : (ncol > 8 ? 2 : 1));
+    const int QG = Qp / 64, ncg = (ncol + 8 * ct - 1) / (8 * ct);
+    const unsigned grid = xcd_chunk_grid(nleaf * QG, QG, ncg);
+#define AB(NC, CT)                                                                                                    \
+    k_aggregate_bf3<NC, CT><<<grid, 256, 0, st>>>(nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, x, Sall)
+#define ABN(NC)                                                                                                       \
+    if (ct == 8) AB(NC, 8);                                                                                           \
+    else if (ct == 4) AB(NC, 4);                                                                                      \
+    else if (ct == 2) AB(NC, 2);                                                                                      \
+    else AB(NC, 1);
+    if (ncur == 1) {
+        ABN(1)
+    } else {
+        ABN(2)
+    }
+#undef ABN
+#undef AB
+}
+
 #define MLFMA_DISAGG_MFMA_INSTANTIATE(T)                                                                            \
     template void launch_beff<T>(hipStream_t, int, int, int, bool, const cplx_t<T>*, const T*, double2, double2,   \
                                  double2, double2, cplx_t<T>*);                                                     \

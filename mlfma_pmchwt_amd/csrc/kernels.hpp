@@ -153,6 +153,15 @@ void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, in
 // Sall + nleaf ncur r0 3K).
 void launch_aggregate_all(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, const int* leaf_ptr,
                           const int* perm, const float2* Vs, const double2* x, float2* Sall);
+// The same product on split-pair BF16 from transposed pattern planes
+// [hi Re, hi Im, lo Re, lo Im][Qp][Np] (launch_split_cols; Qp a multiple of
+// 64, leaf t's RWGs from the 32-aligned column joff[t] on, jmap[column] = RWG
+// or -1).
+void launch_split_cols(hipStream_t st, int Q, int Qp, int Np, const int* jmap, const float2* V,
+                       unsigned short* planes);
+void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                          const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
+                          const double2* x, float2* Sall);
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
