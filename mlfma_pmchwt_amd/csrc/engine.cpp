@@ -1412,6 +1412,21 @@ void Engine::bs_start(BlockState& s, const mlfma_solve_params& sp, const double2
         bZf_cols_ = m;
         bZf_nR_ = nR;
     }
+    // FP32-mode solves keep the Krylov bases in FP32 (half the traffic of the
+    // Gram-Schmidt passes); the reductions, W and the solution stay FP64
+    s.vf = apply_prec == MLFMA_FP32;
+    if (s.vf && (bVf_cols_ < m + 1 || bVf_nR_ != nR)) {
+        dfree(d_bVf_);
+        d_bVf_ = dalloc<float2>(nR * (m + 1));
+        bVf_cols_ = m + 1;
+        bVf_nR_ = nR;
+    }
+    if (s.vf && s.pc && (bZff_cols_ < m || bZff_nR_ != nR)) {
+        dfree(d_bZff_);
+        d_bZff_ = dalloc<float2>(nR * m);
+        bZff_cols_ = m;
+        bZff_nR_ = nR;
+    }
     if (s.pkind == 2) ensure_poly(R);
     const double eps = 1e-16;
     s.iters.assign(R, 0);
@@ -1447,7 +1462,8 @@ void Engine::bs_cycle_begin(BlockState& s) {
     std::vector<double> sc(R);
     for (int r = 0; r < R; ++r) sc[r] = (s.done[r] || s.rnorm[r] == 0.0) ? 0.0 : 1.0 / s.rnorm[r];
     bs_put_scale(sc);
-    launch_scale_copy_b(st, s.n, R, d_bscale_, d_bRr_, d_bV_);
+    if (s.vf) launch_scale_copy_b(st, s.n, R, d_bscale_, d_bRr_, d_bVf_);
+    else launch_scale_copy_b(st, s.n, R, d_bscale_, d_bRr_, d_bV_);
     std::fill(s.H.begin(), s.H.end(), cplx(0, 0));
     std::fill(s.sv.begin(), s.sv.end(), cplx(0, 0));
     s.live.assign(R, 0);
@@ -1473,6 +1489,23 @@ int Engine::bs_graph_kind(const BlockState& s) const { return s.pc && s.pkind !=
 void Engine::bs_pre(BlockState& s, bool recorded) {
     ++s.total;
     const size_t nR = s.nR;
+    if (s.vf) {   // FP32 bases: v_k widened into G; z_k = M^-1 v_k rounded into the FP32 basis afterwards
+        launch_widen(st, nR, d_bVf_ + (size_t)s.k * nR, d_bG_);
+        if (s.pc && (s.pkind == 3 || !recorded)) {
+            precond_block(s.pkind, d_bG_, d_bZ_, s.R, apply_prec);
+            launch_narrow(st, nR, d_bZ_, d_bZff_ + (size_t)s.k * nR);
+            if (recorded) {
+                MLFMA_HIP(hipMemcpyAsync(d_bG_, d_bZ_, nR * sizeof(double2), hipMemcpyDeviceToDevice, st));
+            } else {
+                apply_block(d_bZ_, d_bW_, s.R, apply_prec);
+            }
+        } else if (s.pc && s.pkind == 2) {
+            ensure_poly(s.R);
+        } else if (!recorded) {
+            apply_block(d_bG_, d_bW_, s.R, apply_prec);
+        }
+        return;
+    }
     double2* Vk = d_bV_ + (size_t)s.k * nR;
     double2* Zk = s.pc ? d_bZf_ + (size_t)s.k * nR : nullptr;
     if (recorded) {
@@ -1501,9 +1534,13 @@ void Engine::bs_body(BlockState& s) {
 }
 
 void Engine::bs_post(BlockState& s, bool recorded) {
-    if (recorded && s.pc && s.pkind != 3)
-        MLFMA_HIP(hipMemcpyAsync(d_bZf_ + (size_t)s.k * s.nR, d_bZ_, s.nR * sizeof(double2), hipMemcpyDeviceToDevice,
-                                 st));
+    if (recorded && s.pc && s.pkind != 3) {
+        if (s.vf)
+            launch_narrow(st, s.nR, d_bZ_, d_bZff_ + (size_t)s.k * s.nR);
+        else
+            MLFMA_HIP(hipMemcpyAsync(d_bZf_ + (size_t)s.k * s.nR, d_bZ_, s.nR * sizeof(double2),
+                                     hipMemcpyDeviceToDevice, st));
+    }
     s.matvecs += s.R;
 }
 
@@ -1519,10 +1556,12 @@ void Engine::bs_orth(BlockState& s, int phase) {
     const size_t nh = (size_t)(k + 1) * R;
     if (phase == 0) {
         for (int pass = 0; pass < 2; ++pass) {
-            launch_multidot_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bW_, d_bpart_, kRedBlocks);
+            if (s.vf) launch_multidot_b(st, s.n, R, k + 1, d_bVf_, s.nR, d_bW_, d_bpart_, kRedBlocks);
+            else launch_multidot_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bW_, d_bpart_, kRedBlocks);
             launch_reduce_partials_b(st, k + 1, R, kRedBlocks, d_bpart_, d_bsmall_);
             MLFMA_HIP(hipMemcpyAsync(h_get_ + pass * nh, d_bsmall_, nh * sizeof(double2), hipMemcpyDeviceToHost, st));
-            launch_sub_combination_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bsmall_, d_bW_);
+            if (s.vf) launch_sub_combination_b(st, s.n, R, k + 1, d_bVf_, s.nR, d_bsmall_, d_bW_);
+            else launch_sub_combination_b(st, s.n, R, k + 1, d_bV_, s.nR, d_bsmall_, d_bW_);
         }
         launch_norm2_partial_b(st, s.n, R, d_bW_, d_bpart_, kRedBlocks);
         launch_reduce_partials_b(st, 1, R, kRedBlocks, d_bpart_, d_bsmall_);
@@ -1572,7 +1611,8 @@ void Engine::bs_orth(BlockState& s, int phase) {
         if (breakdown || s.err[r] < s.sp.tol_rel || s.err[r] < s.sp.tol_abs) s.live[r] = 0;
     }
     bs_put_scale(inv);
-    launch_scale_copy_b(st, s.n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * s.nR);
+    if (s.vf) launch_scale_copy_b(st, s.n, R, d_bscale_, d_bW_, d_bVf_ + (size_t)(k + 1) * s.nR);
+    else launch_scale_copy_b(st, s.n, R, d_bscale_, d_bW_, d_bV_ + (size_t)(k + 1) * s.nR);
     ++s.k;
 }
 
@@ -1602,9 +1642,11 @@ void Engine::bs_cycle_end(BlockState& s, bool compaction) {
     if (kmax > 0) {
         bs_put(yv, d_bsmall_);
         if (s.pc) {   // x += Z y with the kept z_k = M^-1 v_k
-            launch_add_combination_b(st, n, R, kmax, d_bZf_, s.nR, d_bsmall_, s.X);
+            if (s.vf) launch_add_combination_b(st, n, R, kmax, d_bZff_, s.nR, d_bsmall_, s.X);
+            else launch_add_combination_b(st, n, R, kmax, d_bZf_, s.nR, d_bsmall_, s.X);
         } else {
-            launch_add_combination_b(st, n, R, kmax, d_bV_, s.nR, d_bsmall_, s.X);
+            if (s.vf) launch_add_combination_b(st, n, R, kmax, d_bVf_, s.nR, d_bsmall_, s.X);
+            else launch_add_combination_b(st, n, R, kmax, d_bV_, s.nR, d_bsmall_, s.X);
         }
     }
     const std::vector<double> err_prev = s.err_cycle;   // true residuals at the start of this cycle

@@ -89,6 +89,7 @@ struct BlockState {
     size_t nR = 0;
     int pkind = 0;        // 0 none, 1 block-diagonal, 2 polynomial, 3 inner near-field solve
     bool pc = false;
+    bool vf = false;      // Krylov bases in FP32 (FP32-mode solves)
     int matvecs = 0, fp64_matvecs = 0, total = 0, cyc = 0, k = 0, fallbacks = 0;
     bool stopped = false;   // the divergence guard fired, or the block was handed on narrower
     double t0 = 0;
@@ -341,6 +342,10 @@ private:
     double2* d_bZf_ = nullptr;   // same for the batched solver (n R x bZf_cols_)
     int bZf_cols_ = 0;
     size_t bZf_nR_ = 0;
+    float2* d_bVf_ = nullptr;    // FP32 Krylov bases of FP32-mode batched solves
+    float2* d_bZff_ = nullptr;
+    int bVf_cols_ = 0, bZff_cols_ = 0;
+    size_t bVf_nR_ = 0, bZff_nR_ = 0;
     double2* d_pt_ = nullptr;
     double2* d_pu_ = nullptr;
     int poly_cap_ = 0;

@@ -149,18 +149,20 @@ __device__ __forceinline__ void fold_rows(double2 v, int R, double2* red) {
 }  // namespace
 
 // partial[(j * nblk + b) * R + r] = sum over block b of conj(V_j[u][r]) w[u][r]
-__global__ __launch_bounds__(256) void k_multidot_b(int n, int R, const double2* __restrict__ V, size_t ldv,
+// (V in FP64 or, for FP32-mode solves, FP32: widened exactly before the FP64 sums)
+template <typename TV>
+__global__ __launch_bounds__(256) void k_multidot_b(int n, int R, const TV* __restrict__ V, size_t ldv,
                                                     const double2* __restrict__ w, double2* __restrict__ partial) {
     __shared__ double2 red[256];
     const int j = blockIdx.y;
     const int rows = blockDim.x / R;
     const int r = threadIdx.x % R, q = threadIdx.x / R;
-    const double2* v = V + (size_t)j * ldv;
+    const TV* v = V + (size_t)j * ldv;
     double2 s = make_double2(0, 0);
     if (q < rows)
         for (int u = blockIdx.x * rows + q; u < n; u += gridDim.x * rows) {
             const size_t e = (size_t)u * R + r;
-            s = cadd(s, cmulc(v[e], w[e]));
+            s = cadd(s, cmulc(widen(v[e]), w[e]));
         }
     fold_rows(s, R, red);
     if ((int)threadIdx.x < R) partial[((size_t)j * gridDim.x + blockIdx.x) * R + threadIdx.x] = red[threadIdx.x];
@@ -168,7 +170,12 @@ __global__ __launch_bounds__(256) void k_multidot_b(int n, int R, const double2*
 
 void launch_multidot_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* w,
                        double2* partial, int nblk) {
-    k_multidot_b<<<dim3(nblk, m), 256, 0, st>>>(n, R, V, ldv, w, partial);
+    k_multidot_b<double2><<<dim3(nblk, m), 256, 0, st>>>(n, R, V, ldv, w, partial);
+}
+
+void launch_multidot_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* w,
+                       double2* partial, int nblk) {
+    k_multidot_b<float2><<<dim3(nblk, m), 256, 0, st>>>(n, R, V, ldv, w, partial);
 }
 
 // out[j * R + r] = sum_b partial[(j * nblk + b) * R + r], b ascending
@@ -206,51 +213,92 @@ void launch_norm2_partial_b(hipStream_t st, int n, int R, const double2* x, doub
 }
 
 // w[u][r] -= sum_j h[j * R + r] V_j[u][r]
-__global__ __launch_bounds__(256) void k_sub_combination_b(size_t nR, int R, int m, const double2* __restrict__ V,
+template <typename TV>
+__global__ __launch_bounds__(256) void k_sub_combination_b(size_t nR, int R, int m, const TV* __restrict__ V,
                                                            size_t ldv, const double2* __restrict__ h,
                                                            double2* __restrict__ w) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nR) return;
     const int r = (int)(e % R);
     double2 a = w[e];
-    for (int j = 0; j < m; ++j) cfms(a, h[(size_t)j * R + r], V[(size_t)j * ldv + e]);
+    for (int j = 0; j < m; ++j) cfms(a, h[(size_t)j * R + r], widen(V[(size_t)j * ldv + e]));
     w[e] = a;
+}
+
+template <typename TV>
+static void sub_combination_b(hipStream_t st, int n, int R, int m, const TV* V, size_t ldv, const double2* h,
+                              double2* w) {
+    const size_t nR = (size_t)n * R;
+    k_sub_combination_b<TV><<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, m, V, ldv, h, w);
 }
 
 void launch_sub_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* h,
                               double2* w) {
-    const size_t nR = (size_t)n * R;
-    k_sub_combination_b<<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, m, V, ldv, h, w);
+    sub_combination_b(st, n, R, m, V, ldv, h, w);
+}
+
+void launch_sub_combination_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* h,
+                              double2* w) {
+    sub_combination_b(st, n, R, m, V, ldv, h, w);
 }
 
 // x[u][r] += sum_j y[j * R + r] V_j[u][r]
-__global__ __launch_bounds__(256) void k_add_combination_b(size_t nR, int R, int m, const double2* __restrict__ V,
+template <typename TV>
+__global__ __launch_bounds__(256) void k_add_combination_b(size_t nR, int R, int m, const TV* __restrict__ V,
                                                            size_t ldv, const double2* __restrict__ y,
                                                            double2* __restrict__ x) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nR) return;
     const int r = (int)(e % R);
     double2 a = x[e];
-    for (int j = 0; j < m; ++j) cfma(a, y[(size_t)j * R + r], V[(size_t)j * ldv + e]);
+    for (int j = 0; j < m; ++j) cfma(a, y[(size_t)j * R + r], widen(V[(size_t)j * ldv + e]));
     x[e] = a;
+}
+
+template <typename TV>
+static void add_combination_b(hipStream_t st, int n, int R, int m, const TV* V, size_t ldv, const double2* y,
+                              double2* x) {
+    const size_t nR = (size_t)n * R;
+    k_add_combination_b<TV><<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, m, V, ldv, y, x);
 }
 
 void launch_add_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* y,
                               double2* x) {
-    const size_t nR = (size_t)n * R;
-    k_add_combination_b<<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, m, V, ldv, y, x);
+    add_combination_b(st, n, R, m, V, ldv, y, x);
 }
 
-// y[u][r] = a[r] * x[u][r]  (a real, one per right-hand side)
+void launch_add_combination_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* y,
+                              double2* x) {
+    add_combination_b(st, n, R, m, V, ldv, y, x);
+}
+
+// y[u][r] = a[r] * x[u][r]  (a real, one per right-hand side), y in FP64 or FP32
+template <typename TY>
 __global__ void k_scale_copy_b(size_t nR, int R, const double* __restrict__ a, const double2* __restrict__ x,
-                               double2* __restrict__ y) {
+                               TY* __restrict__ y) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nR) y[e] = cscale(x[e], a[e % R]);
+    if (e < nR) narrow(cscale(x[e], a[e % R]), y[e]);
 }
 
 void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, double2* y) {
     const size_t nR = (size_t)n * R;
-    k_scale_copy_b<<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, a, x, y);
+    k_scale_copy_b<double2><<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, a, x, y);
+}
+
+void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, float2* y) {
+    const size_t nR = (size_t)n * R;
+    k_scale_copy_b<float2><<<(unsigned)((nR + 255) / 256), 256, 0, st>>>(nR, R, a, x, y);
+}
+
+// FP32 -> FP64 (exact)
+__global__ void k_widen(size_t n, const float2* __restrict__ in, double2* __restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = widen(in[e]);
+}
+
+void launch_widen(hipStream_t st, size_t n, const float2* in, double2* out) {
+    if (n == 0) return;
+    k_widen<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, in, out);
 }
 
 // column-major (vector r at in + r * n) <-> RHS-fastest (element (u, r) at u * R + r)

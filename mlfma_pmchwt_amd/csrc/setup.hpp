@@ -71,6 +71,16 @@ void launch_sub_combination_b(hipStream_t st, int n, int R, int m, const double2
 void launch_add_combination_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* y,
                               double2* x);
 void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, double2* y);
+// the same with an FP32 basis V (FP32-mode batched solves): V is widened exactly
+// and the sums stay in FP64; scale_copy rounds its output to FP32
+void launch_multidot_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* w,
+                       double2* partial, int nblk);
+void launch_sub_combination_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* h,
+                              double2* w);
+void launch_add_combination_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* y,
+                              double2* x);
+void launch_scale_copy_b(hipStream_t st, int n, int R, const double* a, const double2* x, float2* y);
+void launch_widen(hipStream_t st, size_t n, const float2* in, double2* out);
 // out (cols x rows) = transpose of in (rows x cols), both row-major
 void launch_transpose(hipStream_t st, int rows, int cols, const double2* in, double2* out);
 void launch_gather_cols(hipStream_t st, int n, int R, int Rc, const int* idx, const double2* in, double2* out);
