@@ -493,6 +493,10 @@ void Engine::build_near() {
     }
     d_dinv_ = dalloc<double2>(nunk);
     launch_near_diag_inv(st, nd_, d_Z_, d_dinv_);
+    if (d_Zbf_) {   // the FP32-vector polynomial's diagonal
+        d_dinvf_ = dalloc<float2>(nunk);
+        launch_narrow(st, nunk, d_dinv_, d_dinvf_);
+    }
     MLFMA_HIP(hipGetLastError());
 }
 
@@ -714,7 +718,30 @@ void Engine::apply_one(const double2* x, double2* y, int prec) { apply_pass(x, y
 // Y = A X for R right-hand sides, RHS-fastest (element (u, r) at u * R + r).
 // With the FP32 near field the near product of all R runs first on the
 // matrix cores; the far field follows in passes of rb right-hand sides.
-void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
+// Whether apply_block can read an FP32 input directly: the matrix-core near
+// product of split pairs and the all-RHS split-pair aggregation are the only
+// readers of X on that path.
+bool Engine::fp32_input_ok(int R, int prec) const {
+    if (!(prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32 && R >= 2 && near_mfma_ && disagg_mfma_ && d_Zbs_))
+        return false;
+    for (int w = 0; w < nw; ++w)
+        if (!d_Va_[w]) return false;
+    return true;
+}
+
+// Y = A X from an FP32 X (the FP32 Krylov bases): read directly where every
+// reader of X takes FP32, else widened into scratch (nunk x R FP64) first.
+void Engine::apply_block_f(const float2* X, double2* Y, int R, int prec, double2* scratch) {
+    if (fp32_input_ok(R, prec)) {
+        apply_block(nullptr, Y, R, prec, X);
+    } else {
+        launch_widen(st, (size_t)nunk * R, X, scratch);
+        apply_block(scratch, Y, R, prec);
+    }
+}
+
+void Engine::apply_block(const double2* X, double2* Y, int R, int prec, const float2* Xf) {
+    if (Xf && !fp32_input_ok(R, prec)) throw Error(MLFMA_ERR_STATE, "FP32 matvec input on a path that reads FP64");
     const bool f32 = prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32;
     int cap = f32 ? rb_max<float>() : rb_max<double>();
     if (rb_tuned_[f32 ? 1 : 0] > 0) cap = std::min(cap, rb_tuned_[f32 ? 1 : 0]);
@@ -723,7 +750,8 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
         if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
         if (prec == MLFMA_FP32 && d_Zbs_) {
             ensure_xf(R);
-            launch_near_bf16x3(st, R, R, 0, nd_, d_Zbs_, d_bsoff_, X, reinterpret_cast<uint2*>(d_xf_), Y);
+            if (Xf) launch_near_bf16x3(st, R, R, 0, nd_, d_Zbs_, d_bsoff_, Xf, reinterpret_cast<uint2*>(d_xf_), Y);
+            else launch_near_bf16x3(st, R, R, 0, nd_, d_Zbs_, d_bsoff_, X, reinterpret_cast<uint2*>(d_xf_), Y);
         } else if (prec == MLFMA_FP32 && d_Z32_) {
             ensure_xf(R);
             launch_near_mfma(st, R, R, 0, nd_, d_Z32_, X, d_xf_, Y);
@@ -762,7 +790,10 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec) {
         const int L = tree.max_level;
         timed(0, nw, [&] {
             for (int w = 0; w < nw; ++w) {
-                if (d_Va_[w])   // split-pair BF16 (near_precision = MLFMA_BF16)
+                if (d_Va_[w] && Xf)   // split-pair BF16 (near_precision = MLFMA_BF16)
+                    launch_aggregate_bf3(st, ncur, nleaf, wd_[w].K[L], N, R, cap, Qa_[w], Np_, d_leaf_ptr_, d_joff_,
+                                         d_perm_, d_Va_[w], Xf, d_Sall_[w]);
+                else if (d_Va_[w])
                     launch_aggregate_bf3(st, ncur, nleaf, wd_[w].K[L], N, R, cap, Qa_[w], Np_, d_leaf_ptr_, d_joff_,
                                          d_perm_, d_Va_[w], X, d_Sall_[w]);
                 else
@@ -1489,6 +1520,18 @@ int Engine::bs_graph_kind(const BlockState& s) const { return s.pc && s.pkind !=
 void Engine::bs_pre(BlockState& s, bool recorded) {
     ++s.total;
     const size_t nR = s.nR;
+    if (poly_f32(s)) {   // FP32 bases and the FP32-vector polynomial: no FP64 copy of v_k or z_k
+        const float2* Vk = d_bVf_ + (size_t)s.k * nR;
+        ensure_poly(s.R);
+        if (recorded) {
+            MLFMA_HIP(hipMemcpyAsync(d_bG_, Vk, nR * sizeof(float2), hipMemcpyDeviceToDevice, st));
+        } else {
+            float2* Zk = d_bZff_ + (size_t)s.k * nR;
+            poly_apply_bf16f(Vk, Zk, s.R);
+            apply_block_f(Zk, d_bW_, s.R, apply_prec, d_bG_);
+        }
+        return;
+    }
     if (s.vf) {   // FP32 bases: v_k widened into G; z_k = M^-1 v_k rounded into the FP32 basis afterwards
         launch_widen(st, nR, d_bVf_ + (size_t)s.k * nR, d_bG_);
         if (s.pc && (s.pkind == 3 || !recorded)) {
@@ -1525,6 +1568,13 @@ void Engine::bs_pre(BlockState& s, bool recorded) {
 }
 
 void Engine::bs_body(BlockState& s) {
+    if (poly_f32(s)) {   // FP32 v_k and z_k in the first halves of G and Z (G is then free as scratch)
+        float2* Gf = reinterpret_cast<float2*>(d_bG_);
+        float2* Zf = reinterpret_cast<float2*>(d_bZ_);
+        poly_apply_bf16f(Gf, Zf, s.R);
+        apply_block_f(Zf, d_bW_, s.R, apply_prec, d_bG_);
+        return;
+    }
     if (s.pc && s.pkind != 3) {
         precond_block(s.pkind, d_bG_, d_bZ_, s.R, apply_prec);
         apply_block(d_bZ_, d_bW_, s.R, apply_prec);
@@ -1534,7 +1584,10 @@ void Engine::bs_body(BlockState& s) {
 }
 
 void Engine::bs_post(BlockState& s, bool recorded) {
-    if (recorded && s.pc && s.pkind != 3) {
+    if (recorded && poly_f32(s)) {
+        MLFMA_HIP(hipMemcpyAsync(d_bZff_ + (size_t)s.k * s.nR, d_bZ_, s.nR * sizeof(float2), hipMemcpyDeviceToDevice,
+                                 st));
+    } else if (recorded && s.pc && s.pkind != 3) {
         if (s.vf)
             launch_narrow(st, s.nR, d_bZ_, d_bZff_ + (size_t)s.k * s.nR);
         else
@@ -2080,6 +2133,9 @@ void Engine::ensure_xf(int R) {
     xf_cap_ = R;
 }
 
+// (the FP32-vector polynomial of launch_poly_init_bf16 / launch_near_bf16_poly
+// keeps its basis vector in d_pt_ and the second BF16 input buffer in d_pu_,
+// both large enough, as the FP64 polynomial does not run at the same time)
 void Engine::ensure_poly(int R) {
     ensure_xf(R);
     ensure_eall(R);
@@ -2133,6 +2189,25 @@ void Engine::poly_apply(const double2* X, double2* Y, int R, int prec) {
     for (size_t k = 1; k < poly_alpha_.size(); ++k) {
         near_block(d_pt_, d_pu_, R, prec, true);
         launch_poly_step(st, nunk, R, d_dinv_, d_pu_, d2(poly_alpha_[k]), d_pt_, Y);
+    }
+}
+
+// The same polynomial with FP32 vectors (x, y: nunk x R, RHS-fastest) around
+// the single-BF16 products: each product's epilogue takes the polynomial's
+// step and rounds the next input, so between products nothing else runs.
+// Allocates nothing (ensure_poly(R) first).
+void Engine::poly_apply_bf16f(const float2* X, float2* Y, int R) {
+    if (poly_alpha_.empty()) throw Error(MLFMA_ERR_STATE, "polynomial preconditioner coefficients not set");
+    if (!d_Zbf_ || !d_dinvf_ || R > poly_cap_)
+        throw Error(MLFMA_ERR_STATE, "FP32 polynomial: BF16 preconditioner records and sized buffers needed");
+    auto f2 = [](std::complex<double> a) { return make_float2((float)a.real(), (float)a.imag()); };
+    float2* t = reinterpret_cast<float2*>(d_pt_);
+    unsigned* xb[2] = {d_xb_, reinterpret_cast<unsigned*>(d_pu_)};
+    launch_poly_init_bf16(st, R, nd_, d_dinvf_, X, f2(poly_alpha_[0]), t, Y, xb[0]);
+    const size_t d = poly_alpha_.size() - 1;
+    for (size_t k = 1; k <= d; ++k) {
+        const PolyEpi pe{d_dinvf_, f2(poly_alpha_[k]), t, Y, k < d ? xb[k & 1] : nullptr};
+        launch_near_bf16_poly(st, R, nd_, d_Zbf_, d_bfoff_, xb[(k - 1) & 1], pe);
     }
 }
 

@@ -138,6 +138,10 @@ public:
     void bs_cycle_begin(BlockState& s);
     bool bs_step_open(const BlockState& s) const;
     int bs_graph_kind(const BlockState& s) const;
+    // FP32 bases with the polynomial preconditioner on the BF16 records: the
+    // polynomial runs in FP32 vectors (poly_apply_bf16f) and the matvec reads
+    // z_k in FP32
+    bool poly_f32(const BlockState& s) const { return s.vf && s.pc && s.pkind == 2 && d_Zbf_ != nullptr; }
     void bs_pre(BlockState& s, bool recorded);
     void bs_body(BlockState& s);
     void bs_post(BlockState& s, bool recorded);
@@ -166,6 +170,7 @@ public:
     void set_inner_steps(int k);
     void set_poly(const std::vector<cplx>& alpha);
     void poly_apply(const double2* X, double2* Y, int R, int prec);
+    void poly_apply_bf16f(const float2* X, float2* Y, int R);
     void precond_features(float* f);
     void poly_apply_device(const double2* x, double2* y, int nrhs);
     void precond_apply_kind(int kind, const double2* x, double2* y);
@@ -251,7 +256,9 @@ private:
     void mark(int slot);
     void collect_profile(int count);
     void apply_one(const double2* x, double2* y, int prec);
-    void apply_block(const double2* X, double2* Y, int R, int prec);
+    void apply_block(const double2* X, double2* Y, int R, int prec, const float2* Xf = nullptr);
+    void apply_block_f(const float2* X, double2* Y, int R, int prec, double2* scratch);
+    bool fp32_input_ok(int R, int prec) const;
     std::vector<double2> near_block(int p);   // planes x (ns x nt column-major) of pair p
     void precond_setup();
     void precond_apply(const double2* x, double2* y);
@@ -336,6 +343,7 @@ private:
     hipEvent_t ev_[18] = {};   // 16..17: the all-RHS near product of a batched matvec
     // block-diagonal preconditioner
     double2* d_dinv_ = nullptr;   // 1 / diagonal of the operator, caller order
+    float2* d_dinvf_ = nullptr;   // its FP32 copy (with the BF16 preconditioner records)
     std::vector<cplx> poly_alpha_;
     double2* d_Zf_ = nullptr;    // flexible GMRES: preconditioned basis z_k = M^-1 v_k (n x Zf_cols_)
     int Zf_cols_ = 0;

@@ -744,12 +744,12 @@ void launch_split_cols(hipStream_t st, int Q, int Qp, int Np, const int* jmap, c
     k_split_cols<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(Q, Qp, Np, jmap, V, planes);
 }
 
-template <int NCUR, int CT>
+template <int NCUR, int CT, typename TX>
 __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, int R, int cap, int Qp, int Np,
                                                        const int* __restrict__ leaf_ptr,
                                                        const int* __restrict__ joff, const int* __restrict__ perm,
                                                        const unsigned short* __restrict__ planes,
-                                                       const double2* __restrict__ x, float2* __restrict__ Sall) {
+                                                       const TX* __restrict__ x, float2* __restrict__ Sall) {
     constexpr int CC = 8 * CT;            // complex columns per workgroup
     constexpr int kJ = 32, kP = kJ + 8;   // staged RWGs per block, padded LDS row (80 B)
     constexpr int NE = kJ * CC, NU = (NE + 255) / 256;
@@ -779,7 +779,7 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
 #pragma unroll
             for (int p = 0; p < 4; ++p) a[p] = *reinterpret_cast<const uint4*>(arow + p * plane + kc);
         };
-        double2 px[NU];
+        TX px[NU];
         unsigned pmask = 0;
         auto load_x = [&](int kb) {
             pmask = 0;
@@ -801,8 +801,8 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
                 const int e = (int)threadIdx.x + 256 * u;
                 if (NE % 256 == 0 || e < NE) {
                     const int jj = e / CC, c = e - jj * CC;
-                    const double2 v = ((pmask >> u) & 1u) ? px[u] : make_double2(0.0, 0.0);
-                    const float fr = (float)v.x, fi = (float)v.y;
+                    const bool on = (pmask >> u) & 1u;
+                    const float fr = on ? (float)px[u].x : 0.f, fi = on ? (float)px[u].y : 0.f;
                     const unsigned hr = bf16b(fr), hi = bf16b(fi);
                     xs[0][0][c][jj] = (unsigned short)hr;
                     xs[0][1][c][jj] = (unsigned short)hi;
@@ -871,16 +871,18 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
     }
 }
 
-void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
-                          const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
-                          const double2* x, float2* Sall) {
+template <typename TX>
+static void aggregate_bf3_t(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                            const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
+                            const TX* x, float2* Sall) {
     if (nleaf == 0 || R < 1) return;
     const int ncol = ncur * R;
     const int ct = ncol > 32 ? 8 : (ncol > 16 ? 4 : (ncol > 8 ? 2 : 1));
     const int QG = Qp / 64, ncg = (ncol + 8 * ct - 1) / (8 * ct);
     const unsigned grid = xcd_chunk_grid(nleaf * QG, QG, ncg);
 #define AB(NC, CT)                                                                                                    \
-    k_aggregate_bf3<NC, CT><<<grid, 256, 0, st>>>(nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, x, Sall)
+    k_aggregate_bf3<NC, CT, TX><<<grid, 256, 0, st>>>(nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, x, \
+                                                      Sall)
 #define ABN(NC)                                                                                                       \
     if (ct == 8) AB(NC, 8);                                                                                           \
     else if (ct == 4) AB(NC, 4);                                                                                      \
@@ -893,6 +895,18 @@ void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int
     }
 #undef ABN
 #undef AB
+}
+
+void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                          const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
+                          const double2* x, float2* Sall) {
+    aggregate_bf3_t(st, ncur, nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, x, Sall);
+}
+
+void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                          const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
+                          const float2* x, float2* Sall) {
+    aggregate_bf3_t(st, ncur, nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, x, Sall);
 }
 
 #define MLFMA_DISAGG_MFMA_INSTANTIATE(T)                                                                            \

@@ -110,17 +110,32 @@ __global__ void k_x_bf16(size_t total, int R, int ldr, int r0, const double2* __
     xb[e] = bf16_bits((float)v.x) | (bf16_bits((float)v.y) << 16);
 }
 
-// split pairs of x: .x = (hi Re, hi Im), .y = (lo Re, lo Im) of the FP32 value
-__global__ void k_x_bf16x2(size_t total, int R, int ldr, int r0, const double2* __restrict__ x,
-                           uint2* __restrict__ xb) {
+// split pairs of x (FP64 or FP32 input): .x = (hi Re, hi Im), .y = (lo Re, lo
+// Im) of the FP32 value
+template <typename TX>
+__global__ void k_x_bf16x2(size_t total, int R, int ldr, int r0, const TX* __restrict__ x, uint2* __restrict__ xb) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
     const size_t u = e / R;
     const int r = (int)(e - u * R);
-    const double2 v = x[u * ldr + r0 + r];
+    const TX v = x[u * ldr + r0 + r];
     const float fr = (float)v.x, fi = (float)v.y;
     const unsigned hr = bf16_bits(fr), hi = bf16_bits(fi);
     xb[e] = make_uint2(hr | (hi << 16), bf16_bits(fr - bf16_float(hr)) | (bf16_bits(fi - bf16_float(hi)) << 16));
+}
+
+// Polynomial preconditioner in FP32 vectors around the single-BF16 products
+// (t_0 = D^-1 x, y = a_0 t_0, then per product t_k = t_{k-1} - D^-1 Z t_{k-1},
+// y += a_k t_k): the first step, which also rounds t_0 to BF16 for the first
+// product.  x, t, y: nu x R, right-hand side fastest.
+__global__ void k_poly_init_bf16(size_t total, int R, const float2* __restrict__ dinv, const float2* __restrict__ x,
+                                 float2 a0, float2* __restrict__ t, float2* __restrict__ y, unsigned* __restrict__ xb) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const float2 v = cmul(dinv[e / R], x[e]);
+    t[e] = v;
+    y[e] = cmul(a0, v);
+    xb[e] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
 }
 
 // A workgroup owns one work item (a target leaf's range of up to 4 16-row
@@ -139,10 +154,18 @@ __global__ void k_x_bf16x2(size_t total, int R, int ldr, int r0, const double2* 
 // MLFMA_BF16): Z = Zh + Zl and X = Xh + Xl in BF16, and each product is
 // Zh Xh + Zh Xl + Zl Xh (the dropped Zl Xl and the representation error are
 // about 2^-17 of |Z||X|), three matrix-core products per record.
-template <int P, int CT, bool S>
+//
+// F (single BF16, the polynomial preconditioner's products in FP32 vectors):
+// instead of storing Z X, the epilogue takes the polynomial's step for its
+// rows, t -= D^-1 (Z X), y += a t, and rounds the new t to BF16 into the
+// next product's input (pe.xbn, a buffer other than the one read here, since
+// other workgroups still gather from it); every (row, right-hand side) has one
+// owner, so t and y are updated without races.
+template <int P, int CT, bool S, bool F>
 __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, int r0, const uint4* __restrict__ Zb,
                                                    const long long* __restrict__ bf_off,
-                                                   const void* __restrict__ xbv, double2* __restrict__ y) {
+                                                   const void* __restrict__ xbv, double2* __restrict__ y,
+                                                   PolyEpi pe) {
     constexpr int NX = (P == 3) ? 2 : 1;   // staged currents
     constexpr int NO = (P == 3) ? 2 : 1;   // outputs
     constexpr int RT = 8 * CT;             // right-hand sides per workgroup
@@ -323,7 +346,18 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
                 const int rc = ct * 8 + li;
                 if (part == 0 && rc < rv && row < nt) {
                     const size_t u = (size_t)nd.perm[i0 + row] + (o ? (size_t)nd.N : 0);
-                    y[u * ldr + r0 + rb0 + rc] = make_double2(re, im);
+                    const size_t e = u * ldr + r0 + rb0 + rc;
+                    if constexpr (F) {
+                        float2 v = pe.t[e];
+                        cfms(v, pe.dinv[u], make_float2(re, im));
+                        pe.t[e] = v;
+                        float2 w = pe.y[e];
+                        cfma(w, pe.a, v);
+                        pe.y[e] = w;
+                        if (pe.xbn) pe.xbn[e] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+                    } else {
+                        y[e] = make_double2(re, im);
+                    }
                 }
             }
 }
@@ -347,17 +381,15 @@ void launch_pack_bf16(hipStream_t st, const NearDev& nd, const double2* Z, const
     else k_pack_bf16<false><<<nd.nleaf, 256, 0, st>>>(nd, Z, bf_off, out);
 }
 
-template <bool S>
-static void near_bf16_t(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
-                        const long long* bf_off, const double2* x, void* xb, double2* y) {
-    if (nd.nitem == 0 || R < 1) return;
-    const size_t total = (size_t)nd.N * (nd.planes == 3 ? 2 : 1) * R;   // unknowns x right-hand sides
-    if (S) k_x_bf16x2<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, (uint2*)xb);
-    else k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, (unsigned*)xb);
+// the products of xb (already staged as BF16: single values, or split pairs
+// with S); F: the polynomial step of pe instead of y = Z X
+template <bool S, bool F>
+static void near_bf16_products(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                               const long long* bf_off, const void* xb, double2* y, const PolyEpi& pe) {
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     const int ny = (R + 8 * ct - 1) / (8 * ct);
     const unsigned grid = xcd_chunk_grid(nd.nitem, 4, ny);
-#define NB(P, CT) k_near_bf16<P, CT, S><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y)
+#define NB(P, CT) k_near_bf16<P, CT, S, F><<<grid, 256, 0, st>>>(nd, R, ldr, r0, Zb, bf_off, xb, y, pe)
 #define NBP(P)                                                                                                        \
     if (ct == 4) NB(P, 4);                                                                                            \
     else if (ct == 2) NB(P, 2);                                                                                       \
@@ -371,14 +403,46 @@ static void near_bf16_t(hipStream_t st, int R, int ldr, int r0, const NearDev& n
 #undef NB
 }
 
+static size_t near_unknowns(const NearDev& nd) { return (size_t)nd.N * (nd.planes == 3 ? 2 : 1); }
+
 void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
                       const long long* bf_off, const double2* x, unsigned* xb, double2* y) {
-    near_bf16_t<false>(st, R, ldr, r0, nd, Zb, bf_off, x, xb, y);
+    if (nd.nitem == 0 || R < 1) return;
+    const size_t total = near_unknowns(nd) * R;
+    k_x_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
+    near_bf16_products<false, false>(st, R, ldr, r0, nd, Zb, bf_off, xb, y, PolyEpi{});
+}
+
+template <typename TX>
+static void near_bf16x3_t(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                          const long long* bf_off, const TX* x, uint2* xb, double2* y) {
+    if (nd.nitem == 0 || R < 1) return;
+    const size_t total = near_unknowns(nd) * R;
+    k_x_bf16x2<TX><<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, ldr, r0, x, xb);
+    near_bf16_products<true, false>(st, R, ldr, r0, nd, Zb, bf_off, xb, y, PolyEpi{});
 }
 
 void launch_near_bf16x3(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
                         const long long* bf_off, const double2* x, uint2* xb, double2* y) {
-    near_bf16_t<true>(st, R, ldr, r0, nd, Zb, bf_off, x, xb, y);
+    near_bf16x3_t(st, R, ldr, r0, nd, Zb, bf_off, x, xb, y);
+}
+
+void launch_near_bf16x3(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                        const long long* bf_off, const float2* x, uint2* xb, double2* y) {
+    near_bf16x3_t(st, R, ldr, r0, nd, Zb, bf_off, x, xb, y);
+}
+
+void launch_poly_init_bf16(hipStream_t st, int R, const NearDev& nd, const float2* dinv, const float2* x, float2 a0,
+                           float2* t, float2* y, unsigned* xb) {
+    const size_t total = near_unknowns(nd) * R;
+    if (total == 0) return;
+    k_poly_init_bf16<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, R, dinv, x, a0, t, y, xb);
+}
+
+void launch_near_bf16_poly(hipStream_t st, int R, const NearDev& nd, const uint4* Zb, const long long* bf_off,
+                           const unsigned* xb, const PolyEpi& pe) {
+    if (nd.nitem == 0 || R < 1) return;
+    near_bf16_products<false, true>(st, R, R, 0, nd, Zb, bf_off, xb, nullptr, pe);
 }
 
 }  // namespace mlfma

@@ -162,6 +162,9 @@ void launch_split_cols(hipStream_t st, int Q, int Qp, int Np, const int* jmap, c
 void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
                           const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
                           const double2* x, float2* Sall);
+void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
+                          const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
+                          const float2* x, float2* Sall);
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
@@ -180,5 +183,24 @@ void launch_near_bf16(hipStream_t st, int R, int ldr, int r0, const NearDev& nd,
                       const long long* bf_off, const double2* x, unsigned* xb, double2* y);
 void launch_near_bf16x3(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
                         const long long* bf_off, const double2* x, uint2* xb, double2* y);
+void launch_near_bf16x3(hipStream_t st, int R, int ldr, int r0, const NearDev& nd, const uint4* Zb,
+                        const long long* bf_off, const float2* x, uint2* xb, double2* y);
+
+// The polynomial preconditioner with FP32 vectors around the single-BF16
+// products (all unknowns, R right-hand sides, stride R):
+//   t = D^-1 x, y = a0 t, xb = BF16(t)                       (launch_poly_init_bf16)
+//   t -= D^-1 Z xb, y += pe.a t, pe.xbn = BF16(t)            (launch_near_bf16_poly)
+// pe.xbn may be null (the last step) and must not be xb.
+struct PolyEpi {
+    const float2* dinv;   // 1 / diagonal, per unknown
+    float2 a;             // coefficient of this step
+    float2* t;            // current polynomial basis vector (updated)
+    float2* y;            // accumulated result (updated)
+    unsigned* xbn;        // BF16 of the new t, the next product's input
+};
+void launch_poly_init_bf16(hipStream_t st, int R, const NearDev& nd, const float2* dinv, const float2* x, float2 a0,
+                           float2* t, float2* y, unsigned* xb);
+void launch_near_bf16_poly(hipStream_t st, int R, const NearDev& nd, const uint4* Zb, const long long* bf_off,
+                           const unsigned* xb, const PolyEpi& pe);
 
 }  // namespace mlfma
