@@ -395,6 +395,28 @@ __device__ __forceinline__ floatx4 mfb(uint4 a, uint4 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
                                                    c, 0, 0, 0);
 }
+
+// Split pairs of 8 consecutive staged values (bit i of mask: value i is
+// staged, else zero), packed for 16-byte LDS stores: the BF16 hi parts of
+// Re and of Im, then the lo parts (lo = BF16 of what hi leaves of the FP32
+// value).
+template <typename V>
+__device__ __forceinline__ void split8(const V (&v)[8], unsigned mask, uint4 (&o)[4]) {
+    unsigned w[4][4] = {};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const bool on = (mask >> i) & 1u;
+        const float fr = on ? (float)v[i].x : 0.f, fi = on ? (float)v[i].y : 0.f;
+        const unsigned hr = bf16b(fr), hi = bf16b(fi);
+        const int sh = 16 * (i & 1);
+        w[0][i >> 1] |= hr << sh;
+        w[1][i >> 1] |= hi << sh;
+        w[2][i >> 1] |= bf16b(fr - bf16f(hr)) << sh;
+        w[3][i >> 1] |= bf16b(fi - bf16f(hi)) << sh;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) o[p] = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+}
 }  // namespace
 
 // planes[p][i][q] from V[i][q] (FP32 complex), q < Qp (zero past Q)
@@ -426,7 +448,11 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
                                                     const float2* __restrict__ E2, double2* __restrict__ y) {
     constexpr int RT = 8 * CT;
     constexpr int kQ = 32, kP = kQ + 8;   // staged q per chunk, padded LDS row (80 B)
-    constexpr int NE = NO * kQ * RT, NU = (NE + 255) / 256;
+    // staging units: a thread loads 8 consecutive q of one (output, rhs) and
+    // stores their split pairs with four 16-byte writes, which 8 consecutive
+    // lanes (consecutive rhs, rows 80 B apart) spread over all banks
+    constexpr int NUS = NO * (kQ / 8) * RT;
+    static_assert(NUS <= 256, "one staging unit per thread");
     __shared__ __align__(16) unsigned short es[2][NO][2][RT][kP];   // [hi, lo][output][Re, Im][rhs][q]
     // chunks of 4 items (mostly the row ranges of one leaf, which read the
     // same effective matrices) with their ny right-hand-side groups (which
@@ -451,7 +477,10 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
     for (int c = 0; c < CT; ++c)
 #pragma unroll
         for (int o = 0; o < NO; ++o) acc[c][o] = floatx4{0.f, 0.f, 0.f, 0.f};
-    float2 pe[NU];
+    const bool sact = NUS == 256 || (int)threadIdx.x < NUS;
+    const int su = min((int)threadIdx.x, NUS - 1);   // spare threads repeat the last unit (not stored)
+    const int sr = su % RT, sg = (su / RT) % (kQ / 8), so = su / (RT * (kQ / 8));
+    float2 pe[8];
     unsigned pmask = 0;
     for (int w = 0; w < nw; ++w) {
         const int Q = 3 * (w ? K2 : K1), Qp = (Q + 31) & ~31;
@@ -466,13 +495,12 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
         };
         auto load_e = [&](int q0) {
             pmask = 0;
+            const float2* Er = E + (size_t)so * Q * R + rb0 + min(sr, rv - 1);
 #pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int e = threadIdx.x + 256 * u, ec = min(e, NE - 1);
-                const int o = ec / (kQ * RT), rem = ec - o * (kQ * RT);
-                const int qq = rem / RT, r = rem - qq * RT;
-                pe[u] = E[((size_t)o * Q + min(q0 + qq, Q - 1)) * R + rb0 + min(r, rv - 1)];
-                pmask |= (unsigned)((NE % 256 == 0 || e < NE) & (q0 + qq < Q) & (r < rv)) << u;
+            for (int i = 0; i < 8; ++i) {
+                const int q = q0 + 8 * sg + i;
+                pe[i] = Er[(size_t)min(q, Q - 1) * R];
+                pmask |= (unsigned)((q < Q) & (sr < rv)) << i;
             }
         };
         // one 32-q chunk: stage Eall's (split when stored), then the next
@@ -481,19 +509,13 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
         // and the next patterns stay in flight during the products
         auto chunk = [&](int q0, const uint4* cur, uint4* nxt) {
             __syncthreads();
-#pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int e = threadIdx.x + 256 * u;
-                if (NE % 256 == 0 || e < NE) {
-                    const int o = e / (kQ * RT), rem = e - o * (kQ * RT);
-                    const int qq = rem / RT, r = rem - qq * RT;
-                    const float2 v = ((pmask >> u) & 1u) ? pe[u] : make_float2(0.f, 0.f);
-                    const unsigned hr = bf16b(v.x), hi = bf16b(v.y);
-                    es[0][o][0][r][qq] = (unsigned short)hr;
-                    es[0][o][1][r][qq] = (unsigned short)hi;
-                    es[1][o][0][r][qq] = (unsigned short)bf16b(v.x - bf16f(hr));
-                    es[1][o][1][r][qq] = (unsigned short)bf16b(v.y - bf16f(hi));
-                }
+            if (sact) {
+                uint4 sp[4];
+                split8(pe, pmask, sp);
+                *reinterpret_cast<uint4*>(&es[0][so][0][sr][8 * sg]) = sp[0];
+                *reinterpret_cast<uint4*>(&es[0][so][1][sr][8 * sg]) = sp[1];
+                *reinterpret_cast<uint4*>(&es[1][so][0][sr][8 * sg]) = sp[2];
+                *reinterpret_cast<uint4*>(&es[1][so][1][sr][8 * sg]) = sp[3];
             }
             __syncthreads();
             load_e(q0 + kQ);
@@ -752,7 +774,11 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
                                                        const TX* __restrict__ x, float2* __restrict__ Sall) {
     constexpr int CC = 8 * CT;            // complex columns per workgroup
     constexpr int kJ = 32, kP = kJ + 8;   // staged RWGs per block, padded LDS row (80 B)
-    constexpr int NE = kJ * CC, NU = (NE + 255) / 256;
+    // staging units: a thread loads 8 consecutive RWGs of one column and
+    // stores their split pairs with four 16-byte writes, which 8 consecutive
+    // lanes (consecutive columns, rows 80 B apart) spread over all banks
+    constexpr int NUS = (kJ / 8) * CC;
+    static_assert(NUS <= 256, "one staging unit per thread");
     __shared__ __align__(16) unsigned short xs[2][2][CC][kP];   // [hi, lo][Re, Im][column][RWG]
     const int Q = 3 * K, QG = Qp / 64;
     const int ncol = NCUR * R, ncg = (ncol + CC - 1) / CC;
@@ -779,37 +805,32 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
 #pragma unroll
             for (int p = 0; p < 4; ++p) a[p] = *reinterpret_cast<const uint4*>(arow + p * plane + kc);
         };
-        TX px[NU];
+        const bool sact = NUS == 256 || (int)threadIdx.x < NUS;
+        const int su = min((int)threadIdx.x, NUS - 1);   // spare threads repeat the last unit (not stored)
+        const int sc = su % CC, sg = su / CC;
+        const int scol = min(cb0 + sc, ncol - 1);
+        const int scur = scol / R;
+        const TX* xc = x + (size_t)scur * N * R + (scol - scur * R);
+        const bool scok = cb0 + sc < ncol;
+        TX px[8];
         unsigned pmask = 0;
         auto load_x = [&](int kb) {
             pmask = 0;
 #pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int e = min((int)threadIdx.x + 256 * u, NE - 1);
-                const int jj = e / CC, c = e - jj * CC;
-                const int col = min(cb0 + c, ncol - 1);
-                const int cur = col / R, r = col - cur * R;
-                const int jg = kb * kJ + jj;
-                px[u] = x[((size_t)cur * N + perm[i0 + min(jg, nt - 1)]) * R + r];
-                pmask |= (unsigned)((NE % 256 == 0 || (int)threadIdx.x + 256 * u < NE) & (jg < nt) &
-                                    (cb0 + c < ncol)) << u;
+            for (int i = 0; i < 8; ++i) {
+                const int jg = kb * kJ + 8 * sg + i;
+                px[i] = xc[(size_t)perm[i0 + min(jg, nt - 1)] * R];
+                pmask |= (unsigned)((jg < nt) & scok) << i;
             }
         };
         auto store_x = [&] {
-#pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int e = (int)threadIdx.x + 256 * u;
-                if (NE % 256 == 0 || e < NE) {
-                    const int jj = e / CC, c = e - jj * CC;
-                    const bool on = (pmask >> u) & 1u;
-                    const float fr = on ? (float)px[u].x : 0.f, fi = on ? (float)px[u].y : 0.f;
-                    const unsigned hr = bf16b(fr), hi = bf16b(fi);
-                    xs[0][0][c][jj] = (unsigned short)hr;
-                    xs[0][1][c][jj] = (unsigned short)hi;
-                    xs[1][0][c][jj] = (unsigned short)bf16b(fr - bf16f(hr));
-                    xs[1][1][c][jj] = (unsigned short)bf16b(fi - bf16f(hi));
-                }
-            }
+            if (!sact) return;
+            uint4 sp[4];
+            split8(px, pmask, sp);
+            *reinterpret_cast<uint4*>(&xs[0][0][sc][8 * sg]) = sp[0];
+            *reinterpret_cast<uint4*>(&xs[0][1][sc][8 * sg]) = sp[1];
+            *reinterpret_cast<uint4*>(&xs[1][0][sc][8 * sg]) = sp[2];
+            *reinterpret_cast<uint4*>(&xs[1][1][sc][8 * sg]) = sp[3];
         };
         // one block: stage its X, then the next block's loads, then the
         // products from the current patterns (cur); blocks alternate between
