@@ -595,19 +595,20 @@ void Engine::disaggregate(double2* y, int ldr, int r0, int rb) {
     const int L = tree.max_level;
     if (std::is_same<R, float>::value && eall_R_ > 0) {   // deferred: this pass's columns of Eall only
         const bool pm = ie == MLFMA_PMCHWT;
+        const bool split = d_Vb_[0] != nullptr;   // the split-pair product reads Eall as BF16 pairs
         const FarDev<float>& F1 = far<float>(0);
         if (!pm) {
             const cplx c = (med.k1 * med.eta1 / (4.0 * kPi)) * (med.k1 / (4.0 * kPi));   // Code/MLFMM.cpp:226
             const double2 z = make_double2(0, 0);
-            launch_beff_all(st, nleaf, rb, wd_[0].K[L], false, F1.Bm[L], F1.kp0, d2(c), z, z, z, eall_R_, r0,
+            launch_beff_all(st, nleaf, rb, wd_[0].K[L], false, F1.Bm[L], F1.kp0, d2(c), z, z, z, eall_R_, r0, split,
                             d_Eall_[0]);
         } else {
             const FarDev<float>& F2 = far<float>(1);
             const cplx c1 = med.k1 * med.k1 / (16.0 * kPi * kPi), c2 = med.k2 * med.k2 / (16.0 * kPi * kPi);
             launch_beff_all(st, nleaf, rb, wd_[0].K[L], true, F1.Bm[L], F1.kp0, d2(c1), d2(c1), d2(c1), d2(c1),
-                            eall_R_, r0, d_Eall_[0]);
+                            eall_R_, r0, split, d_Eall_[0]);
             launch_beff_all(st, nleaf, rb, wd_[1].K[L], true, F2.Bm[L], F2.kp0, d2(c2 * med.etai), d2(c2), d2(c2),
-                            d2(c2 / med.etai), eall_R_, r0, d_Eall_[1]);
+                            d2(c2 / med.etai), eall_R_, r0, split, d_Eall_[1]);
         }
         return;
     }
@@ -718,9 +719,9 @@ void Engine::apply_one(const double2* x, double2* y, int prec) { apply_pass(x, y
 // Y = A X for R right-hand sides, RHS-fastest (element (u, r) at u * R + r).
 // With the FP32 near field the near product of all R runs first on the
 // matrix cores; the far field follows in passes of rb right-hand sides.
-// Whether apply_block can read an FP32 input directly: the matrix-core near
-// product of split pairs and the all-RHS split-pair aggregation are the only
-// readers of X on that path.
+// Whether apply_block can read an FP32 input directly: on that path only the
+// matrix-core near product of split pairs reads X (the all-RHS split-pair
+// aggregation reads the pairs it leaves).
 bool Engine::fp32_input_ok(int R, int prec) const {
     if (!(prec == MLFMA_FP32 && P.far_precision == MLFMA_FP32 && R >= 2 && near_mfma_ && disagg_mfma_ && d_Zbs_))
         return false;
@@ -746,6 +747,8 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec, const fl
     int cap = f32 ? rb_max<float>() : rb_max<double>();
     if (rb_tuned_[f32 ? 1 : 0] > 0) cap = std::min(cap, rb_tuned_[f32 ? 1 : 0]);
     const bool mfma_near = R >= 2 && near_mfma_;
+    // the split-pair near product leaves X's BF16 pairs in d_xf_ (all R, RHS-fastest)
+    const bool xsplit = mfma_near && prec == MLFMA_FP32 && d_Zbs_;
     if (mfma_near) {
         if (prof_on_) MLFMA_HIP(hipEventRecord(ev_[16], st));
         if (prec == MLFMA_FP32 && d_Zbs_) {
@@ -790,9 +793,9 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec, const fl
         const int L = tree.max_level;
         timed(0, nw, [&] {
             for (int w = 0; w < nw; ++w) {
-                if (d_Va_[w] && Xf)   // split-pair BF16 (near_precision = MLFMA_BF16)
+                if (d_Va_[w] && xsplit)   // split-pair BF16 (near_precision = MLFMA_BF16), X split above
                     launch_aggregate_bf3(st, ncur, nleaf, wd_[w].K[L], N, R, cap, Qa_[w], Np_, d_leaf_ptr_, d_joff_,
-                                         d_perm_, d_Va_[w], Xf, d_Sall_[w]);
+                                         d_perm_, d_Va_[w], reinterpret_cast<const uint2*>(d_xf_), d_Sall_[w]);
                 else if (d_Va_[w])
                     launch_aggregate_bf3(st, ncur, nleaf, wd_[w].K[L], N, R, cap, Qa_[w], Np_, d_leaf_ptr_, d_joff_,
                                          d_perm_, d_Va_[w], X, d_Sall_[w]);
@@ -821,8 +824,9 @@ void Engine::apply_block(const double2* X, double2* Y, int R, int prec, const fl
         const bool pm = ie == MLFMA_PMCHWT;
         timed(4, 1, [&] {
             if (d_Vb_[0])   // split-pair BF16 (near_precision = MLFMA_BF16)
-                launch_disagg_bf3(st, R, nd_, pm, wd_[0].K[L], d_Vb_[0], d_Eall_[0], pm ? wd_[1].K[L] : 0,
-                                  pm ? d_Vb_[1] : nullptr, d_Eall_[1], Y);
+                launch_disagg_bf3(st, R, nd_, pm, wd_[0].K[L], d_Vb_[0], reinterpret_cast<const uint2*>(d_Eall_[0]),
+                                  pm ? wd_[1].K[L] : 0, pm ? d_Vb_[1] : nullptr,
+                                  reinterpret_cast<const uint2*>(d_Eall_[1]), Y);
             else
                 launch_disagg_all(st, R, nd_, pm, wd_[0].K[L], far<float>(0).Vf, d_Eall_[0], pm ? wd_[1].K[L] : 0,
                                   pm ? far<float>(1).Vf : nullptr, d_Eall_[1], Y);

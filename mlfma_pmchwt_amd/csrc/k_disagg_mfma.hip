@@ -199,6 +199,25 @@ void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int r
 // Eall[w][leaf][o][q][R] (q = d K + k as in E, right-hand sides fastest,
 // pass columns r0 .. r0 + rb - 1), and one product per matvec then reads
 // each pattern row once for all R right-hand sides instead of once per pass.
+// SPLIT (the split-pair disaggregation): each element is stored as its BF16
+// pairs, .x = hi Re | hi Im << 16, .y = lo Re | lo Im << 16 (8 bytes, as the
+// FP32 value), so the product stages them without converting.
+__device__ __forceinline__ unsigned bf16r(float f) {   // round to nearest even (finite inputs)
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+template <bool SPLIT>
+__device__ __forceinline__ void put_e(float2* E, size_t i, float2 v) {
+    if constexpr (SPLIT) {
+        const unsigned hr = bf16r(v.x), hi = bf16r(v.y);
+        const unsigned lr = bf16r(v.x - __uint_as_float(hr << 16)), li = bf16r(v.y - __uint_as_float(hi << 16));
+        reinterpret_cast<uint2*>(E)[i] = make_uint2(hr | (hi << 16), lr | (li << 16));
+    } else {
+        E[i] = v;
+    }
+}
+template <bool SPLIT>
 __global__ void k_beff_all(int nleaf, int rb, int K, int pmchwt, const float2* __restrict__ Bm,
                            const float* __restrict__ kp, double2 ca, double2 cb, double2 cc, double2 cd, int R,
                            int r0, float2* __restrict__ Eall) {
@@ -212,7 +231,7 @@ __global__ void k_beff_all(int nleaf, int rb, int K, int pmchwt, const float2* _
     const float2* BJ = Bm + ((size_t)leaf * rb * ncur + (size_t)rr * ncur) * Q;
     float2* EJ = Eall + (size_t)leaf * no * Q * R + r0 + rr;   // element (o, q) at EJ[(o Q + q) R]
     if (!pmchwt) {
-        for (int d = 0; d < 3; ++d) EJ[((size_t)d * K + k) * R] = M::mulc(ca, BJ[(size_t)d * K + k]);
+        for (int d = 0; d < 3; ++d) put_e<SPLIT>(EJ, ((size_t)d * K + k) * R, M::mulc(ca, BJ[(size_t)d * K + k]));
         return;
     }
     const float2* BM = BJ + Q;
@@ -225,17 +244,20 @@ __global__ void k_beff_all(int nleaf, int rb, int K, int pmchwt, const float2* _
     const float2 cm[3] = {csub(cscale(m[2], ky), cscale(m[1], kz)), csub(cscale(m[0], kz), cscale(m[2], kx)),
                           csub(cscale(m[1], kx), cscale(m[0], ky))};
     for (int d = 0; d < 3; ++d) {
-        EJ[((size_t)d * K + k) * R] = csub(M::mulc(ca, j[d]), M::mulc(cb, cm[d]));
-        EM[((size_t)d * K + k) * R] = cadd(M::mulc(cc, cj[d]), M::mulc(cd, m[d]));
+        put_e<SPLIT>(EJ, ((size_t)d * K + k) * R, csub(M::mulc(ca, j[d]), M::mulc(cb, cm[d])));
+        put_e<SPLIT>(EM, ((size_t)d * K + k) * R, cadd(M::mulc(cc, cj[d]), M::mulc(cd, m[d])));
     }
 }
 
 void launch_beff_all(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, const float2* Bm, const float* kp,
-                     double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, float2* Eall) {
+                     double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, bool split, float2* Eall) {
     if (nleaf == 0) return;
     const size_t total = (size_t)nleaf * rb * K;
-    k_beff_all<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(nleaf, rb, K, pmchwt ? 1 : 0, Bm, kp, ca, cb, cc, cd,
-                                                              R, r0, Eall);
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    if (split)
+        k_beff_all<true><<<grid, 256, 0, st>>>(nleaf, rb, K, pmchwt ? 1 : 0, Bm, kp, ca, cb, cc, cd, R, r0, Eall);
+    else
+        k_beff_all<false><<<grid, 256, 0, st>>>(nleaf, rb, K, pmchwt ? 1 : 0, Bm, kp, ca, cb, cc, cd, R, r0, Eall);
 }
 
 // Y[perm[i] (+N for output 1)][r] += sum_w V_w[i] . Eall_w[leaf][o][.][r] for all R.
@@ -417,6 +439,22 @@ __device__ __forceinline__ void split8(const V (&v)[8], unsigned mask, uint4 (&o
 #pragma unroll
     for (int p = 0; p < 4; ++p) o[p] = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
 }
+// The same from values already split: .x = hi Re | hi Im << 16, .y = lo Re |
+// lo Im << 16 (as the near product stages X), only regrouped.
+__device__ __forceinline__ void split8(const uint2 (&v)[8], unsigned mask, uint4 (&o)[4]) {
+    unsigned w[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint2 a = ((mask >> (2 * k)) & 1u) ? v[2 * k] : make_uint2(0u, 0u);
+        const uint2 b = ((mask >> (2 * k + 1)) & 1u) ? v[2 * k + 1] : make_uint2(0u, 0u);
+        w[0][k] = (a.x & 0xffffu) | (b.x << 16);
+        w[1][k] = (a.x >> 16) | (b.x & 0xffff0000u);
+        w[2][k] = (a.y & 0xffffu) | (b.y << 16);
+        w[3][k] = (a.y >> 16) | (b.y & 0xffff0000u);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) o[p] = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+}
 }  // namespace
 
 // planes[p][i][q] from V[i][q] (FP32 complex), q < Qp (zero past Q)
@@ -443,9 +481,9 @@ void launch_split_rows(hipStream_t st, int N, int Q, int Qp, const float2* V, un
 template <int NO, int CT>
 __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, int K1,
                                                     const unsigned short* __restrict__ P1,
-                                                    const float2* __restrict__ E1, int K2,
+                                                    const uint2* __restrict__ E1, int K2,
                                                     const unsigned short* __restrict__ P2,
-                                                    const float2* __restrict__ E2, double2* __restrict__ y) {
+                                                    const uint2* __restrict__ E2, double2* __restrict__ y) {
     constexpr int RT = 8 * CT;
     constexpr int kQ = 32, kP = kQ + 8;   // staged q per chunk, padded LDS row (80 B)
     // staging units: a thread loads 8 consecutive q of one (output, rhs) and
@@ -480,14 +518,14 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
     const bool sact = NUS == 256 || (int)threadIdx.x < NUS;
     const int su = min((int)threadIdx.x, NUS - 1);   // spare threads repeat the last unit (not stored)
     const int sr = su % RT, sg = (su / RT) % (kQ / 8), so = su / (RT * (kQ / 8));
-    float2 pe[8];
+    uint2 pe[8];   // BF16 pairs as the passes stored them
     unsigned pmask = 0;
     for (int w = 0; w < nw; ++w) {
         const int Q = 3 * (w ? K2 : K1), Qp = (Q + 31) & ~31;
         const unsigned short* Pl = w ? P2 : P1;
         const size_t plane = (size_t)nd.N * Qp;
         const unsigned short* arow_p = Pl + ((size_t)i0 + arow) * Qp + 8 * lk;   // this lane's 8 q of plane 0
-        const float2* E = (w ? E2 : E1) + (size_t)t * NO * Q * R;
+        const uint2* E = (w ? E2 : E1) + (size_t)t * NO * Q * R;
         auto load_a = [&](int q0, uint4* a) {
             const int qc = min(q0, Qp - kQ);
 #pragma unroll
@@ -495,7 +533,7 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
         };
         auto load_e = [&](int q0) {
             pmask = 0;
-            const float2* Er = E + (size_t)so * Q * R + rb0 + min(sr, rv - 1);
+            const uint2* Er = E + (size_t)so * Q * R + rb0 + min(sr, rv - 1);
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int q = q0 + 8 * sg + i;
@@ -574,7 +612,7 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
 }
 
 void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const unsigned short* P1,
-                       const float2* E1, int K2, const unsigned short* P2, const float2* E2, double2* y) {
+                       const uint2* E1, int K2, const unsigned short* P2, const uint2* E2, double2* y) {
     if (nd.nitem == 0 || R < 1) return;
     const int ct = R > 16 ? 4 : (R > 8 ? 2 : 1);
     const int ny = (R + 8 * ct - 1) / (8 * ct);
@@ -813,13 +851,18 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
         const TX* xc = x + (size_t)scur * N * R + (scol - scur * R);
         const bool scok = cb0 + sc < ncol;
         TX px[8];
+        int pidx[8];   // unknowns of the block whose X is loaded next (read one block earlier)
         unsigned pmask = 0;
+        auto load_perm = [&](int kb) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) pidx[i] = perm[i0 + min(kb * kJ + 8 * sg + i, nt - 1)];
+        };
         auto load_x = [&](int kb) {
             pmask = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int jg = kb * kJ + 8 * sg + i;
-                px[i] = xc[(size_t)perm[i0 + min(jg, nt - 1)] * R];
+                px[i] = xc[(size_t)pidx[i] * R];
                 pmask |= (unsigned)((jg < nt) & scok) << i;
             }
         };
@@ -840,6 +883,7 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
             store_x();
             __syncthreads();
             load_x(kb + 1);
+            load_perm(kb + 2);
             load_a(kb + 1, nxt);
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) {
@@ -861,7 +905,9 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
             }
         };
         uint4 aa[4], ab[4];
+        load_perm(0);
         load_x(0);
+        load_perm(1);
         load_a(0, aa);
         for (int kb = 0; kb < nkb; kb += 2) {
             block(kb, aa, ab);
@@ -926,8 +972,8 @@ void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int
 
 void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
                           const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
-                          const float2* x, float2* Sall) {
-    aggregate_bf3_t(st, ncur, nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, x, Sall);
+                          const uint2* xsp, float2* Sall) {
+    aggregate_bf3_t(st, ncur, nleaf, K, N, R, cap, Qp, Np, leaf_ptr, joff, perm, planes, xsp, Sall);
 }
 
 #define MLFMA_DISAGG_MFMA_INSTANTIATE(T)                                                                            \

@@ -137,16 +137,18 @@ void launch_disagg_mfma(hipStream_t st, int nleaf, int N, int ldr, int r0, int r
 // All right-hand sides at once (FP32): the passes' effective matrices go into
 // Eall[leaf][o][3K][R] at columns r0 .. r0 + rb - 1, then one product per
 // matvec over the near product's work items (nd.items, leaf rows).
+// (split: each element stored as its BF16 pairs, uint2 .x = hi Re | hi Im << 16,
+// .y = lo Re | lo Im << 16, for launch_disagg_bf3)
 void launch_beff_all(hipStream_t st, int nleaf, int rb, int K, bool pmchwt, const float2* Bm, const float* kp,
-                     double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, float2* Eall);
+                     double2 ca, double2 cb, double2 cc, double2 cd, int R, int r0, bool split, float2* Eall);
 void launch_disagg_all(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const float2* V1,
                        const float2* E1, int K2, const float2* V2, const float2* E2, double2* y);
 // Split-pair BF16 disaggregation of all R: pattern planes [Re hi, Im hi,
 // Re lo, Im lo][N][Qp] (Qp = 3K rounded up to 32) from the FP32 patterns,
-// and the product from Eall (FP32, split when staged).
+// and the product from Eall stored as BF16 pairs (launch_beff_all, split).
 void launch_split_rows(hipStream_t st, int N, int Q, int Qp, const float2* V, unsigned short* planes);
 void launch_disagg_bf3(hipStream_t st, int R, const NearDev& nd, bool pmchwt, int K1, const unsigned short* P1,
-                       const float2* E1, int K2, const unsigned short* P2, const float2* E2, double2* y);
+                       const uint2* E1, int K2, const unsigned short* P2, const uint2* E2, double2* y);
 // Aggregation of all R right-hand sides (FP32 patterns, x RHS-fastest with
 // stride R) into the leaf workspaces of every pass of the batched matvec
 // (passes of rb = largest power of two <= cap, pass (r0, rb) at
@@ -162,9 +164,11 @@ void launch_split_cols(hipStream_t st, int Q, int Qp, int Np, const int* jmap, c
 void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
                           const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
                           const double2* x, float2* Sall);
+// (x already split into BF16 pairs, RHS-fastest: .x = hi Re | hi Im << 16,
+// .y = lo Re | lo Im << 16, as the split-pair near product stages it)
 void launch_aggregate_bf3(hipStream_t st, int ncur, int nleaf, int K, int N, int R, int cap, int Qp, int Np,
                           const int* leaf_ptr, const int* joff, const int* perm, const unsigned short* planes,
-                          const float2* x, float2* Sall);
+                          const uint2* xsp, float2* Sall);
 
 // FP32 copy of an FP64 device array (set-up: patterns computed in FP64)
 void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out);
