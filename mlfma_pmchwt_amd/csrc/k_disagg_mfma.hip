@@ -491,7 +491,10 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
     // lanes (consecutive rhs, rows 80 B apart) spread over all banks
     constexpr int NUS = NO * (kQ / 8) * RT;
     static_assert(NUS <= 256, "one staging unit per thread");
-    __shared__ __align__(16) unsigned short es[2][NO][2][RT][kP];   // [hi, lo][output][Re, Im][rhs][q]
+    // [hi, lo][output][Re, -Im, Im][rhs][q]: a lane of the real-part columns
+    // (part 0) reads planes 0 and 1 as its two B operands, of the imaginary
+    // part (part 1) planes 2 and 0, so the products need no sign flips
+    __shared__ __align__(16) unsigned short es[2][NO][3][RT][kP];
     // chunks of 4 items (mostly the row ranges of one leaf, which read the
     // same effective matrices) with their ny right-hand-side groups (which
     // read the same pattern planes) run together on one XCD
@@ -551,9 +554,11 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
                 uint4 sp[4];
                 split8(pe, pmask, sp);
                 *reinterpret_cast<uint4*>(&es[0][so][0][sr][8 * sg]) = sp[0];
-                *reinterpret_cast<uint4*>(&es[0][so][1][sr][8 * sg]) = sp[1];
+                *reinterpret_cast<uint4*>(&es[0][so][1][sr][8 * sg]) = neg8b(sp[1]);
+                *reinterpret_cast<uint4*>(&es[0][so][2][sr][8 * sg]) = sp[1];
                 *reinterpret_cast<uint4*>(&es[1][so][0][sr][8 * sg]) = sp[2];
-                *reinterpret_cast<uint4*>(&es[1][so][1][sr][8 * sg]) = sp[3];
+                *reinterpret_cast<uint4*>(&es[1][so][1][sr][8 * sg]) = neg8b(sp[3]);
+                *reinterpret_cast<uint4*>(&es[1][so][2][sr][8 * sg]) = sp[3];
             }
             __syncthreads();
             load_e(q0 + kQ);
@@ -567,9 +572,8 @@ __global__ __launch_bounds__(256) void k_disagg_bf3(NearDev nd, int R, int nw, i
                     floatx4 d = acc[ct][o];
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {   // E part: hi, then lo (lo meets only the hi patterns)
-                        const uint4 xa = *reinterpret_cast<const uint4*>(&es[h][o][part][rc][8 * lk]);
-                        const uint4 xo = *reinterpret_cast<const uint4*>(&es[h][o][1 - part][rc][8 * lk]);
-                        const uint4 b1 = xa, b2 = part ? xo : neg8b(xo);
+                        const uint4 b1 = *reinterpret_cast<const uint4*>(&es[h][o][2 * part][rc][8 * lk]);
+                        const uint4 b2 = *reinterpret_cast<const uint4*>(&es[h][o][1 - part][rc][8 * lk]);
                         d = mfb(cur[0], b1, d);
                         d = mfb(cur[1], b2, d);
                         if (h == 0) {
@@ -817,7 +821,10 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
     // lanes (consecutive columns, rows 80 B apart) spread over all banks
     constexpr int NUS = (kJ / 8) * CC;
     static_assert(NUS <= 256, "one staging unit per thread");
-    __shared__ __align__(16) unsigned short xs[2][2][CC][kP];   // [hi, lo][Re, Im][column][RWG]
+    // [hi, lo][Re, -Im, Im][column][RWG]: a lane of the real-part columns
+    // (part 0) reads planes 0 and 1 as its two B operands, of the imaginary
+    // part (part 1) planes 2 and 0, so the products need no sign flips
+    __shared__ __align__(16) unsigned short xs[2][3][CC][kP];
     const int Q = 3 * K, QG = Qp / 64;
     const int ncol = NCUR * R, ncg = (ncol + CC - 1) / CC;
     int lq, cg;
@@ -871,9 +878,11 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
             uint4 sp[4];
             split8(px, pmask, sp);
             *reinterpret_cast<uint4*>(&xs[0][0][sc][8 * sg]) = sp[0];
-            *reinterpret_cast<uint4*>(&xs[0][1][sc][8 * sg]) = sp[1];
+            *reinterpret_cast<uint4*>(&xs[0][1][sc][8 * sg]) = neg8b(sp[1]);
+            *reinterpret_cast<uint4*>(&xs[0][2][sc][8 * sg]) = sp[1];
             *reinterpret_cast<uint4*>(&xs[1][0][sc][8 * sg]) = sp[2];
-            *reinterpret_cast<uint4*>(&xs[1][1][sc][8 * sg]) = sp[3];
+            *reinterpret_cast<uint4*>(&xs[1][1][sc][8 * sg]) = neg8b(sp[3]);
+            *reinterpret_cast<uint4*>(&xs[1][2][sc][8 * sg]) = sp[3];
         };
         // one block: stage its X, then the next block's loads, then the
         // products from the current patterns (cur); blocks alternate between
@@ -891,9 +900,8 @@ __global__ __launch_bounds__(256) void k_aggregate_bf3(int nleaf, int K, int N, 
                 floatx4 d = acc[ct];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {   // X part: hi, then lo (lo meets only the hi patterns)
-                    const uint4 xa = *reinterpret_cast<const uint4*>(&xs[h][part][rc][8 * lk]);       // Re X | Im X
-                    const uint4 xo = *reinterpret_cast<const uint4*>(&xs[h][1 - part][rc][8 * lk]);   // Im X | Re X
-                    const uint4 b1 = xa, b2 = part ? xo : neg8b(xo);
+                    const uint4 b1 = *reinterpret_cast<const uint4*>(&xs[h][2 * part][rc][8 * lk]);   // Re X | Im X
+                    const uint4 b2 = *reinterpret_cast<const uint4*>(&xs[h][1 - part][rc][8 * lk]);   // -Im X | Re X
                     d = mfb(cur[0], b1, d);
                     d = mfb(cur[1], b2, d);
                     if (h == 0) {
