@@ -145,10 +145,11 @@ __global__ void k_poly_init_bf16(size_t total, int R, const float2* __restrict__
 // tile, so each record is read from memory once per workgroup, by one
 // wavefront, for up to 32 right-hand sides; the next
 // 32-column block's records are loaded while the current block's products
-// run.  X is staged in LDS 64 columns at a time as BF16, [Re, Im][rhs][column]
-// rows of 144 B: a thread gathers 8 consecutive columns of one right-hand side
-// (each load coalesced over the wavefront's right-hand sides) and writes them
-// with two 16-byte stores, which 8 consecutive lanes spread over all 32 banks.
+// run.  X is staged in LDS 64 columns at a time as BF16, [plane][rhs][column]
+// rows of 144 B (planes Re, -Im, Im, and -Re for X_M): a thread gathers 8
+// consecutive columns of one right-hand side (each load coalesced over the
+// wavefront's right-hand sides) and writes them with 16-byte stores, which 8
+// consecutive lanes spread over all 32 banks.
 //
 // S (split pairs, the operator's batched products with near_precision =
 // MLFMA_BF16): Z = Zh + Zl and X = Xh + Xl in BF16, and each product is
@@ -173,7 +174,13 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
     constexpr int W = S ? 256 : 128;       // uint4 per record
     using xb_t = typename std::conditional<S, uint2, unsigned>::type;
     const xb_t* __restrict__ xb = static_cast<const xb_t*>(xbv);
-    __shared__ __align__(16) unsigned short xs[NS][NX][2][RT][kKP];   // [part][current][Re, Im][rhs][column]
+    // [part][current][plane][rhs][column], planes Re, -Im, Im and (X_M only)
+    // -Re: every B operand of the complex products, with the signs of
+    // Y_J = Z11 X_J - ZK X_M, is one LDS read with a lane-dependent plane (the
+    // real-part columns, part 0, and the imaginary-part columns, part 1, take
+    // different planes), so the products need no sign flips
+    constexpr int NPL = P == 3 ? 4 : 3;
+    __shared__ __align__(16) unsigned short xs[NS][NX][NPL][RT][kKP];
     // Chunks of 4 items (mostly the row ranges of one leaf, which gather the
     // same X columns) with their ny right-hand-side groups (which read the
     // same records) run together on one XCD, so the later reads hit its L2.
@@ -269,21 +276,19 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
         for (int ct = 0; ct < CT; ++ct) {
             const int rc = ct * 8 + (li & 7);
             floatx4 a = acc[ct][0], b = acc[ct][NO - 1];
+            auto ld = [&](int xp, int xj, int pl) { return *reinterpret_cast<const uint4*>(&xs[xp][xj][pl][rc][kof]); };
             // (Z part, X part) pairs: (hi, hi), and with S also (hi, lo), (lo, hi)
 #pragma unroll
             for (int q = 0; q < (S ? 3 : 1); ++q) {
                 const int zp = q == 2 ? P : 0, xp = q == 1 ? 1 : 0;
-                const uint4 xa = *reinterpret_cast<const uint4*>(&xs[xp][0][part][rc][kof]);       // Re X | Im X
-                const uint4 xo = *reinterpret_cast<const uint4*>(&xs[xp][0][1 - part][rc][kof]);   // Im X | Re X
-                const uint4 bJ1 = xa, bJ2 = part ? xo : neg8(xo);
+                const uint4 bJ1 = ld(xp, 0, 2 * part);   // Re X | Im X
+                const uint4 bJ2 = ld(xp, 0, 1 - part);   // -Im X | Re X
                 a = mfma(zr[zp], bJ1, a);
                 a = mfma(zi[zp], bJ2, a);
                 if (P == 3) {
-                    const uint4 ma = *reinterpret_cast<const uint4*>(&xs[xp][NX - 1][part][rc][kof]);
-                    const uint4 mo = *reinterpret_cast<const uint4*>(&xs[xp][NX - 1][1 - part][rc][kof]);
-                    const uint4 bM1 = ma, bM2 = part ? mo : neg8(mo);
-                    a = mfma(zr[zp + 1], neg8(bM1), a);   // - ZK X_M
-                    a = mfma(zi[zp + 1], neg8(bM2), a);
+                    const uint4 bM1 = ld(xp, NX - 1, 2 * part), bM2 = ld(xp, NX - 1, 1 - part);
+                    a = mfma(zr[zp + 1], ld(xp, NX - 1, 3 - 2 * part), a);   // - ZK X_M: -Re | -Im
+                    a = mfma(zi[zp + 1], ld(xp, NX - 1, 2 + part), a);       //           Im | -Re
                     b = mfma(zr[zp + 1], bJ1, b);
                     b = mfma(zi[zp + 1], bJ2, b);
                     b = mfma(zr[zp + 2], bM1, b);
@@ -313,8 +318,12 @@ __global__ __launch_bounds__(256) void k_near_bf16(NearDev nd, int R, int ldr, i
                     re[i >> 1] |= (v & 0xffffu) << (16 * (i & 1));
                     im[i >> 1] |= (v >> 16) << (16 * (i & 1));
                 }
-                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][0][ur[u]][ucg[u] * 8]) = make_uint4(re[0], re[1], re[2], re[3]);
-                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][1][ur[u]][ucg[u] * 8]) = make_uint4(im[0], im[1], im[2], im[3]);
+                const uint4 vr = make_uint4(re[0], re[1], re[2], re[3]), vi = make_uint4(im[0], im[1], im[2], im[3]);
+                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][0][ur[u]][ucg[u] * 8]) = vr;
+                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][1][ur[u]][ucg[u] * 8]) = neg8(vi);
+                *reinterpret_cast<uint4*>(&xs[h][uxj[u]][2][ur[u]][ucg[u] * 8]) = vi;
+                if (P == 3 && uxj[u] == NX - 1)
+                    *reinterpret_cast<uint4*>(&xs[h][NX - 1][NPL - 1][ur[u]][ucg[u] * 8]) = neg8(vr);
             }
         }
         __syncthreads();
