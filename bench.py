@@ -118,7 +118,11 @@ def group_roofline(op, R, dom, groups):
         work, unit, peak, bound = translation_flops(op, R, 2) / launches, "TFLOP/s", FP32_PEAK_TFS, "mfma"
         note = ("compute roofline at the FP32 dense peak; the kernel does its FP32 FMAs on the vector ALUs, where "
                 "packed FP32 instructions are disabled in this build (half that rate); counters: the vector-memory "
-                "address unit is busy about 70% of the kernel's time, VALU issue about 44%")
+                "address unit is busy about 60% of the kernel's time, VALU issue about 38%")
+    elif dom == "aggregate" and op.params.near_precision == 2:
+        kern = "k_aggregate_bf3 (split-pair BF16 matrix cores, all right-hand sides)"
+        work, unit, peak, bound = far_flops(op, R, dom) / launches, "TFLOP/s", 2500.0, "mfma"
+        note = "BF16 MFMA dense peak; the split-pair product issues three BF16 products per complex multiply-add"
     elif dom == "aggregate":
         kern = "k_aggregate_all (FP32 matrix cores, all right-hand sides)"
         work, unit, peak, bound = far_flops(op, R, dom) / launches, "TFLOP/s", FP32_PEAK_TFS, "mfma"
