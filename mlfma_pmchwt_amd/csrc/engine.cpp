@@ -931,7 +931,10 @@ void Engine::set_graphs(bool on) {
 // cores; bit 2 translation by sibling clusters) is
 // flipped in turn and kept when faster, then narrower passes are tried.  The
 // choices act on separate kernels, so they barely interact, and the search
-// times about six configurations instead of all of them.  out[0] = chosen rb,
+// times about six configurations instead of all of them.  Each configuration
+// runs one untimed matvec and two timed ones; one whose untimed matvec already
+// took twice the best time so far is not timed further (it counts with that
+// time, so it cannot be chosen).  out[0] = chosen rb,
 // out[1] = choice bits, out[2] = milliseconds per matvec of the choice, out[3]
 // = of the slowest configuration timed.
 void Engine::autotune(int R, int prec, double* out) {
@@ -947,27 +950,34 @@ void Engine::autotune(int R, int prec, double* out) {
     hipEvent_t a, b;
     MLFMA_HIP(hipEventCreate(&a));
     MLFMA_HIP(hipEventCreate(&b));
-    double worst = 0;
+    double worst = 0, best = 0;
     auto time = [&](int rb, int m) {
         near_mfma_ = (m & 1) != 0;
         disagg_mfma_ = (m & 2) != 0;
         m2l_cluster_ = (m & 4) != 0;
         m2l_variant_ = (m & 16) ? 2 : ((m & 8) ? 1 : 0);
         rb_tuned_[f32 ? 1 : 0] = rb;
-        apply_block(d_bx_, d_by_, R, prec);   // warm-up
+        float ms = 0;
         MLFMA_HIP(hipEventRecord(a, st));
-        for (int k = 0; k < 2; ++k) apply_block(d_bx_, d_by_, R, prec);
+        apply_block(d_bx_, d_by_, R, prec);   // warm-up
         MLFMA_HIP(hipEventRecord(b, st));
         MLFMA_HIP(hipEventSynchronize(b));
-        float ms = 0;
         MLFMA_HIP(hipEventElapsedTime(&ms, a, b));
-        const double per = ms / 2.0;
+        double per = ms;
+        if (best == 0 || per < 2.0 * best) {
+            MLFMA_HIP(hipEventRecord(a, st));
+            for (int k = 0; k < 2; ++k) apply_block(d_bx_, d_by_, R, prec);
+            MLFMA_HIP(hipEventRecord(b, st));
+            MLFMA_HIP(hipEventSynchronize(b));
+            MLFMA_HIP(hipEventElapsedTime(&ms, a, b));
+            per = ms / 2.0;
+        }
         if (P.verbose) std::fprintf(stderr, "[mlfma] autotune R=%d prec=%d rb=%d bits=%d: %.3f ms\n", R, prec, rb, m, per);
         worst = std::max(worst, per);
         return per;
     };
     int best_rb = rbmax, best_m = valid;
-    double best = time(best_rb, best_m);
+    best = time(best_rb, best_m);
     for (int bit = 1; bit <= 4; bit <<= 1) {
         if (!(valid & bit)) continue;
         const double t = time(best_rb, best_m ^ bit);

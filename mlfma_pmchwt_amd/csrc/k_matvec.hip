@@ -213,17 +213,19 @@ void launch_m2m(hipStream_t st, int ncur, int nparent, int Kp, int Kc, const int
 
 // Separable form of the same interpolation: the 3 x 3 stencil of a parent
 // sample (i, j) is child rows th_tap[i][a] x child columns ph_tap[j][b] with
-// weights th_w[i][a] ph_w[j][b], so a workgroup owning parent p's row i first
-// interpolates in theta, U(m) = sum_a th_w[i][a] S_c(th_tap[i][a], m), for all
-// 2 Lc child columns m of each of its CPT columns into LDS (contiguous rows of
-// the child workspace, each read once per parent row), then in phi,
-// r(j) = sum_b ph_w[j][b] U(ph_tap[j][b]), and adds shift x r, child after
-// child.  About half the loads and multiply-adds of the 9-tap form; the sums
-// are regrouped (theta inside), which changes only rounding.  A thread owns
-// parent columns j = lane + 64 jj (jj < JJ) of the row; G = nc x 3 / CPT
-// wavefronts, one per column group, each with its own LDS rows.
+// weights th_w[i][a] ph_w[j][b], so a workgroup owning RW consecutive rows of
+// parent p first interpolates in theta, U(i, m) = sum_a th_w[i][a]
+// S_c(th_tap[i][a], m), for all 2 Lc child columns m of each of its CPT
+// columns into LDS (contiguous rows of the child workspace), then in phi,
+// r(i, j) = sum_b ph_w[j][b] U(i, ph_tap[j][b]), and adds shift x r, child
+// after child.  About half the loads and multiply-adds of the 9-tap form; the
+// sums are regrouped (theta inside), which changes only rounding.  The lanes
+// of a wavefront run over (row, column) pairs of its rows, so short rows
+// still fill the wavefront and a workgroup's barriers serve RW rows; a thread
+// owns outputs t = lane + 64 jj (jj < JJ) of the RW x 2 Lp block.  G = nc x 3 /
+// CPT wavefronts, one per column group, each with its own LDS rows.
 template <typename R, int NC, int CPT, int JJ>
-__global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, const int* __restrict__ son_ptr,
+__global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, int RW, const int* __restrict__ son_ptr,
                                                         const int* __restrict__ son_idx,
                                                         const int* __restrict__ oct_child,
                                                         const int* __restrict__ th_tap, const R* __restrict__ th_w,
@@ -234,27 +236,29 @@ __global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, c
     using C = cplx_t<R>;
     extern __shared__ __align__(16) unsigned char m2m_lds[];
     const int pfn = 2 * Lp, psn = 2 * Lc, Kp = Lp * pfn, Kc = Lc * psn;
+    const int nrg = (Lp + RW - 1) / RW;   // row groups per parent
     const int lin = xcd_remap(blockIdx.x, (int)gridDim.x);
-    const int p = lin / Lp, i = lin % Lp;   // the rows of one parent are consecutive
+    const int p = lin / nrg, i0 = (lin % nrg) * RW;   // the row groups of one parent are consecutive
+    const int nr = min(RW, Lp - i0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int c0 = wave * CPT;
-    C* U = reinterpret_cast<C*>(m2m_lds) + (size_t)wave * CPT * psn;   // this wavefront's [CPT][2 Lc]
-    int ti[3];
-    R lt[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        ti[a] = th_tap[3 * i + a] * psn;
-        lt[a] = th_w[3 * i + a];
-    }
-    int pj[JJ][3];
+    const int us = RW * psn;                                            // one column's U block
+    C* U = reinterpret_cast<C*>(m2m_lds) + (size_t)wave * CPT * us;   // this wavefront's [CPT][RW][2 Lc]
+    // phi-pass outputs of this thread: row ir[jj] (relative), column jc[jj]
+    int ir[JJ], jc[JJ], pj[JJ][3];
     R wj[JJ][3];
+    bool ok[JJ];
 #pragma unroll
     for (int jj = 0; jj < JJ; ++jj) {
-        const int j = min(lane + 64 * jj, pfn - 1);
+        const int t = lane + 64 * jj;
+        ok[jj] = t < nr * pfn;
+        const int tc = min(t, nr * pfn - 1);
+        ir[jj] = tc / pfn;
+        jc[jj] = tc - ir[jj] * pfn;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            pj[jj][b] = ph_tap[3 * j + b];
-            wj[jj][b] = ph_w[3 * j + b];
+            pj[jj][b] = ir[jj] * psn + ph_tap[3 * jc[jj] + b];
+            wj[jj][b] = ph_w[3 * jc[jj] + b];
         }
     }
     C acc[JJ][CPT];
@@ -265,27 +269,35 @@ __global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, c
     for (int s = son_ptr[p]; s < son_ptr[p + 1]; ++s) {
         const int c = son_idx[s];
         const C* S = Smc + ((size_t)c * NC * 3 + c0) * Kc;
-        for (int m = lane; m < psn; m += 64) {
+        for (int t = lane; t < nr * psn; t += 64) {
+            const int r = t / psn, m = t - r * psn;
+            const int i = i0 + r;
+            int ti[3];
+            R lt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                ti[a] = th_tap[3 * i + a] * psn + m;
+                lt[a] = th_w[3 * i + a];
+            }
 #pragma unroll
             for (int d = 0; d < CPT; ++d) {
-                const C* sd = S + (size_t)d * Kc + m;
+                const C* sd = S + (size_t)d * Kc;
                 const C f0 = sd[ti[0]], f1 = sd[ti[1]], f2 = sd[ti[2]];
                 C u;
                 u.x = f0.x * lt[0] + f1.x * lt[1] + f2.x * lt[2];
                 u.y = f0.y * lt[0] + f1.y * lt[1] + f2.y * lt[2];
-                U[d * psn + m] = u;
+                U[d * us + t] = u;
             }
         }
         __syncthreads();
-        const C* sh = shup + (size_t)oct_child[c] * Kp + (size_t)i * pfn;
+        const C* sh = shup + (size_t)oct_child[c] * Kp + (size_t)i0 * pfn;
 #pragma unroll
         for (int jj = 0; jj < JJ; ++jj) {
-            const int j = lane + 64 * jj;
-            if (j < pfn) {
-                const C e = sh[j];
+            if (ok[jj]) {
+                const C e = sh[ir[jj] * pfn + jc[jj]];
 #pragma unroll
                 for (int d = 0; d < CPT; ++d) {
-                    const C u0 = U[d * psn + pj[jj][0]], u1 = U[d * psn + pj[jj][1]], u2 = U[d * psn + pj[jj][2]];
+                    const C u0 = U[d * us + pj[jj][0]], u1 = U[d * us + pj[jj][1]], u2 = U[d * us + pj[jj][2]];
                     C r;
                     r.x = u0.x * wj[jj][0] + u1.x * wj[jj][1] + u2.x * wj[jj][2];
                     r.y = u0.y * wj[jj][0] + u1.y * wj[jj][1] + u2.y * wj[jj][2];
@@ -295,32 +307,33 @@ __global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, c
         }
         __syncthreads();
     }
-    C* o = Smp + ((size_t)p * NC * 3 + c0) * Kp + (size_t)i * pfn;
+    C* o = Smp + ((size_t)p * NC * 3 + c0) * Kp + (size_t)i0 * pfn;
 #pragma unroll
-    for (int jj = 0; jj < JJ; ++jj) {
-        const int j = lane + 64 * jj;
-        if (j < pfn)
+    for (int jj = 0; jj < JJ; ++jj)
+        if (ok[jj])
 #pragma unroll
-            for (int d = 0; d < CPT; ++d) o[(size_t)d * Kp + j] = acc[jj][d];
-    }
+            for (int d = 0; d < CPT; ++d) o[(size_t)d * Kp + ir[jj] * pfn + jc[jj]] = acc[jj][d];
 }
 
-// The separable interpolation when its LDS rows fit (64 KB per workgroup) and
-// a parent row has at most 256 samples; false otherwise (the caller takes the
-// 9-tap kernel).
+// The separable interpolation when a parent row has at most 256 samples and
+// one row of every column fits in LDS (64 KB per workgroup); false otherwise
+// (the caller takes the 9-tap kernel).  A workgroup takes as many rows as fit
+// both (RW 2 Lp <= 256 outputs per lane group, RW rows of LDS).
 template <typename R>
 bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const int* son_ptr, const int* son_idx,
                     const int* octant_child, const int* th_tap, const R* th_w, const int* ph_tap, const R* ph_w,
                     const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child, cplx_t<R>* Sm_parent) {
     if (nparent == 0) return true;
-    const int JJ = (2 * Lp + 63) / 64;
-    if (JJ > 4) return false;
-    const size_t lds = (size_t)ncur * 3 * 2 * Lc * sizeof(cplx_t<R>);   // all column groups' rows
-    if (lds > 64 * 1024) return false;
-    const unsigned grid = (unsigned)nparent * Lp;
+    const size_t row_lds = (size_t)ncur * 3 * 2 * Lc * sizeof(cplx_t<R>);   // one row of all column groups
+    if (2 * Lp > 256 || row_lds > 64 * 1024) return false;
+    const int RW = std::max(1, std::min({Lp, (int)(64 * 1024 / row_lds), 256 / (2 * Lp)}));
+    const int JJ = (RW * 2 * Lp + 63) / 64;
+    const size_t lds = row_lds * RW;
+    const unsigned grid = (unsigned)nparent * ((Lp + RW - 1) / RW);
 #define M2MS(NC, CPT, J)                                                                                              \
-    k_m2m_sep<R, NC, CPT, J><<<grid, 64 * NC * 3 / CPT, lds, st>>>(Lp, Lc, son_ptr, son_idx, octant_child, th_tap,   \
-                                                                    th_w, ph_tap, ph_w, shift_up, Sm_child, Sm_parent)
+    k_m2m_sep<R, NC, CPT, J><<<grid, 64 * NC * 3 / CPT, lds, st>>>(Lp, Lc, RW, son_ptr, son_idx, octant_child,      \
+                                                                    th_tap, th_w, ph_tap, ph_w, shift_up, Sm_child, \
+                                                                    Sm_parent)
 #define M2MS_J(NC, CPT)                                                                                               \
     if (JJ == 1) M2MS(NC, CPT, 1);                                                                                    \
     else if (JJ == 2) M2MS(NC, CPT, 2);                                                                               \
