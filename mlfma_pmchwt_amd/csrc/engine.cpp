@@ -258,6 +258,10 @@ void Engine::build_wave(int w) {
     D.t_kp.assign(L + 1, nullptr);
     D.th_tap.assign(L + 1, nullptr);
     D.ph_tap.assign(L + 1, nullptr);
+    D.at_ptr.assign(L + 1, nullptr);
+    D.at_idx.assign(L + 1, nullptr);
+    D.ap_ptr.assign(L + 1, nullptr);
+    D.ap_idx.assign(L + 1, nullptr);
     D.Lp.assign(L + 1, 0);
     D.Lc.assign(L + 1, 0);
     for (int l = 2; l <= L; ++l) D.K[l] = T.grid[l].K();
@@ -266,6 +270,10 @@ void Engine::build_wave(int w) {
         D.tap[c] = upload(it.tap);
         D.th_tap[c] = upload(it.th_tap);
         D.ph_tap[c] = upload(it.ph_tap);
+        D.at_ptr[c] = upload(it.at_ptr);
+        D.at_idx[c] = upload(it.at_idx);
+        D.ap_ptr[c] = upload(it.ap_ptr);
+        D.ap_idx[c] = upload(it.ap_idx);
         D.Lp[c] = it.Lp;
         D.Lc[c] = it.Lc;
         D.t_ptr[c] = upload(it.t_ptr);
@@ -344,6 +352,8 @@ void Engine::build_far(int w, const WaveTables& T) {
     F.tw.assign(L + 1, nullptr);
     F.th_w.assign(L + 1, nullptr);
     F.ph_w.assign(L + 1, nullptr);
+    F.at_w.assign(L + 1, nullptr);
+    F.ap_w.assign(L + 1, nullptr);
     F.t_w.assign(L + 1, nullptr);
     F.sh_up.assign(L + 1, nullptr);
     F.sh_dn.assign(L + 1, nullptr);
@@ -384,6 +394,8 @@ void Engine::build_far(int w, const WaveTables& T) {
         F.tw[c] = upload(to_real<R>(it.w));
         F.th_w[c] = upload(to_real<R>(it.th_w));
         F.ph_w[c] = upload(to_real<R>(it.ph_w));
+        F.at_w[c] = upload(to_real<R>(it.at_w));
+        F.ap_w[c] = upload(to_real<R>(it.ap_w));
         F.t_w[c] = upload(to_real<R>(it.t_w));
         std::vector<cplx_t<R>> su(T.shift_up[c].size()), sd(T.shift_down[c].size());
         for (size_t e = 0; e < su.size(); ++e) su[e] = to_cx<R>(T.shift_up[c][e]);
@@ -573,8 +585,11 @@ template <typename R>
 void Engine::translate(int w, int l, int nc) {
     WaveDev& D = wd_[w];
     FarDev<R>& F = far<R>(w);
-    const bool up = l > 2;
     const LevelDev& v = lv_[l];
+    // the anterpolation from the parents either fused into the translation
+    // (gather over the 9-tap stencil's transpose) or as a separable pass after it
+    const int rw = (l > 2 && l2l_separable_) ? l2l_sep_rows(nc, D.Lp[l], D.Lc[l], sizeof(cplx_t<R>)) : 0;
+    const bool up = l > 2 && rw == 0;
     if (m2l_cluster_ && m2l_variant_ == 2 && std::is_same<R, float>::value && (nc == 4 || nc == 8 || nc == 16))
         launch_m2l_cls<R>(st, nc, v.ncl8, D.K[l], v.cl8_child, v.cl8_ptr, v.cl8_src, v.cl8_vec, F.TFw[l], F.Sm[l],
                           up ? D.K[l - 1] : 0, v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l],
@@ -587,6 +602,9 @@ void Engine::translate(int w, int l, int nc) {
         launch_m2l_l2l<R>(st, nc, v.n, D.K[l], v.dr_ptr, v.dr_idx, v.dr_vec, F.TFw[l], F.Sm[l], up ? D.K[l - 1] : 0,
                           v.parent, v.oct, D.t_ptr[l], D.t_kp[l], F.t_w[l], F.sh_dn[l], up ? F.Bm[l - 1] : nullptr,
                           F.Bm[l]);
+    if (rw > 0)
+        launch_l2l_sep<R>(st, nc, v.n, D.Lp[l], D.Lc[l], rw, v.parent, v.oct, D.at_ptr[l], D.at_idx[l], F.at_w[l],
+                          D.ap_ptr[l], D.ap_idx[l], F.ap_w[l], F.sh_dn[l], F.Bm[l - 1], F.Bm[l]);
 }
 
 // y += far field (finest Bm of every wavenumber), same right-hand-side range

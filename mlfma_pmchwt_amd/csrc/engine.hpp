@@ -64,6 +64,7 @@ struct FarDev {
     std::vector<cplx_t<R>*> Sm, Bm, TFw;     // per tree level
     std::vector<R*> tw, t_w;             // per child tree level
     std::vector<R*> th_w, ph_w;          // per child tree level: the stencil's weights by direction
+    std::vector<R*> at_w, ap_w;          // per child tree level: the transposed stencils' weights
     std::vector<cplx_t<R>*> sh_up, sh_dn;
 };
 
@@ -71,6 +72,7 @@ struct WaveDev {
     std::vector<int> K;                  // per tree level
     std::vector<int*> tap, t_ptr, t_kp;  // per child tree level
     std::vector<int*> th_tap, ph_tap;    // per child tree level: the stencil's taps by direction
+    std::vector<int*> at_ptr, at_idx, ap_ptr, ap_idx;   // per child tree level: transposed, by direction
     std::vector<int> Lp, Lc;             // per child tree level: theta rows of the parent and child grids
     FarDev<double> f64;                  // always built (true residuals, stage API)
     FarDev<float> f32;                   // built when far_precision = MLFMA_FP32
@@ -370,6 +372,7 @@ private:
     bool disagg_mfma_ = true;   // batched disaggregation on the matrix cores
     bool m2l_cluster_ = true;   // translation by sibling clusters
     bool m2m_separable_ = true;  // interpolation: separable (theta, then phi) form where it applies
+    bool l2l_separable_ = true;  // anterpolation: separable pass after the translation where it applies
     int m2l_variant_ = 0;       // FP32 cluster translation: 1 twelve columns per thread, 2 all-sibling clusters, LDS-staged samples
     double2* d_Dinv_ = nullptr;
     long long* d_Doff_ = nullptr;

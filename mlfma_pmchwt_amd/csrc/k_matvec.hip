@@ -357,6 +357,126 @@ bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const
     return true;
 }
 
+// Anterpolation from the parent level as a separable pass (the transpose of
+// k_m2m_sep): for box b with parent P and octant o, child sample (i, j) gets
+//   sum over (parent row ip, wt) with th_tap[ip][a] = i, (parent column jp, wp)
+//   with ph_tap[jp][b] = j of  wt wp shift_o(ip, jp) P(ip, jp),
+// the same products as the fused gather (Code/ProcessMLFMM.h:152-202) with the
+// sums regrouped (theta first, into LDS), which changes only rounding.  A
+// workgroup takes RW consecutive child rows of one box; lanes run over (row,
+// parent column) pairs for the theta pass and over (row, child column) pairs
+// for the phi pass, which adds into Bm (the translation wrote it before).
+template <typename R, int NC, int CPT, int JJ>
+__global__ __launch_bounds__(64 * NC * 3 / CPT) void k_l2l_sep(int Lp, int Lc, int RW, const int* __restrict__ parent,
+                                                        const int* __restrict__ oct, const int* __restrict__ at_ptr,
+                                                        const int* __restrict__ at_idx, const R* __restrict__ at_w,
+                                                        const int* __restrict__ ap_ptr, const int* __restrict__ ap_idx,
+                                                        const R* __restrict__ ap_w, const cplx_t<R>* __restrict__ shdn,
+                                                        const cplx_t<R>* __restrict__ Bmp, cplx_t<R>* __restrict__ Bm) {
+    using C = cplx_t<R>;
+    extern __shared__ __align__(16) unsigned char l2l_lds[];
+    const int pfn = 2 * Lp, psn = 2 * Lc, Kp = Lp * pfn, Kc = Lc * psn;
+    const int nrg = (Lc + RW - 1) / RW;   // row groups per box
+    const int lin = xcd_remap(blockIdx.x, (int)gridDim.x);
+    const int b = lin / nrg, r0 = (lin % nrg) * RW;   // the row groups of one box are consecutive
+    const int nr = min(RW, Lc - r0);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c0 = wave * CPT;
+    const int us = RW * pfn;                                             // one column's block
+    C* V = reinterpret_cast<C*>(l2l_lds) + (size_t)wave * CPT * us;    // this wavefront's [CPT][RW][2 Lp]
+    const C* P = Bmp + ((size_t)parent[b] * NC * 3 + c0) * Kp;
+    const C* sh = shdn + (size_t)oct[b] * Kp;
+    for (int t = lane; t < nr * pfn; t += 64) {
+        const int r = t / pfn, jp = t - r * pfn;
+        C acc[CPT];
+#pragma unroll
+        for (int d = 0; d < CPT; ++d) acc[d] = C{0, 0};
+        for (int k = at_ptr[r0 + r]; k < at_ptr[r0 + r + 1]; ++k) {
+            const int kp = at_idx[k] * pfn + jp;
+            const R w = at_w[k];
+            C e = sh[kp];
+            e.x *= w;
+            e.y *= w;
+#pragma unroll
+            for (int d = 0; d < CPT; ++d) cfma(acc[d], e, P[(size_t)d * Kp + kp]);
+        }
+#pragma unroll
+        for (int d = 0; d < CPT; ++d) V[d * us + t] = acc[d];
+    }
+    __syncthreads();
+    C* O = Bm + ((size_t)b * NC * 3 + c0) * Kc + (size_t)r0 * psn;
+#pragma unroll
+    for (int jj = 0; jj < JJ; ++jj) {
+        const int t = lane + 64 * jj;
+        if (t < nr * psn) {
+            const int r = t / psn, jc = t - r * psn;
+            C s[CPT];
+#pragma unroll
+            for (int d = 0; d < CPT; ++d) s[d] = C{0, 0};
+            for (int k = ap_ptr[jc]; k < ap_ptr[jc + 1]; ++k) {
+                const int vi = r * pfn + ap_idx[k];
+                const R w = ap_w[k];
+#pragma unroll
+                for (int d = 0; d < CPT; ++d) {
+                    const C v = V[d * us + vi];
+                    s[d].x += w * v.x;
+                    s[d].y += w * v.y;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < CPT; ++d) {
+                C o = O[(size_t)d * Kc + t];
+                o.x += s[d].x;
+                o.y += s[d].y;
+                O[(size_t)d * Kc + t] = o;
+            }
+        }
+    }
+}
+
+// child rows per workgroup: as many as fit 64 KB of LDS (the theta pass's rows
+// of every column) and 256 outputs per lane group (JJ <= 4); 0 if none fits
+int l2l_sep_rows(int ncur, int Lp, int Lc, size_t csize) {
+    const size_t row_lds = (size_t)ncur * 3 * 2 * Lp * csize;
+    if (row_lds > 64 * 1024 || 2 * Lc > 256) return 0;
+    return std::max(1, std::min({Lc, (int)(64 * 1024 / row_lds), 256 / (2 * Lc)}));
+}
+
+template <typename R>
+void launch_l2l_sep(hipStream_t st, int ncur, int nbox, int Lp, int Lc, int rw, const int* parent, const int* oct,
+                    const int* at_ptr, const int* at_idx, const R* at_w, const int* ap_ptr, const int* ap_idx,
+                    const R* ap_w, const cplx_t<R>* shift_dn, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm) {
+    if (nbox == 0) return;
+    const int JJ = (rw * 2 * Lc + 63) / 64;
+    if (rw < 1 || rw > Lc || JJ > 4 || (size_t)ncur * 3 * 2 * Lp * sizeof(cplx_t<R>) * rw > 64 * 1024)
+        throw std::invalid_argument("separable anterpolation: rows per workgroup out of range");
+    const size_t lds = (size_t)ncur * 3 * 2 * Lp * sizeof(cplx_t<R>) * rw;
+    const unsigned grid = (unsigned)nbox * ((Lc + rw - 1) / rw);
+#define L2LS(NC, CPT, J)                                                                                              \
+    k_l2l_sep<R, NC, CPT, J><<<grid, 64 * NC * 3 / CPT, lds, st>>>(Lp, Lc, rw, parent, oct, at_ptr, at_idx, at_w,    \
+                                                                    ap_ptr, ap_idx, ap_w, shift_dn, Bm_parent, Bm)
+#define L2LS_J(NC, CPT)                                                                                               \
+    if (JJ == 1) L2LS(NC, CPT, 1);                                                                                    \
+    else if (JJ == 2) L2LS(NC, CPT, 2);                                                                               \
+    else if (JJ == 3) L2LS(NC, CPT, 3);                                                                               \
+    else L2LS(NC, CPT, 4);
+    if (ncur == 1) {
+        L2LS_J(1, 3)
+    } else if (ncur == 2) {
+        L2LS_J(2, 3)
+    } else if (ncur == 4) {
+        L2LS_J(4, 3)
+    } else if (ncur == 8) {
+        L2LS_J(8, 3)
+    } else if constexpr (sizeof(R) == 4) {
+        L2LS_J(16, 6)
+    } else {
+        throw std::invalid_argument("FP64 far field: at most 8 workspace currents per pass");
+    }
+#undef L2LS_J
+#undef L2LS
+}
+
 // ---------------------------------------------------------------- translation + anterpolation
 // Code/ProcessMLFMM.h:112-147 (m2l, quadrature weight folded into TFw) and
 // :152-202 (l2l) in gather form: each child sample sums the parent samples whose
@@ -1000,7 +1120,10 @@ void launch_narrow(hipStream_t st, size_t n, const double2* in, float2* out) {
                                     const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);        \
     template void launch_m2l_cl<R>(hipStream_t, int, int, int, int, const int*, const int*, const int*, const int*,  \
                                    const cplx_t<R>*, const cplx_t<R>*, int, const int*, const int*, const int*,   \
-                                   const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);
+                                   const int*, const R*, const cplx_t<R>*, const cplx_t<R>*, cplx_t<R>*);       \
+    template void launch_l2l_sep<R>(hipStream_t, int, int, int, int, int, const int*, const int*, const int*,        \
+                                    const int*, const R*, const int*, const int*, const R*, const cplx_t<R>*,     \
+                                    const cplx_t<R>*, cplx_t<R>*);
 MLFMA_FAR_INSTANTIATE(double)
 MLFMA_FAR_INSTANTIATE(float)
 #undef MLFMA_FAR_INSTANTIATE

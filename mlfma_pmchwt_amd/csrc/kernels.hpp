@@ -46,6 +46,18 @@ bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const
                     const int* octant_child, const int* th_tap, const R* th_w, const int* ph_tap, const R* ph_w,
                     const cplx_t<R>* shift_up, const cplx_t<R>* Sm_child, cplx_t<R>* Sm_parent);
 
+// Separable anterpolation, after the translation of a level: for every box of
+// the level, Bm += the parent's local expansion shifted to the box and
+// anterpolated through the transposed direction stencils (theta rows into
+// LDS, then phi).  l2l_sep_rows gives the child rows per workgroup that fit
+// (0 when the pass does not apply; the translation then fuses the
+// anterpolation).
+int l2l_sep_rows(int ncur, int Lp, int Lc, size_t csize);
+template <typename R>
+void launch_l2l_sep(hipStream_t st, int ncur, int nbox, int Lp, int Lc, int rw, const int* parent, const int* oct,
+                    const int* at_ptr, const int* at_idx, const R* at_w, const int* ap_ptr, const int* ap_idx,
+                    const R* ap_w, const cplx_t<R>* shift_dn, const cplx_t<R>* Bm_parent, cplx_t<R>* Bm);
+
 // Bm[b][..][q] = sum_dr TFw[vec][q] * Sm[dr][..][q]
 //              + (parent present) sum_{(kp,w) in T[q]} w * shift_down[oct][kp] * Bm_parent[par][..][kp]
 template <typename R>

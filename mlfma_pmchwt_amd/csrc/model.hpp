@@ -86,6 +86,14 @@ struct InterpTable {
     std::vector<int> t_ptr;      // Kc + 1
     std::vector<int> t_kp;
     std::vector<double> t_w;
+    // the same by direction: for each child theta row the (parent row, weight)
+    // pairs with th_tap[row][a] = child row, ascending parent row, and for each
+    // child phi column the (parent column, weight) pairs with ph_tap (the
+    // weight of a parent sample in a child sample is the product of the two)
+    std::vector<int> at_ptr, at_idx;    // Lc + 1, entries
+    std::vector<double> at_w;
+    std::vector<int> ap_ptr, ap_idx;    // 2 Lc + 1, entries
+    std::vector<double> ap_w;
 };
 
 struct WaveTables {

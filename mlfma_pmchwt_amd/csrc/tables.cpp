@@ -259,6 +259,26 @@ void build_wave_tables(const Octree& t, cplx k, double kabs, int tf_mode, WaveTa
                 it.ph_w[3 * (size_t)j + a] = lp[a];
             }
         }
+        // transposed stencils by direction (CSR, ascending parent row / column)
+        auto transpose = [](int np, int nc, const std::vector<int>& tap, const std::vector<double>& w,
+                            std::vector<int>& ptr, std::vector<int>& idx, std::vector<double>& tw) {
+            ptr.assign(nc + 1, 0);
+            for (int i = 0; i < np; ++i)
+                for (int a = 0; a < 3; ++a) ptr[tap[3 * (size_t)i + a] + 1]++;
+            for (int c = 0; c < nc; ++c) ptr[c + 1] += ptr[c];
+            idx.assign(ptr[nc], 0);
+            tw.assign(ptr[nc], 0.0);
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (int i = 0; i < np; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    const int c = tap[3 * (size_t)i + a];
+                    idx[fill[c]] = i;
+                    tw[fill[c]] = w[3 * (size_t)i + a];
+                    fill[c]++;
+                }
+        };
+        transpose(it.Lp, it.Lc, it.th_tap, it.th_w, it.at_ptr, it.at_idx, it.at_w);
+        transpose(pfn, psn, it.ph_tap, it.ph_w, it.ap_ptr, it.ap_idx, it.ap_w);
         it.t_ptr.assign(it.Kc + 1, 0);
         for (int q = 0; q < it.Kp; ++q)
             for (int s = 0; s < 9; ++s) it.t_ptr[it.tap[(size_t)s * it.Kp + q] + 1]++;
