@@ -114,7 +114,7 @@ def group_roofline(op, R, dom, groups):
     launches = max(g["launches"], 1)
     per_launch_ms = g["ms"] / launches
     if dom == "translate_anterpolate":
-        kern = "k_m2l_cl (sibling-cluster translation + anterpolation, FP32 vector ALUs)"
+        kern = "k_m2l_cl + k_l2l_sep (sibling-cluster translation, then separable anterpolation; FP32 vector ALUs)"
         work, unit, peak, bound = translation_flops(op, R, 2) / launches, "TFLOP/s", FP32_PEAK_TFS, "mfma"
         note = ("compute roofline at the FP32 dense peak; the kernel does its FP32 FMAs on the vector ALUs, where "
                 "packed FP32 instructions are disabled in this build (half that rate); counters: the vector-memory "
