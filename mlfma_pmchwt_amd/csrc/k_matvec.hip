@@ -386,22 +386,26 @@ __global__ __launch_bounds__(64 * NC * 3 / CPT) void k_l2l_sep(int Lp, int Lc, i
     C* V = reinterpret_cast<C*>(l2l_lds) + (size_t)wave * CPT * us;    // this wavefront's [CPT][RW][2 Lp]
     const C* P = Bmp + ((size_t)parent[b] * NC * 3 + c0) * Kp;
     const C* sh = shdn + (size_t)oct[b] * Kp;
-    for (int t = lane; t < nr * pfn; t += 64) {
-        const int r = t / pfn, jp = t - r * pfn;
-        C acc[CPT];
+    // rows one after another, so a row's taps are the same for every lane
+    // (scalar loads) and each parent sample load depends on nothing per lane
+    for (int r = 0; r < nr; ++r) {
+        const int k0 = at_ptr[r0 + r], k1 = at_ptr[r0 + r + 1];
+        for (int jp = lane; jp < pfn; jp += 64) {
+            C acc[CPT];
 #pragma unroll
-        for (int d = 0; d < CPT; ++d) acc[d] = C{0, 0};
-        for (int k = at_ptr[r0 + r]; k < at_ptr[r0 + r + 1]; ++k) {
-            const int kp = at_idx[k] * pfn + jp;
-            const R w = at_w[k];
-            C e = sh[kp];
-            e.x *= w;
-            e.y *= w;
+            for (int d = 0; d < CPT; ++d) acc[d] = C{0, 0};
+            for (int k = k0; k < k1; ++k) {
+                const int kp = at_idx[k] * pfn + jp;
+                const R w = at_w[k];
+                C e = sh[kp];
+                e.x *= w;
+                e.y *= w;
 #pragma unroll
-            for (int d = 0; d < CPT; ++d) cfma(acc[d], e, P[(size_t)d * Kp + kp]);
+                for (int d = 0; d < CPT; ++d) cfma(acc[d], e, P[(size_t)d * Kp + kp]);
+            }
+#pragma unroll
+            for (int d = 0; d < CPT; ++d) V[d * us + r * pfn + jp] = acc[d];
         }
-#pragma unroll
-        for (int d = 0; d < CPT; ++d) V[d * us + t] = acc[d];
     }
     __syncthreads();
     C* O = Bm + ((size_t)b * NC * 3 + c0) * Kc + (size_t)r0 * psn;
