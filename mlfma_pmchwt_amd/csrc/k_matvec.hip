@@ -266,27 +266,37 @@ __global__ __launch_bounds__(64 * NC * 3 / CPT) void k_m2m_sep(int Lp, int Lc, i
     for (int jj = 0; jj < JJ; ++jj)
 #pragma unroll
         for (int d = 0; d < CPT; ++d) acc[jj][d] = C{0, 0};
+    // theta-pass items of this thread (t = lane + 64 k < RW 2 Lc <= 192 with the
+    // launch's LDS bound): child offsets and weights, the same for every child
+    constexpr int KT = 3;
+    int ti[KT][3];
+    R lt[KT][3];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        const int t = min(lane + 64 * k, nr * psn - 1);
+        const int r = t / psn, m = t - r * psn;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ti[k][a] = th_tap[3 * (i0 + r) + a] * psn + m;
+            lt[k][a] = th_w[3 * (i0 + r) + a];
+        }
+    }
     for (int s = son_ptr[p]; s < son_ptr[p + 1]; ++s) {
         const int c = son_idx[s];
         const C* S = Smc + ((size_t)c * NC * 3 + c0) * Kc;
-        for (int t = lane; t < nr * psn; t += 64) {
-            const int r = t / psn, m = t - r * psn;
-            const int i = i0 + r;
-            int ti[3];
-            R lt[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                ti[a] = th_tap[3 * i + a] * psn + m;
-                lt[a] = th_w[3 * i + a];
-            }
+        for (int k = 0; k < KT; ++k) {
+            const int t = lane + 64 * k;
+            if (t < nr * psn) {
 #pragma unroll
-            for (int d = 0; d < CPT; ++d) {
-                const C* sd = S + (size_t)d * Kc;
-                const C f0 = sd[ti[0]], f1 = sd[ti[1]], f2 = sd[ti[2]];
-                C u;
-                u.x = f0.x * lt[0] + f1.x * lt[1] + f2.x * lt[2];
-                u.y = f0.y * lt[0] + f1.y * lt[1] + f2.y * lt[2];
-                U[d * us + t] = u;
+                for (int d = 0; d < CPT; ++d) {
+                    const C* sd = S + (size_t)d * Kc;
+                    const C f0 = sd[ti[k][0]], f1 = sd[ti[k][1]], f2 = sd[ti[k][2]];
+                    C u;
+                    u.x = f0.x * lt[k][0] + f1.x * lt[k][1] + f2.x * lt[k][2];
+                    u.y = f0.y * lt[k][0] + f1.y * lt[k][1] + f2.y * lt[k][2];
+                    U[d * us + t] = u;
+                }
             }
         }
         __syncthreads();
@@ -326,7 +336,8 @@ bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const
     if (nparent == 0) return true;
     const size_t row_lds = (size_t)ncur * 3 * 2 * Lc * sizeof(cplx_t<R>);   // one row of all column groups
     if (2 * Lp > 256 || row_lds > 64 * 1024) return false;
-    const int RW = std::max(1, std::min({Lp, (int)(64 * 1024 / row_lds), 256 / (2 * Lp)}));
+    if (2 * Lc > 192) return false;   // theta-pass items per lane (KT = 3)
+    const int RW = std::max(1, std::min({Lp, (int)(64 * 1024 / row_lds), 256 / (2 * Lp), 192 / (2 * Lc)}));
     const int JJ = (RW * 2 * Lp + 63) / 64;
     const size_t lds = row_lds * RW;
     const unsigned grid = (unsigned)nparent * ((Lp + RW - 1) / RW);
