@@ -374,9 +374,9 @@ bool launch_m2m_sep(hipStream_t st, int ncur, int nparent, int Lp, int Lc, const
 //   with ph_tap[jp][b] = j of  wt wp shift_o(ip, jp) P(ip, jp),
 // the same products as the fused gather (Code/ProcessMLFMM.h:152-202) with the
 // sums regrouped (theta first, into LDS), which changes only rounding.  A
-// workgroup takes RW consecutive child rows of one box; lanes run over (row,
-// parent column) pairs for the theta pass and over (row, child column) pairs
-// for the phi pass, which adds into Bm (the translation wrote it before).
+// workgroup takes RW consecutive child rows of one box; the theta pass goes
+// row by row with lanes over parent columns, the phi pass has lanes over
+// (row, child column) pairs and adds into Bm (the translation wrote it before).
 template <typename R, int NC, int CPT, int JJ>
 __global__ __launch_bounds__(64 * NC * 3 / CPT) void k_l2l_sep(int Lp, int Lc, int RW, const int* __restrict__ parent,
                                                         const int* __restrict__ oct, const int* __restrict__ at_ptr,
