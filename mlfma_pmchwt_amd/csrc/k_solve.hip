@@ -149,33 +149,48 @@ __device__ __forceinline__ void fold_rows(double2 v, int R, double2* red) {
 }  // namespace
 
 // partial[(j * nblk + b) * R + r] = sum over block b of conj(V_j[u][r]) w[u][r]
-// (V in FP64 or, for FP32-mode solves, FP32: widened exactly before the FP64 sums)
+// (V in FP64 or, for FP32-mode solves, FP32: widened exactly before the FP64
+// sums).  A block takes kDotJ basis vectors at once, so w is read once per
+// kDotJ vectors; each vector's sum runs over the same rows in the same order
+// whichever vectors share its block.
+constexpr int kDotJ = 4;
 template <typename TV>
-__global__ __launch_bounds__(256) void k_multidot_b(int n, int R, const TV* __restrict__ V, size_t ldv,
+__global__ __launch_bounds__(256) void k_multidot_b(int n, int R, int m, const TV* __restrict__ V, size_t ldv,
                                                     const double2* __restrict__ w, double2* __restrict__ partial) {
     __shared__ double2 red[256];
-    const int j = blockIdx.y;
+    const int j0 = blockIdx.y * kDotJ;
     const int rows = blockDim.x / R;
     const int r = threadIdx.x % R, q = threadIdx.x / R;
-    const TV* v = V + (size_t)j * ldv;
-    double2 s = make_double2(0, 0);
+    const TV* v[kDotJ];
+#pragma unroll
+    for (int jj = 0; jj < kDotJ; ++jj) v[jj] = V + (size_t)min(j0 + jj, m - 1) * ldv;   // past m: read, not used
+    double2 s[kDotJ];
+#pragma unroll
+    for (int jj = 0; jj < kDotJ; ++jj) s[jj] = make_double2(0, 0);
     if (q < rows)
         for (int u = blockIdx.x * rows + q; u < n; u += gridDim.x * rows) {
             const size_t e = (size_t)u * R + r;
-            s = cadd(s, cmulc(widen(v[e]), w[e]));
+            const double2 we = w[e];
+#pragma unroll
+            for (int jj = 0; jj < kDotJ; ++jj) s[jj] = cadd(s[jj], cmulc(widen(v[jj][e]), we));
         }
-    fold_rows(s, R, red);
-    if ((int)threadIdx.x < R) partial[((size_t)j * gridDim.x + blockIdx.x) * R + threadIdx.x] = red[threadIdx.x];
+#pragma unroll
+    for (int jj = 0; jj < kDotJ; ++jj) {
+        if (j0 + jj >= m) break;   // uniform over the block
+        fold_rows(s[jj], R, red);
+        if ((int)threadIdx.x < R)
+            partial[((size_t)(j0 + jj) * gridDim.x + blockIdx.x) * R + threadIdx.x] = red[threadIdx.x];
+    }
 }
 
 void launch_multidot_b(hipStream_t st, int n, int R, int m, const double2* V, size_t ldv, const double2* w,
                        double2* partial, int nblk) {
-    k_multidot_b<double2><<<dim3(nblk, m), 256, 0, st>>>(n, R, V, ldv, w, partial);
+    k_multidot_b<double2><<<dim3(nblk, (m + kDotJ - 1) / kDotJ), 256, 0, st>>>(n, R, m, V, ldv, w, partial);
 }
 
 void launch_multidot_b(hipStream_t st, int n, int R, int m, const float2* V, size_t ldv, const double2* w,
                        double2* partial, int nblk) {
-    k_multidot_b<float2><<<dim3(nblk, m), 256, 0, st>>>(n, R, V, ldv, w, partial);
+    k_multidot_b<float2><<<dim3(nblk, (m + kDotJ - 1) / kDotJ), 256, 0, st>>>(n, R, m, V, ldv, w, partial);
 }
 
 // out[j * R + r] = sum_b partial[(j * nblk + b) * R + r], b ascending
